@@ -1,0 +1,269 @@
+/* mpet_hip.h — C ABI of libmpet_hip.so, the MI355X (gfx950) backend of the
+ * total-pressure MPET time step.
+ *
+ * The reference (meg-simula/waterscapes, package `mpet`) has no FFI: its
+ * boundary is the Python class API of MPETTotalPressureSolver, whose numerics
+ * run inside DOLFIN / PETSc / hypre.  Each entry point below replaces one of
+ * those calls; the reference call site is cited on each declaration.  The
+ * Python package waterscapes_amd binds these symbols with ctypes.
+ *
+ * Conventions
+ *  - All array pointers are DEVICE pointers unless marked (host).  Arrays
+ *    passed in stay owned by the caller; handles own what they allocate and
+ *    free it in their *_destroy call.
+ *  - Return value: 0 on success, otherwise an MpetStatus code; the message is
+ *    available from mpet_last_error (thread-local).
+ *  - Work is enqueued on the given stream; nothing synchronises unless the
+ *    entry point says so.  One host thread per handle.
+ *  - Canonical DOF numbering: u DOF = 3*a + j (P2 node a, component j);
+ *    p_f DOF = 3*nP2 + (A+1)*c + f (P1 node c, field f = 0..A, f = 0 is the
+ *    total pressure).
+ */
+#ifndef MPET_HIP_H
+#define MPET_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPET_MAXF 8 /* at most 7 networks: fields p_0..p_7 */
+
+typedef enum {
+  MPET_OK = 0,
+  MPET_ERR_INVALID = 1,
+  MPET_ERR_HIP = 2,
+  MPET_ERR_NOT_CONVERGED = 3,
+  MPET_ERR_NAN = 4,
+  MPET_ERR_OOM = 5,
+  MPET_ERR_BUDGET = 6
+} MpetStatus;
+
+/* Mesh of one instance.  cells are the P1 nodes (vertices, ascending per
+ * cell); cell_p2 the 10 P2 nodes per cell in UFC order.  Cells are listed
+ * by colour (colour_cells[colour_offsets[k] .. colour_offsets[k+1]]) such
+ * that cells of one colour share no vertex.  colour_offsets is a HOST array. */
+typedef struct {
+  int32_t nV, nC, nP2, nP1;
+  const double* coords;          /* nV*3 */
+  const int32_t* cells;          /* nC*4 */
+  const int32_t* cell_p2;        /* nC*10 */
+  const int32_t* colour_cells;   /* nC */
+  const int64_t* colour_offsets; /* (host) ncolours+1 */
+  int32_t ncolours;
+} MpetMeshDesc;
+
+/* Node-pair sparsity patterns (CSR, sorted columns): rows/cols are P2 or P1
+ * nodes: 22 = P2xP2, 21 = P2xP1, 12 = P1xP2, 11 = P1xP1. */
+typedef struct {
+  const int32_t *rp22, *ci22;
+  const int32_t *rp21, *ci21;
+  const int32_t *rp12, *ci12;
+  const int32_t *rp11, *ci11;
+  int32_t nnz22, nnz21, nnz12, nnz11;
+} MpetPatterns;
+
+/* Primitive operator values on the node patterns (FP64):
+ *   Kuu  nnz22*9 : 2 mu eps(phi_b e_j):eps(phi_a e_i)  (row-major 3x3 blocks)
+ *   Lap  nnz22   : grad phi_a . grad phi_b (unscaled)
+ *   Dv   nnz21*3 : int psi_c d_i phi_a   (row P2 node a, col P1 node c)
+ *   Dt   nnz12*3 : the same values on the transposed pattern
+ *   M1   nnz11   : P1 mass
+ *   St   A*nnz11 : int K_i grad psi . grad psi, network-major */
+typedef struct {
+  double *Kuu, *Lap, *Dv, *Dt, *M1, *St;
+} MpetPrimitives;
+
+/* (A+1)x(A+1) pressure coupling (row-major, stride MPET_MAXF) multiplying
+ * the P1 mass, plus per-field stiffness weights (field f uses St[f-1]). */
+typedef struct {
+  double C[MPET_MAXF * MPET_MAXF];
+  double stiff[MPET_MAXF];
+  double mu;
+} MpetCoef;
+
+/* Structured total-pressure operator (the matrix of lhs(F), or its FP32
+ * copy), values structure-of-arrays on the node patterns:
+ *   Kuu[c*nnz22 + k] (c = 3i+j), Dv[c*nnz21 + k], Dt[c*nnz12 + k],
+ *   M1[k], St[i*nnz11 + k].  dtype 0 = float64, 1 = float32.
+ * Rows with dmask[dof] != 0 (Dirichlet DOFs) are returned as 0. */
+typedef struct {
+  int32_t nP2, nP1, A;
+  MpetPatterns pat;
+  int32_t dtype;
+  const void *Kuu, *Dv, *Dt, *M1, *St;
+  MpetCoef coef;
+  const uint8_t* dmask;
+} MpetOpDesc;
+
+/* One level of the displacement-block hierarchy (device arrays, int32
+ * CSR).  Values / dinv in the precision of the owning MpetPrecDesc. */
+typedef struct {
+  int32_t n, nc;                 /* rows, coarse columns (0 on the coarsest) */
+  const int32_t *A_rp, *A_ci;
+  const void *A_v, *dinv;
+  double rho;                    /* spectral radius estimate of D^-1 A */
+  const int32_t *P_rp, *P_ci;    /* prolongator n x nc */
+  const void* P_v;
+  const int32_t *R_rp, *R_ci;    /* restriction nc x n (= P^T) */
+  const void* R_v;
+  int32_t lpr, lprP, lprR;       /* lanes per row of the three SpMMs */
+} MpetAmgLevelDesc;
+
+#define MPET_MAXLEV 16
+
+/* Block-diagonal preconditioner of prec (:279-283) with Dirichlet rows and
+ * columns eliminated (apply_symmetric(bc, P), :445-446). */
+typedef struct {
+  int32_t nlev;
+  MpetAmgLevelDesc lev[MPET_MAXLEV];
+  const void* coarse_inv;        /* dense n_coarse^2 */
+  int32_t degree;                /* Chebyshev degree of the smoother */
+  double eig_ratio, eig_safety;  /* interval [safety*rho/ratio, safety*rho] */
+  double inv_mu;                 /* the hierarchy is built on grad:grad */
+  const uint8_t* mask0;          /* level-0 rows (P2 nodes) that are Dirichlet */
+  /* pressure blocks: field f matrix = Pd[f] M1 + Ps[f] St[f-1] */
+  int32_t nP1, F;
+  const int32_t *rp11, *ci11;
+  long long nnz11;
+  const void *M1, *St, *dinvp;   /* dinvp: F*nP1 inverse diagonal */
+  const uint8_t* pmask;          /* F*nP1 Dirichlet flags */
+  double Pd[MPET_MAXF], Ps[MPET_MAXF];
+  int32_t pdeg;
+  double plmin[MPET_MAXF], plmax[MPET_MAXF];
+} MpetPrecDesc;
+
+typedef struct {
+  long long nrows;
+  const long long* rp;
+  const int32_t* ci;
+  const double* v;
+  int32_t lpr;
+} MpetCsrDesc;
+
+typedef struct {
+  int32_t mode;         /* 0: FP64 MINRES; 1: FP32 MINRES in FP64 refinement */
+  int32_t use_csr;      /* mode 0: use the scalar CSR A instead of the structured one */
+  double rtol;          /* mode 0: relative preconditioned residual */
+  int32_t maxit;        /* per MINRES solve */
+  int32_t check_every;  /* host reads the device state every k iterations */
+  int32_t use_graphs;   /* replay k-iteration chunks from a captured hipGraph */
+  double outer_tol;     /* mode 1: ||Pi(b - A x)|| <= outer_tol * bnorm */
+  double inner_rtol;    /* mode 1: FP32 MINRES relative tolerance */
+  int32_t max_outer;
+} MpetSolverOptions;
+
+typedef struct {
+  int32_t inner_iterations, outer_iterations, converged;
+  double residual_norm;
+} MpetSolveStats;
+
+/* Error message of the last failing call on this thread. */
+int mpet_last_error(char* buf, size_t n);
+
+/* Upload the reference-element tables (host, MPET_TAB_SIZE doubles) to the
+ * device constant bank of the current device. */
+int mpet_upload_tables(const double* tab_host, int n);
+
+/* Element tensors of lhs(F) (mpet/mpettotalpressuresolver.py:267-274) and of
+ * prec (:279-283), scattered into the primitive arrays in colour order:
+ * replaces `A = assemble(a)` (:437) and `P = assemble(prec)` (:444).
+ * Kcell: nC*A per-cell K_i (DG0, cell-major).  Deterministic. */
+int mpet_assemble_primitives(const MpetMeshDesc* mesh, const MpetPatterns* pat,
+                             const double* Kcell, int A, double mu,
+                             const MpetPrimitives* out, hipStream_t stream);
+
+/* Scalar CSR (int64 rowptr computed by the caller) of the un-eliminated A
+ * (which = 0) or P (which = 1) in canonical numbering — the PETSc AIJ matrix
+ * that `assemble(a)` / `assemble(prec)` produce (:437, :444). */
+int mpet_expand_csr(const MpetPatterns* pat, const MpetPrimitives* prim, int nP2, int nP1,
+                    int A, const MpetCoef* coef, int which, const long long* rowptr,
+                    int* cols, double* vals, hipStream_t stream);
+
+/* Pressure rows of b = rhs(F) at t_n (linear in up_): replaces
+ * `b = assemble(L)` (:461); zeroes the u rows.  coef.C = rhs coupling,
+ * coef.stiff[f] = dt*(1-theta). */
+int mpet_rhs_pressure(const MpetPatterns* pat, const MpetPrimitives* prim, int nP2, int nP1,
+                      int A, const MpetCoef* coef, const double* up_prev, double* b,
+                      hipStream_t stream);
+
+/* Volume source b += scale * int f_h . v for f interpolated into P_k on each
+ * cell (fval: nC*nk*ncomp nodal values, Q: ntest*nk reference weights):
+ * replaces `assemble(L1_i)` volume part (dt g_i w_i, :322, :471) and
+ * `assemble(L0)` volume part (f . v, :313, :482). */
+int mpet_cell_source(const MpetMeshDesc* mesh, const int* test_nodes, int ntest,
+                     const double* Q, int nk, const double* fval, int ncomp, double scale,
+                     long long dof_base, int dof_stride, int dof_off, double* b,
+                     hipStream_t stream);
+
+/* y = A x (b == NULL) or y = b - A x, Dirichlet rows zeroed: the MatMult
+ * of PETSc's KSP on the symmetrically eliminated Acopy (:457, :489, :493)
+ * for iterates with zero Dirichlet components; with b it is the lifting
+ * b_I - A_ID g (apply_symmetric, bc_symmetric.py:11-18) and the outer
+ * FP64 residual of iterative refinement.  x, b, y of the op's dtype. */
+int mpet_op_apply(const MpetOpDesc* op, const void* x, const void* b, void* y,
+                  hipStream_t stream);
+
+/* Scalar CSR SpMV (PETSc MatMult on AIJ, :493), int64 rowptr, lpr lanes
+ * per row in {4,8,16,32,64}; optional residual b and row mask as above. */
+int mpet_csr_spmv(int dtype, long long nrows, const long long* rp, const int* ci,
+                  const void* val, const void* x, const void* b, const uint8_t* dmask,
+                  void* y, int lpr, hipStream_t stream);
+
+/* FP64 AoS primitive (ncomp per entry) -> SoA copy in float64/float32. */
+int mpet_soa_convert(int dtype, const double* src, int ncomp, long long n, void* dst,
+                     hipStream_t stream);
+
+/* Smoothed-aggregation setup (host arrays, FP64) of the displacement-block
+ * hierarchy: replaces hypre BoomerAMG's setup inside
+ * PETScKrylovSolver("minres", "hypre_amg") (:448).  P0 optional (n1 > 0):
+ * prescribed first prolongator.  The hierarchy is read back with
+ * mpet_amg_level_sizes / mpet_amg_level_copy / mpet_amg_coarse_inverse. */
+typedef struct {
+  double strength;
+  int max_coarse, max_levels, power_iters;
+  double omega_scale;
+} MpetAmgParams;
+int mpet_amg_setup(int n, const int64_t* rp, const int* ci, const double* v,
+                   const MpetAmgParams* prm, int n1, const int64_t* p0_rp,
+                   const int* p0_ci, const double* p0_v, void** handle);
+int mpet_amg_num_levels(void* h);
+int mpet_amg_level_sizes(void* h, int l, int64_t* out);
+int mpet_amg_level_copy(void* h, int l, int64_t* A_rp, int* A_ci, double* A_v, double* dinv,
+                        double* rho, int64_t* P_rp, int* P_ci, double* P_v, int64_t* R_rp,
+                        int* R_ci, double* R_v);
+int mpet_amg_coarse_inverse(void* h, double* out);
+void mpet_amg_destroy(void* h);
+
+/* Krylov solver handle: MINRES + block preconditioner on the structured
+ * (or CSR) operator — PETScKrylovSolver("minres","hypre_amg") (:448).
+ * Borrows every device array of the descriptors (caller keeps them alive);
+ * owns its workspaces.  op32/prec32 are needed for mode 1, prec64 for
+ * mode 0. */
+int mpet_solver_create(const MpetOpDesc* op64, const MpetOpDesc* op32,
+                       const MpetCsrDesc* csr64, const MpetPrecDesc* prec64,
+                       const MpetPrecDesc* prec32, const MpetSolverOptions* opt,
+                       void** handle);
+int mpet_solver_destroy(void* h);
+
+/* One time-step solve: solver.set_operators(Acopy, P); solver.solve(x, b)
+ * (:492-493) with the symmetric Dirichlet elimination of :487-489 folded in.
+ * b: full right-hand side (FP64, un-lifted); x: in = initial guess with the
+ * Dirichlet values in place, out = solution.  Synchronises the stream.
+ * Returns MPET_ERR_NOT_CONVERGED (stats filled) when maxit is reached. */
+int mpet_solver_solve(void* h, const double* b, double* x, MpetSolveStats* stats,
+                      hipStream_t stream);
+
+/* Reference norm for the refinement test (||b_elim||); 0 = use ||r_0||. */
+int mpet_solver_set_bnorm(void* h, double bnorm);
+
+/* z = P^-1 r with the FP64 (dtype 0) or FP32 (1) preconditioner. */
+int mpet_precond_apply(void* h, int dtype, const void* r, void* z, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPET_HIP_H */

@@ -1,0 +1,807 @@
+"""FP64 CPU oracle for the total-pressure MPET time step.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product (waterscapes_amd/) may
+import this module; only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use it, as the checker.
+
+It restates, with numpy/scipy and Gauss quadrature, the forms and the time /
+boundary-condition logic of the reference:
+
+  * bilinear form F, preconditioner prec, right-hand sides L0, L1, L2
+    .................................. mpet/mpettotalpressuresolver.py:239-324
+  * Dirichlet DOFs (closure of facets marked 0, u on sub(0), p_i on
+    sub(i+2), no BC on p_0) ............ mpet/mpettotalpressuresolver.py:123-144
+  * row replacement (DirichletBC.apply) for the direct path
+    .................................. mpet/mpettotalpressuresolver.py:361-362,399-400
+  * symmetric elimination (MatZeroRowsColumns, diag 1, lifting into b)
+    .................................. mpet/bc_symmetric.py:6-22, called at :445-446,487-489
+  * time loop t_theta = t + theta*dt, t_{n+1} = t_theta + (1-theta)*dt,
+    while t < T - 1e-9 ................ mpet/mpettotalpressuresolver.py:369-411,456-502
+  * direct LU (MUMPS in the reference; SuperLU here) .................... :365,403
+  * preconditioned MINRES with the block-diagonal preconditioner ......... :448,493
+
+Canonical numbering (inputs to every function): u DOF = 3*a + j for P2 node a,
+component j; p_f DOF = 3*nP2 + (A+1)*c + f for P1 node c, field f = 0..A.
+
+Pinning (see tests/test_oracle.py): element tensors against sympy-exact
+fixtures (tests/golden/), P1 closed forms, kernels (rigid motions, constants),
+symmetric-elimination == row-replacement solutions, 3D manufactured-solution
+convergence rates at the reference's thresholds, and the uniform-pressure
+known answer p_0 = -sum_j alpha_j P_j.
+"""
+from __future__ import annotations
+
+import math
+from itertools import product
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+from scipy.special import roots_jacobi
+
+EDGES = ((2, 3), (1, 3), (1, 2), (0, 3), (0, 2), (0, 1))
+FACES = ((1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2))
+
+# ---------------------------------------------------------------------------
+# quadrature and Lagrange bases (own implementation: Vandermonde inversion)
+# ---------------------------------------------------------------------------
+
+
+def tet_rule(order):
+    """Stroud conical-product rule on the unit tetrahedron (volume 1/6)."""
+    n = order // 2 + 1
+    a0, w0 = roots_jacobi(n, 0, 0)
+    a1, w1 = roots_jacobi(n, 1, 0)
+    a2, w2 = roots_jacobi(n, 2, 0)
+    P, W = [], []
+    for i, j, k in product(range(n), range(n), range(n)):
+        z = (1 + a2[k]) / 2
+        y = (1 + a1[j]) / 2 * (1 - z)
+        x = (1 + a0[i]) / 2 * (1 - y - z)
+        P.append((x, y, z))
+        W.append(w0[i] * w1[j] * w2[k] / 64.0)
+    return np.array(P), np.array(W)
+
+
+def tri_rule(order):
+    """Collapsed Gauss rule on the unit triangle (area 1/2)."""
+    n = order // 2 + 1
+    a0, w0 = roots_jacobi(n, 0, 0)
+    a1, w1 = roots_jacobi(n, 1, 0)
+    P, W = [], []
+    for i, j in product(range(n), range(n)):
+        y = (1 + a1[j]) / 2
+        x = (1 + a0[i]) / 2 * (1 - y)
+        P.append((x, y))
+        W.append(w0[i] * w1[j] / 8.0)
+    return np.array(P), np.array(W)
+
+
+def _monomials(k, d):
+    return [e for e in product(range(k + 1), repeat=d) if sum(e) <= k]
+
+
+def lagrange_nodes(k, d=3):
+    """P_k nodes in reference coordinates: barycentric multi-indices in the
+    order vertices first, then lexicographic (same order as the product's
+    Expression interpolation)."""
+    n = d + 1
+    if k == 0:
+        return np.full((1, d), 1.0 / n)
+    allidx = [a for a in product(range(k + 1), repeat=n) if sum(a) == k]
+    vert = [tuple(k if s == i else 0 for s in range(n)) for i in range(n)]
+    idx = vert + sorted(a for a in allidx if a not in vert)
+    bary = np.array(idx, dtype=float) / k
+    return bary[:, 1:]                       # reference coords = lambda_1..d
+
+
+def lagrange_eval(k, pts, d=3):
+    """Values (npts, nbasis) of the P_k Lagrange basis at reference points."""
+    nodes = lagrange_nodes(k, d)
+    mons = _monomials(k, d)
+    V = np.array([[np.prod(p ** np.array(e)) for e in mons] for p in nodes])
+    Pm = np.array([[np.prod(p ** np.array(e)) for e in mons] for p in pts])
+    return Pm @ np.linalg.inv(V)
+
+
+def p2_ref_values(pts):
+    x, y, z = pts[:, 0], pts[:, 1], pts[:, 2]
+    lam = np.stack([1 - x - y - z, x, y, z], axis=1)
+    vals = [lam[:, i] * (2 * lam[:, i] - 1) for i in range(4)]
+    vals += [4 * lam[:, i] * lam[:, j] for (i, j) in EDGES]
+    return np.stack(vals, axis=1)
+
+
+def p2_ref_grads(pts):
+    """(npts, 10, 3) reference gradients of the P2 basis."""
+    x, y, z = pts[:, 0], pts[:, 1], pts[:, 2]
+    lam = np.stack([1 - x - y - z, x, y, z], axis=1)
+    dlam = np.array([[-1, -1, -1], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+    g = [(4 * lam[:, i, None] - 1) * dlam[i] for i in range(4)]
+    g += [4 * (lam[:, i, None] * dlam[j] + lam[:, j, None] * dlam[i]) for (i, j) in EDGES]
+    return np.stack(g, axis=1)
+
+
+def p1_ref_values(pts):
+    x, y, z = pts[:, 0], pts[:, 1], pts[:, 2]
+    return np.stack([1 - x - y - z, x, y, z], axis=1)
+
+
+P1_REF_GRADS = np.array([[-1, -1, -1], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+
+# ---------------------------------------------------------------------------
+# geometry
+# ---------------------------------------------------------------------------
+
+
+def cell_geometry(coords, cells):
+    """Jacobian J (nC,3,3) with columns v_k - v_0, |det J|, J^{-1}."""
+    v = coords[cells]                                     # nC,4,3
+    J = np.stack([v[:, 1] - v[:, 0], v[:, 2] - v[:, 0], v[:, 3] - v[:, 0]], axis=2)
+    det = np.linalg.det(J)
+    Jinv = np.linalg.inv(J)
+    return J, np.abs(det), Jinv
+
+
+# ---------------------------------------------------------------------------
+# element tensors
+# ---------------------------------------------------------------------------
+
+
+def element_tensors(coords, cells):
+    """Per-cell FP64 element tensors by degree-2 quadrature (exact for the
+    P2/P1 forms on affine cells).
+
+    Returns dict with
+      'E'   (nC,30,30)  int eps(phi):eps(psi) (unscaled; times 2 mu in F),
+                        local DOF 3*a + j;
+      'G'   (nC,10,10)  int grad phi_a . grad phi_b  (P2 Laplacian);
+      'B'   (nC,30,4)   int psi_c div(phi_a e_j)   (row 3a+j, col c);
+      'M'   (nC,4,4)    P1 mass;
+      'S'   (nC,4,4)    P1 stiffness;
+    """
+    _, det, Jinv = cell_geometry(coords, cells)
+    qp, qw = tet_rule(2)
+    rg = p2_ref_grads(qp)                                  # nq,10,3
+    # physical gradient: grad phi = J^{-T} ref_grad
+    pg = np.einsum("ckm,qak->cqam", Jinv, rg)              # nC,nq,10,3
+    wdet = det[:, None] * qw[None, :]                      # nC,nq
+    nC = cells.shape[0]
+    # L2 product of gradients: Gm[c,a,b,m,n] = int d_m phi_a d_n phi_b
+    Gm = np.einsum("cq,cqam,cqbn->cabmn", wdet, pg, pg)
+    G = np.einsum("cabmm->cab", Gm)
+    # eps(phi_a e_i):eps(phi_b e_j) = 1/2 (delta_ij grad.grad + d_j phi_a d_i phi_b)
+    E = np.zeros((nC, 10, 3, 10, 3))
+    for i in range(3):
+        for j in range(3):
+            E[:, :, i, :, j] = 0.5 * ((i == j) * G + Gm[:, :, :, j, i])
+    E = E.reshape(nC, 30, 30)
+    psi = p1_ref_values(qp)                                # nq,4
+    Bt = np.einsum("cq,cqaj,qd->cajd", wdet, pg, psi)      # nC,10,3,4
+    B = Bt.reshape(nC, 30, 4)
+    M = np.einsum("cq,qa,qb->cab", wdet, psi, psi)
+    g1 = np.einsum("ckm,ak->cam", Jinv, P1_REF_GRADS)      # nC,4,3
+    S = (det / 6.0)[:, None, None] * np.einsum("cam,cbm->cab", g1, g1)
+    return {"E": E, "G": G, "B": B, "M": M, "S": S, "det": det}
+
+
+# ---------------------------------------------------------------------------
+# parameters
+# ---------------------------------------------------------------------------
+
+
+def lame(E, nu):
+    """mu, lambda (mpet/mpetproblem.py:13-16, mpettotalpressuresolver.py:262-263)."""
+    mu = E / (2.0 * (1.0 + nu))
+    lmbda = nu * E / ((1.0 - 2.0 * nu) * (1.0 + nu))
+    return mu, lmbda
+
+
+def pressure_coefficients(alpha, c, S, lmbda, dt, theta):
+    """(A+1)x(A+1) coefficient matrices multiplying the P1 mass in the
+    pressure rows: Cm for the matrix (lhs of F) and Bm for the up_ part of
+    the right-hand side (rhs of F), mpettotalpressuresolver.py:269-274."""
+    A = len(alpha)
+    al = np.asarray(alpha, float)
+    Cm = np.zeros((A + 1, A + 1))
+    Bm = np.zeros((A + 1, A + 1))
+    Cm[0, 0] = -1.0 / lmbda
+    Cm[0, 1:] = -al / lmbda
+    Cm[1:, 0] = -al / lmbda
+    Bm[1:, 0] = -al / lmbda
+    for i in range(A):
+        for j in range(A):
+            base = -(c[i] * (i == j) + al[i] * al[j] / lmbda)
+            if i == j:
+                tr = sum(S[i][k] for k in range(A) if k != i)
+                Cm[i + 1, j + 1] = base - dt * theta * tr
+                Bm[i + 1, j + 1] = base + dt * (1 - theta) * tr
+            else:
+                Cm[i + 1, j + 1] = base + dt * theta * S[i][j]
+                Bm[i + 1, j + 1] = base - dt * (1 - theta) * S[i][j]
+    return Cm, Bm
+
+
+# ---------------------------------------------------------------------------
+# global assembly
+# ---------------------------------------------------------------------------
+
+
+class Layout:
+    def __init__(self, nP2, nP1, A):
+        self.nP2, self.nP1, self.A = nP2, nP1, A
+        self.nu = 3 * nP2
+        self.n = self.nu + (A + 1) * nP1
+
+    def udofs(self, cell_p2):
+        """(nC,30) local u DOFs 3a+j."""
+        return (3 * cell_p2[:, :, None] + np.arange(3)).reshape(cell_p2.shape[0], 30)
+
+    def pdofs(self, cell_p1, f):
+        return self.nu + (self.A + 1) * cell_p1 + f
+
+
+def _coo(rows, cols, vals, n):
+    return sp.coo_matrix((vals.reshape(-1), (rows.reshape(-1), cols.reshape(-1))),
+                         shape=(n, n))
+
+
+def _blk(ld, rd, val):
+    """Broadcast local row dofs (nC,r) / col dofs (nC,c) against val (nC,r,c)."""
+    R = np.broadcast_to(ld[:, :, None], val.shape)
+    Cc = np.broadcast_to(rd[:, None, :], val.shape)
+    return R, Cc
+
+
+def assemble_matrix(coords, cells, cell_p2, cell_p1, nP2, nP1, params,
+                    dt, theta, K_cells=None, robin=None, which="A"):
+    """Global FP64 matrix in CSR (duplicates summed, sorted indices).
+
+    which == "A": lhs(F) + sum_i lhs(L2_i) (mpettotalpressuresolver.py:267-274,
+    324; assembled at :437-441).  which == "P": the preconditioner form prec
+    (:279-283), no boundary conditions.  Pattern: every block the form couples
+    (u-u, u-p0, p0-u, all p-p pairs); u-p_i (i >= 1) is structurally empty.
+
+    K_cells: (nC, A) per-cell K_i (DG0) or None to use params['K'].
+    robin: optional list over networks of (facet_vertex_triples, beta_values
+           at facet nodes) – see robin_facet_mass.
+    """
+    A = int(params["J"])
+    mu, lmbda = lame(params["E"], params["nu"])
+    alpha, c, S = params["alpha"], params["c"], params["S"]
+    lay = Layout(nP2, nP1, A)
+    et = element_tensors(coords, cells)
+    nC = cells.shape[0]
+    if K_cells is None:
+        K_cells = np.tile(np.asarray(params["K"], float), (nC, 1))
+    ud = lay.udofs(cell_p2)
+    blocks = []
+    if which == "A":
+        val = 2 * mu * et["E"]
+        blocks.append((ud, ud, val))
+        pd0 = lay.pdofs(cell_p1, 0)
+        blocks.append((ud, pd0, et["B"]))                          # p0 div v
+        blocks.append((pd0, ud, np.transpose(et["B"], (0, 2, 1))))  # div u w0
+        Cm, _ = pressure_coefficients(alpha, c, S, lmbda, dt, theta)
+        for f in range(A + 1):
+            for g in range(A + 1):
+                v = Cm[f, g] * et["M"]
+                if f == g and f >= 1:
+                    v = v - dt * theta * K_cells[:, f - 1, None, None] * et["S"]
+                blocks.append((lay.pdofs(cell_p1, f), lay.pdofs(cell_p1, g), v))
+    else:
+        G = mu * et["G"]
+        Gu = np.zeros((nC, 10, 3, 10, 3))
+        for j in range(3):
+            Gu[:, :, j, :, j] = G
+        blocks.append((ud, ud, Gu.reshape(nC, 30, 30)))
+        blocks.append((lay.pdofs(cell_p1, 0), lay.pdofs(cell_p1, 0), et["M"]))
+        for i in range(A):
+            a_i = alpha[i] ** 2 / lmbda + c[i] + sum(dt * theta * S[i][j]
+                                                     for j in range(A) if j != i)
+            v = a_i * et["M"] + dt * theta * K_cells[:, i, None, None] * et["S"]
+            pdi = lay.pdofs(cell_p1, i + 1)
+            blocks.append((pdi, pdi, v))
+    rows, cols, vals = [], [], []
+    for (ld, rd, v) in blocks:
+        R, Cc = _blk(ld, rd, v)
+        rows.append(R.reshape(-1))
+        cols.append(Cc.reshape(-1))
+        vals.append(v.reshape(-1))
+    if which == "A" and robin is not None:
+        for i, rb in enumerate(robin):
+            if rb is None:
+                continue
+            tri_dofs, Mf = rb                        # (nF,3) P1 nodes, (nF,3,3)
+            pdi = lay.nu + (A + 1) * tri_dofs + (i + 1)
+            R, Cc = _blk(pdi, pdi, Mf)
+            rows.append(R.reshape(-1))
+            cols.append(Cc.reshape(-1))
+            vals.append((-dt * theta * Mf).reshape(-1))
+    M = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(lay.n, lay.n)).tocsr()
+    M.sum_duplicates()
+    M.sort_indices()
+    return M
+
+
+# ---------------------------------------------------------------------------
+# sources (Expression(degree=k) interpolated into P_k, then exact quadrature)
+# ---------------------------------------------------------------------------
+
+
+def cell_source_vector(coords, cells, cell_test, f, degree, t, ncomp):
+    """int f . v over cells for P2 (cell_test (nC,10)) or P1 (nC,4) test
+    functions, with f first interpolated into P_degree on each cell.
+    Returns (nC, ntest, ncomp) local contributions."""
+    J, det, _ = cell_geometry(coords, cells)
+    nodes = lagrange_nodes(degree)
+    v0 = coords[cells[:, 0]]
+    X = v0[:, None, :] + np.einsum("cij,nj->cni", J, nodes)      # nC,nk,3
+    fv = np.asarray(f(X.reshape(-1, 3), t), float).reshape(X.shape[0], X.shape[1], -1)
+    order = degree + 2 + 1
+    qp, qw = tet_rule(order)
+    Lk = lagrange_eval(degree, qp) if degree > 0 else np.ones((len(qp), 1))
+    T = p2_ref_values(qp) if cell_test.shape[1] == 10 else p1_ref_values(qp)
+    fq = np.einsum("qm,cmk->cqk", Lk, fv)
+    return np.einsum("q,c,qa,cqk->cak", qw, det, T, fq)[:, :, :ncomp]
+
+
+def scatter_vector(n, dofs, vals):
+    b = np.zeros(n)
+    np.add.at(b, dofs.reshape(-1), vals.reshape(-1))
+    return b
+
+
+def facet_geometry(coords, tris):
+    v = coords[tris]
+    e1 = v[:, 1] - v[:, 0]
+    e2 = v[:, 2] - v[:, 0]
+    cr = np.cross(e1, e2)
+    area2 = np.linalg.norm(cr, axis=1)          # = 2 * area
+    return v, e1, e2, area2
+
+
+def facet_p1_mass(coords, tris):
+    """(nF,3,3) P1 mass on triangles: |F|/12 (1 + delta)."""
+    _, _, _, a2 = facet_geometry(coords, tris)
+    base = np.array([[2, 1, 1], [1, 2, 1], [1, 1, 2]], float) / 12.0
+    return (a2 / 2.0)[:, None, None] * base
+
+
+def facet_source_vector(coords, tris, tri_test_nodes, f, degree, t, normals=None):
+    """int_F f . v over facets.  tri_test_nodes: 3 (P1) or 6 (P2, vertices then
+    edges (1,2),(0,2),(0,1)) node coordinates are implied by tris.
+    f(X, t) or f(X, t, normals) -> (n, ncomp)."""
+    v, e1, e2, a2 = facet_geometry(coords, tris)
+    kn = lagrange_nodes(degree, 2)
+    X = v[:, 0][:, None, :] + kn[None, :, 0, None] * e1[:, None, :] + kn[None, :, 1, None] * e2[:, None, :]
+    nF, nk = X.shape[:2]
+    if normals is None:
+        fv = np.asarray(f(X.reshape(-1, 3), t), float)
+    else:
+        fv = np.asarray(f(X.reshape(-1, 3), t, np.repeat(normals, nk, axis=0)), float)
+    fv = fv.reshape(nF, nk, -1)
+    qp, qw = tri_rule(degree + 2 + 1)
+    Lk = lagrange_eval(degree, qp, 2) if degree > 0 else np.ones((len(qp), 1))
+    x, y = qp[:, 0], qp[:, 1]
+    lam = np.stack([1 - x - y, x, y], axis=1)
+    if tri_test_nodes == 3:
+        T = lam
+    else:
+        T = np.concatenate([lam * (2 * lam - 1),
+                            np.stack([4 * lam[:, 1] * lam[:, 2], 4 * lam[:, 0] * lam[:, 2],
+                                      4 * lam[:, 0] * lam[:, 1]], axis=1)], axis=1)
+    fq = np.einsum("qm,fmk->fqk", Lk, fv)
+    return np.einsum("q,f,qa,fqk->fak", qw, a2, T, fq)
+
+
+# ---------------------------------------------------------------------------
+# boundary conditions
+# ---------------------------------------------------------------------------
+
+
+def dirichlet_dofs(mesh_facets, facet_cell, facet_local, cells, cell_p2, cell_p1,
+                   markers, field, lay):
+    """DOFs in the closure of facets marked 0 (DOLFIN topological method).
+    field 0 -> all u DOFs of the facet's 3 vertex + 3 edge P2 nodes;
+    field i >= 1 -> p_i DOFs of its 3 vertices."""
+    sel = np.nonzero(np.asarray(markers) == 0)[0]
+    if sel.size == 0:
+        return np.zeros(0, dtype=np.int64)
+    cl = facet_cell[sel]
+    lf = facet_local[sel]
+    verts = np.array(FACES)[lf]                                   # nF,3
+    if field == 0:
+        nodes = [cell_p2[cl, verts[:, k]] for k in range(3)]
+        for (a, b) in ((1, 2), (0, 2), (0, 1)):
+            pairs = np.sort(np.stack([verts[:, a], verts[:, b]], 1), axis=1)
+            eidx = np.array([EDGES.index(tuple(p)) for p in pairs.tolist()])
+            nodes.append(cell_p2[cl, 4 + eidx])
+        nodes = np.unique(np.concatenate(nodes))
+        return np.sort((3 * nodes[:, None] + np.arange(3)).reshape(-1))
+    nodes = np.unique(np.concatenate([cell_p1[cl, verts[:, k]] for k in range(3)]))
+    return np.sort(lay.nu + (lay.A + 1) * nodes + field)
+
+
+def apply_row_replace(A, dofs):
+    """DirichletBC.apply(A): rows replaced by identity rows (not symmetric)."""
+    A = A.tolil(copy=True)
+    for d in dofs:
+        A.rows[d] = [d]
+        A.data[d] = [1.0]
+    return A.tocsr()
+
+
+def apply_symmetric(A, dofs, b=None):
+    """MatZeroRowsColumns(dofs, diag=1, x=b, b=b) (bc_symmetric.py:11-18):
+    b_other -= A[other, dofs] b[dofs]; rows and columns zeroed, diagonal 1.
+    b[dofs] keeps its values (diag * x = b)."""
+    A = A.tocsr()
+    n = A.shape[0]
+    mask = np.zeros(n, bool)
+    mask[dofs] = True
+    if b is not None:
+        xd = np.zeros(n)
+        xd[dofs] = b[dofs]
+        corr = A @ xd
+        b[~mask] -= corr[~mask]
+    keep = sp.diags((~mask).astype(float))
+    A2 = (keep @ A @ keep).tocsr()
+    A2 = A2 + sp.diags(mask.astype(float))
+    A2 = A2.tocsr()
+    A2.sort_indices()
+    return A2
+
+
+# ---------------------------------------------------------------------------
+# MINRES (Paige & Saunders 1975), preconditioned
+# ---------------------------------------------------------------------------
+
+
+def minres(Aop, b, Minv, x0=None, rtol=1e-10, maxiter=10000):
+    """Textbook preconditioned MINRES.  Stops when the preconditioned
+    residual norm estimate phibar <= rtol * ||r0||_{M^-1}.  Returns
+    (x, iterations, phibar history)."""
+    n = b.shape[0]
+    x = np.zeros(n) if x0 is None else x0.copy()
+    r1 = b - Aop(x)
+    y = Minv(r1)
+    beta1 = math.sqrt(max(float(r1 @ y), 0.0))
+    if beta1 == 0.0:
+        return x, 0, [0.0]
+    oldb, beta, dbar, epsln, phibar = 0.0, beta1, 0.0, 0.0, beta1
+    cs, sn = -1.0, 0.0
+    w = np.zeros(n)
+    w2 = np.zeros(n)
+    r2 = r1.copy()
+    hist = [beta1]
+    it = 0
+    while it < maxiter:
+        it += 1
+        s = 1.0 / beta
+        v = s * y
+        y = Aop(v)
+        if it >= 2:
+            y = y - (beta / oldb) * r1
+        alfa = float(v @ y)
+        y = y - (alfa / beta) * r2
+        r1 = r2
+        r2 = y
+        y = Minv(r2)
+        oldb = beta
+        beta = math.sqrt(max(float(r2 @ y), 0.0))
+        oldeps = epsln
+        delta = cs * dbar + sn * alfa
+        gbar = sn * dbar - cs * alfa
+        epsln = sn * beta
+        dbar = -cs * beta
+        gamma = max(math.hypot(gbar, beta), 1e-300)
+        cs = gbar / gamma
+        sn = beta / gamma
+        phi = cs * phibar
+        phibar = sn * phibar
+        w1 = w2
+        w2 = w
+        w = (v - oldeps * w1 - delta * w2) / gamma
+        x = x + phi * w
+        hist.append(phibar)
+        if phibar <= rtol * beta1:
+            break
+    return x, it, hist
+
+
+# ---------------------------------------------------------------------------
+# a complete problem description + time loop
+# ---------------------------------------------------------------------------
+
+
+class OracleProblem:
+    """Raw-array restatement of MPETProblem + MPETTotalPressureSolver.
+
+    mesh: dict with coords, cells, cell_p2, cell_p1, nP2, nP1, x2 (P2 node
+          coordinates), facets, facet_cell, facet_local, exterior.
+    params: {J, E, nu, alpha, c, K, S}; solver: {dt, theta, T, t}.
+    f(X,t)->(n,3), g[i](X,t)->(n,), u_bar(X,t)->(n,3), p_bar[i](X,t)->(n,)
+    (numpy callables); degrees as in Expression(degree=k).
+    markers: momentum (nfacets,), continuity list of (nfacets,).
+    """
+
+    def __init__(self, mesh, params, solver, f=None, g=None, u_bar=None,
+                 p_bar=None, f_degree=3, g_degree=3, K_cells=None,
+                 momentum_markers=None, continuity_markers=None,
+                 s=None, s_degree=3, I=None, I_degree=3, beta=None,
+                 p_robin=None, robin_degree=1):
+        self.m = mesh
+        self.params = params
+        self.A = int(params["J"])
+        self.dt = float(solver["dt"])
+        self.theta = float(solver["theta"])
+        self.T = float(solver["T"])
+        self.t0 = float(solver.get("t", 0.0))
+        self.f, self.g, self.u_bar, self.p_bar = f, g, u_bar, p_bar
+        self.f_degree, self.g_degree = f_degree, g_degree
+        self.s, self.s_degree = s, s_degree
+        self.I, self.I_degree = I, I_degree
+        self.beta, self.p_robin, self.robin_degree = beta, p_robin, robin_degree
+        self.K_cells = K_cells
+        self.lay = Layout(mesh["nP2"], mesh["nP1"], self.A)
+        nF = mesh["facets"].shape[0]
+        self.mm = momentum_markers if momentum_markers is not None else np.full(nF, 2**63 - 1)
+        self.cm = continuity_markers or [np.full(nF, 2**63 - 1) for _ in range(self.A)]
+        self._robin_mass = self._robin_terms()
+        m = mesh
+        self.A_mat = assemble_matrix(m["coords"], m["cells"], m["cell_p2"], m["cell_p1"],
+                                     m["nP2"], m["nP1"], params, self.dt, self.theta,
+                                     K_cells, robin=self._robin_mass, which="A")
+        self.P_mat = assemble_matrix(m["coords"], m["cells"], m["cell_p2"], m["cell_p1"],
+                                     m["nP2"], m["nP1"], params, self.dt, self.theta,
+                                     K_cells, which="P")
+        self.bc_dofs = self._bc_dofs()
+        self._B_mat = None
+
+    # -- boundary facets -----------------------------------------------------
+    def _marked_facets(self, markers, value):
+        m = self.m
+        sel = np.nonzero((np.asarray(markers) == value) & m["exterior"])[0]
+        cl = m["facet_cell"][sel]
+        lf = m["facet_local"][sel]
+        verts = np.array(FACES)[lf]
+        return sel, cl, verts
+
+    def _robin_terms(self):
+        if self.beta is None:
+            return None
+        out = []
+        m = self.m
+        for i in range(self.A):
+            sel, cl, verts = self._marked_facets(self.cm[i], 2)
+            if sel.size == 0:
+                out.append(None)
+                continue
+            tri_nodes = np.stack([m["cell_p1"][cl, verts[:, k]] for k in range(3)], 1)
+            tri_v = np.stack([m["cells"][cl, verts[:, k]] for k in range(3)], 1)
+            # int_F beta(t0) psi psi with beta interpolated in P_robin_degree
+            vals = facet_source_matrix(m["coords"], tri_v, self.beta[i],
+                                       self.robin_degree, self.t0)
+            out.append((tri_nodes, vals))
+        return out
+
+    def _bc_dofs(self):
+        m = self.m
+        out = [dirichlet_dofs(m["facets"], m["facet_cell"], m["facet_local"], m["cells"],
+                              m["cell_p2"], m["cell_p1"], self.mm, 0, self.lay)]
+        for i in range(self.A):
+            out.append(dirichlet_dofs(m["facets"], m["facet_cell"], m["facet_local"],
+                                      m["cells"], m["cell_p2"], m["cell_p1"],
+                                      self.cm[i], i + 1, self.lay))
+        return out
+
+    def bc_values(self, t):
+        """Nodal Dirichlet values at time t for each BC DOF set."""
+        m = self.m
+        lay = self.lay
+        vals = []
+        d0 = self.bc_dofs[0]
+        if d0.size:
+            nodes = d0 // 3
+            comp = d0 % 3
+            ub = self.u_bar(m["x2"][nodes], t) if self.u_bar is not None else np.zeros((nodes.size, 3))
+            vals.append(np.asarray(ub)[np.arange(nodes.size), comp])
+        else:
+            vals.append(np.zeros(0))
+        for i in range(self.A):
+            d = self.bc_dofs[i + 1]
+            if d.size:
+                nodes = (d - lay.nu) // (self.A + 1)
+                pb = self.p_bar[i](m["coords"][nodes], t) if self.p_bar is not None else np.zeros(nodes.size)
+                vals.append(np.asarray(pb, float).reshape(-1))
+            else:
+                vals.append(np.zeros(0))
+        return vals
+
+    # -- right-hand side ------------------------------------------------------
+    def B_matrix(self):
+        """Matrix of the up_-linear part of b (rhs of F, evaluated at t_n)."""
+        if self._B_mat is not None:
+            return self._B_mat
+        m = self.m
+        A = self.A
+        mu, lmbda = lame(self.params["E"], self.params["nu"])
+        _, Bm = pressure_coefficients(self.params["alpha"], self.params["c"],
+                                      self.params["S"], lmbda, self.dt, self.theta)
+        et = element_tensors(m["coords"], m["cells"])
+        nC = m["cells"].shape[0]
+        K = self.K_cells if self.K_cells is not None else np.tile(
+            np.asarray(self.params["K"], float), (nC, 1))
+        rows, cols, vals = [], [], []
+        for f in range(1, A + 1):
+            for g in range(A + 1):
+                v = Bm[f, g] * et["M"]
+                if f == g:
+                    v = v + self.dt * (1 - self.theta) * K[:, f - 1, None, None] * et["S"]
+                R, Cc = _blk(self.lay.pdofs(m["cell_p1"], f), self.lay.pdofs(m["cell_p1"], g), v)
+                rows.append(R.reshape(-1))
+                cols.append(Cc.reshape(-1))
+                vals.append(v.reshape(-1))
+        if self._robin_mass is not None:
+            for i, rb in enumerate(self._robin_mass):
+                if rb is None:
+                    continue
+                # beta(t_theta) is re-evaluated per step in rhs_sources
+                pass
+        B = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                          shape=(self.lay.n, self.lay.n)).tocsr()
+        self._B_mat = B
+        return B
+
+    def rhs(self, up_prev, t_n):
+        """b assembled as in solve_iterative :461-483 (before BCs)."""
+        m = self.m
+        lay = self.lay
+        b = self.B_matrix() @ up_prev                          # L at t_n
+        t_theta = t_n + self.theta * self.dt
+        t_np1 = t_theta + (1.0 - self.theta) * self.dt
+        # L1: dt g_i w_i dx + dt I_i w_i ds_i(1) at t_theta
+        if self.g is not None:
+            for i in range(self.A):
+                if self.g[i] is None:
+                    continue
+                loc = cell_source_vector(m["coords"], m["cells"], m["cell_p1"],
+                                         lambda X, t, gi=self.g[i]: np.asarray(gi(X, t)).reshape(-1, 1),
+                                         self.g_degree, t_theta, 1)
+                b += self.dt * scatter_vector(lay.n, lay.pdofs(m["cell_p1"], i + 1), loc[..., 0])
+        if self.I is not None:
+            for i in range(self.A):
+                if self.I[i] is None:
+                    continue
+                sel, cl, verts = self._marked_facets(self.cm[i], 1)
+                if sel.size == 0:
+                    continue
+                tri_v = np.stack([m["cells"][cl, verts[:, k]] for k in range(3)], 1)
+                tri_n = np.stack([m["cell_p1"][cl, verts[:, k]] for k in range(3)], 1)
+                loc = facet_source_vector(m["coords"], tri_v, 3,
+                                          lambda X, t, Ii=self.I[i]: np.asarray(Ii(X, t)).reshape(-1, 1),
+                                          self.I_degree, t_theta)
+                b += self.dt * scatter_vector(lay.n, lay.pdofs(tri_n, i + 1), loc[..., 0])
+        # rhs(L2_i): dt beta((1-theta) p_i_ - p_robin) w_i ds_i(2) at t_theta
+        if self.beta is not None:
+            for i in range(self.A):
+                sel, cl, verts = self._marked_facets(self.cm[i], 2)
+                if sel.size == 0:
+                    continue
+                tri_v = np.stack([m["cells"][cl, verts[:, k]] for k in range(3)], 1)
+                tri_n = np.stack([m["cell_p1"][cl, verts[:, k]] for k in range(3)], 1)
+                pprev = up_prev[lay.pdofs(tri_n, i + 1)]                     # nF,3
+
+                def fdat(X, t, bi=self.beta[i], pr=self.p_robin[i] if self.p_robin else None):
+                    return np.asarray(bi(X, t)).reshape(-1, 1)
+                Mf = facet_source_matrix(m["coords"], tri_v, self.beta[i], self.robin_degree, t_theta)
+                loc = (1 - self.theta) * np.einsum("fab,fb->fa", Mf, pprev)
+                if self.p_robin is not None and self.p_robin[i] is not None:
+                    # int beta p_robin w with the product beta*p_robin interpolated
+                    pr = facet_source_vector(
+                        m["coords"], tri_v, 3,
+                        lambda X, t, bi=self.beta[i], pi_=self.p_robin[i]:
+                            (np.asarray(bi(X, t)) * np.asarray(pi_(X, t))).reshape(-1, 1),
+                        self.robin_degree, t_theta)[..., 0]
+                    loc = loc - pr
+                b += self.dt * scatter_vector(lay.n, lay.pdofs(tri_n, i + 1), loc)
+        # L0: f . v dx + s . v ds_m(1) at t_{n+1}
+        if self.f is not None:
+            loc = cell_source_vector(m["coords"], m["cells"], m["cell_p2"], self.f,
+                                     self.f_degree, t_np1, 3)
+            b += scatter_vector(lay.n, lay.udofs(m["cell_p2"]), loc.reshape(-1, 30))
+        if self.s is not None:
+            sel, cl, verts = self._marked_facets(self.mm, 1)
+            if sel.size:
+                tri_v = np.stack([m["cells"][cl, verts[:, k]] for k in range(3)], 1)
+                nodes = [m["cell_p2"][cl, verts[:, k]] for k in range(3)]
+                for (a, bb) in ((1, 2), (0, 2), (0, 1)):
+                    pairs = np.sort(np.stack([verts[:, a], verts[:, bb]], 1), axis=1)
+                    eidx = np.array([EDGES.index(tuple(p)) for p in pairs.tolist()])
+                    nodes.append(m["cell_p2"][cl, 4 + eidx])
+                nodes = np.stack(nodes, 1)                                    # nF,6
+                normals = outward_normals(m["coords"], m["cells"], cl, verts)
+                loc = facet_source_vector(m["coords"], tri_v, 6, self.s, self.s_degree,
+                                          t_np1, normals=normals)              # nF,6,3
+                dofs = (3 * nodes[:, :, None] + np.arange(3)).reshape(-1, 18)
+                b += scatter_vector(lay.n, dofs, loc.reshape(-1, 18))
+        return b, t_theta, t_np1
+
+    # -- time loops -----------------------------------------------------------
+    def solve_direct(self, up0):
+        """Generator mirroring solve_direct (:330-411): LU of row-replaced A."""
+        A = self.A_mat.copy()
+        alld = np.concatenate(self.bc_dofs)
+        A = apply_row_replace(A, alld)
+        lu = spla.splu(A.tocsc())
+        up_prev = up0.copy()
+        t = self.t0
+        while t < self.T - 1e-9:
+            b, t_theta, t_np1 = self.rhs(up_prev, t)
+            for d, v in zip(self.bc_dofs, self.bc_values(t_np1)):
+                b[d] = v
+            up = lu.solve(b)
+            yield up, t_np1
+            up_prev = up
+            t = t_np1
+
+    def eliminated_system(self, b, t_np1):
+        """Symmetric elimination of all BCs in order (:487-489)."""
+        A = self.A_mat.copy()
+        for d, v in zip(self.bc_dofs, self.bc_values(t_np1)):
+            b[d] = v
+            A = apply_symmetric(A, d, b)
+        return A, b
+
+    def preconditioner(self):
+        P = self.P_mat.copy()
+        for d in self.bc_dofs:
+            P = apply_symmetric(P, d)
+        return P
+
+    def solve_iterative(self, up0, rtol=1e-10, exact_blocks=True):
+        """Generator mirroring solve_iterative (:414-505): MINRES on the
+        symmetrically eliminated system, preconditioned by the eliminated P.
+        The block-diagonal P is inverted exactly (sparse LU per block) —
+        the reference applies BoomerAMG instead, which the GPU path
+        replaces by its own AMG; only the converged solution is compared."""
+        P = self.preconditioner()
+        lu = spla.splu(P.tocsc())
+        up_prev = up0.copy()
+        t = self.t0
+        self.niter = []
+        while t < self.T - 1e-9:
+            b, t_theta, t_np1 = self.rhs(up_prev, t)
+            Ael, bel = self.eliminated_system(b, t_np1)
+            x, it, _ = minres(lambda v: Ael @ v, bel, lu.solve, x0=up_prev, rtol=rtol)
+            self.niter.append(it)
+            yield x, t_np1
+            up_prev = x
+            t = t_np1
+
+
+def facet_source_matrix(coords, tris, beta, degree, t):
+    """(nF,3,3) int_F beta psi_a psi_b with beta interpolated into P_degree."""
+    v, e1, e2, a2 = facet_geometry(coords, tris)
+    kn = lagrange_nodes(degree, 2)
+    X = v[:, 0][:, None, :] + kn[None, :, 0, None] * e1[:, None, :] + kn[None, :, 1, None] * e2[:, None, :]
+    nF, nk = X.shape[:2]
+    bv = np.asarray(beta(X.reshape(-1, 3), t), float).reshape(nF, nk)
+    qp, qw = tri_rule(degree + 2 + 1)
+    Lk = lagrange_eval(degree, qp, 2) if degree > 0 else np.ones((len(qp), 1))
+    x, y = qp[:, 0], qp[:, 1]
+    lam = np.stack([1 - x - y, x, y], axis=1)
+    bq = np.einsum("qm,fm->fq", Lk, bv)
+    return np.einsum("q,f,qa,qb,fq->fab", qw, a2, lam, lam, bq)
+
+
+def outward_normals(coords, cells, cl, verts):
+    v = coords[cells[cl]]                                   # nF,4,3
+    tri = np.stack([v[np.arange(len(cl)), verts[:, k]] for k in range(3)], 1)
+    n = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    cen = v.mean(axis=1)
+    sgn = np.sign(np.einsum("fi,fi->f", n, tri[:, 0] - cen))
+    return n * sgn[:, None]

@@ -1,0 +1,238 @@
+"""CPU tests pinning the FP64 oracle (and the product's reference tables).
+
+Checks are chosen so that a plausible slip (dropped term, wrong sign,
+transposed operand, wrong index) fails one of them:
+  * element tensors == sympy-exact integration on 4 tetrahedra (fixtures);
+  * the product's barycentric tables reproduce the same fixtures;
+  * kernels: rigid motions annihilate eps:eps, constants the Laplacians;
+    div of a linear field against P1 tests equals its integral;
+  * symmetric elimination == row replacement (same solution);
+  * known answer: u = 0, p_j = P_j, S = 0 is a steady discrete solution with
+    p_0 = -sum_j alpha_j P_j;
+  * textbook MINRES == scipy's MINRES;
+  * 3D manufactured solution reaches the reference's convergence-rate
+    thresholds (test_convergence_totalpressuresolver.py:237-242)."""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from conftest import CFG1, oracle_mesh
+from oracle import mpet_oracle as O
+from oracle import mms
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "element_tensors.npz")
+
+
+def _gold():
+    return np.load(GOLD)
+
+
+@pytest.mark.parametrize("k", range(4))
+def test_oracle_element_tensors_match_sympy(k):
+    d = _gold()
+    et = O.element_tensors(d["coords"][k], np.array([[0, 1, 2, 3]]))
+    for nm in "EGBMS":
+        ref = d[f"{nm}{k}"]
+        assert np.abs(et[nm][0] - ref).max() <= 1e-13 * np.abs(ref).max(), nm
+
+
+@pytest.mark.parametrize("k", range(4))
+def test_product_tables_match_sympy(k):
+    """The GPU kernel combines W[a,b,k,l] g_k g_l^T and V[b,c,k] g_k; the
+    same combination in numpy must give the exact tensors."""
+    from waterscapes_amd import refel
+    d = _gold()
+    X = d["coords"][k]
+    J = np.stack([X[1] - X[0], X[2] - X[0], X[3] - X[0]], axis=1)
+    vol = abs(np.linalg.det(J)) / 6
+    Gi = np.linalg.inv(J)
+    g = np.vstack([-Gi.sum(0), Gi])                       # grad lambda_k, rows
+    W = refel.grad_weights_p2p2()
+    V = refel.div_weights_p2p1()
+    M = vol * np.einsum("abkl,km,ln->abmn", W, g, g)      # int d_m phi_a d_n phi_b
+    G = np.einsum("abmm->ab", M)
+    assert np.abs(G - d[f"G{k}"]).max() <= 1e-13 * np.abs(d[f"G{k}"]).max()
+    E = np.zeros((10, 3, 10, 3))
+    for i in range(3):
+        for j in range(3):
+            E[:, i, :, j] = 0.5 * ((i == j) * G + M[:, :, j, i])
+    assert np.abs(E.reshape(30, 30) - d[f"E{k}"]).max() <= 1e-13 * np.abs(d[f"E{k}"]).max()
+    B = vol * np.einsum("bck,km->bmc", V, g).reshape(30, 4)
+    assert np.abs(B - d[f"B{k}"]).max() <= 1e-13 * np.abs(d[f"B{k}"]).max()
+    Q = vol * refel.source_weights(1, 3)
+    assert np.abs(Q - d[f"Q3_{k}"]).max() <= 1e-13 * np.abs(d[f"Q3_{k}"]).max()
+    M1 = vol * refel.mass_p1()
+    assert np.abs(M1 - d[f"M{k}"]).max() <= 1e-14 * np.abs(d[f"M{k}"]).max()
+
+
+def test_element_kernels_and_closed_forms():
+    rng = np.random.default_rng(0)
+    X = rng.random((4, 3)) + np.eye(4, 3)
+    et = O.element_tensors(X, np.array([[0, 1, 2, 3]]))
+    # P2 node coordinates (vertices then UFC edges)
+    nodes = np.vstack([X] + [0.5 * (X[i] + X[j]) for (i, j) in O.EDGES])
+    modes = []
+    for k in range(3):
+        t = np.zeros((10, 3))
+        t[:, k] = 1
+        modes.append(t.reshape(-1))
+    for (a, b) in ((0, 1), (0, 2), (1, 2)):
+        r = np.zeros((10, 3))
+        r[:, a] = -nodes[:, b]
+        r[:, b] = nodes[:, a]
+        modes.append(r.reshape(-1))
+    scale = np.abs(et["E"][0]).max()
+    for mvec in modes:
+        assert np.abs(et["E"][0] @ mvec).max() <= 1e-13 * scale * 10
+    assert np.abs(et["G"][0] @ np.ones(10)).max() <= 1e-13 * np.abs(et["G"][0]).max() * 10
+    assert np.abs(et["S"][0] @ np.ones(4)).max() <= 1e-13 * np.abs(et["S"][0]).max() * 10
+    vol = et["det"][0] / 6
+    assert np.allclose(et["M"][0], vol / 20 * (np.ones((4, 4)) + np.eye(4)), rtol=1e-14)
+    # u = x e_0 (div u = 1):  int psi_c div u = |K| / 4
+    u = np.zeros((10, 3))
+    u[:, 0] = nodes[:, 0]
+    assert np.allclose(et["B"][0].T @ u.reshape(-1), vol / 4, rtol=1e-13)
+    # cross-check P1 stiffness closed form |K| grad l_i . grad l_j
+    J = np.stack([X[1] - X[0], X[2] - X[0], X[3] - X[0]], axis=1)
+    Gi = np.linalg.inv(J)
+    g = np.vstack([-Gi.sum(0), Gi])
+    assert np.allclose(et["S"][0], vol * g @ g.T, rtol=1e-13)
+
+
+def _small_problem(n=2, A=2, params=None, **kw):
+    from waterscapes_amd.fem import MixedSpace
+    from waterscapes_amd.mesh import UnitCubeMesh
+    mesh = UnitCubeMesh(n, n, n)
+    space = MixedSpace(mesh, A)
+    return mesh, space, oracle_mesh(space)
+
+
+def test_symmetric_elimination_equals_row_replacement():
+    mesh, space, m = _small_problem(2, 2)
+    A = O.assemble_matrix(m["coords"], m["cells"], m["cell_p2"], m["cell_p1"], m["nP2"],
+                          m["nP1"], CFG1, 0.01, 1.0, which="A")
+    rng = np.random.default_rng(1)
+    b = rng.standard_normal(A.shape[0])
+    dofs = np.sort(rng.choice(A.shape[0], 40, replace=False))
+    b[dofs] = rng.standard_normal(dofs.size)
+    x_row = spla.spsolve(O.apply_row_replace(A, dofs).tocsc(), b.copy())
+    bs = b.copy()
+    As = O.apply_symmetric(A, dofs, bs)
+    assert abs(As - As.T).max() <= 1e-12 * abs(A).max()
+    x_sym = spla.spsolve(As.tocsc(), bs)
+    assert np.allclose(x_row, x_sym, rtol=1e-9, atol=1e-12)
+    assert np.allclose(x_sym[dofs], b[dofs])
+
+
+def _all_dirichlet_markers(mesh):
+    mk = np.full(mesh.num_facets(), 2**63 - 1, dtype=np.uint64)
+    mk[mesh.exterior] = 0
+    return mk
+
+
+def test_uniform_pressure_known_answer():
+    """u = 0, p_j = P_j constant, S = 0: a steady discrete solution with
+    p_0 = -sum_j alpha_j P_j (the p_0 equation, :269)."""
+    params = dict(CFG1)
+    params["S"] = ((0.0, 0.0), (0.0, 0.0))
+    P = (3.0, -1.5)
+    mesh, space, m = _small_problem(2, 2)
+    mk = _all_dirichlet_markers(mesh)
+    lay = O.Layout(m["nP2"], m["nP1"], 2)
+    up0 = np.zeros(lay.n)
+    p0 = -(params["alpha"][0] * P[0] + params["alpha"][1] * P[1])
+    up0[lay.pdofs(np.arange(m["nP1"]), 0)] = p0
+    for j in range(2):
+        up0[lay.pdofs(np.arange(m["nP1"]), j + 1)] = P[j]
+    pb = [lambda X, t, v=P[j]: np.full(X.shape[0], v) for j in range(2)]
+    prob = O.OracleProblem(m, params, dict(dt=0.01, theta=1.0, T=0.02),
+                           u_bar=lambda X, t: np.zeros((X.shape[0], 3)), p_bar=pb,
+                           momentum_markers=mk, continuity_markers=[mk, mk])
+    for up, t in prob.solve_direct(up0):
+        assert np.abs(up - up0).max() <= 1e-11 * np.abs(up0).max()
+    for up, t in prob.solve_iterative(up0, rtol=1e-12):
+        assert np.abs(up - up0).max() <= 1e-9 * np.abs(up0).max()
+
+
+def test_textbook_minres_matches_scipy():
+    mesh, space, m = _small_problem(2, 2)
+    A = O.assemble_matrix(m["coords"], m["cells"], m["cell_p2"], m["cell_p1"], m["nP2"],
+                          m["nP1"], CFG1, 0.01, 1.0, which="A")
+    P = O.assemble_matrix(m["coords"], m["cells"], m["cell_p2"], m["cell_p1"], m["nP2"],
+                          m["nP1"], CFG1, 0.01, 1.0, which="P")
+    # make the un-eliminated P SPD with an identity shift on its u block
+    P = P + sp.diags(np.r_[np.full(3 * m["nP2"], 1e2), np.zeros(A.shape[0] - 3 * m["nP2"])])
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal(A.shape[0])
+    lu = spla.splu(P.tocsc())
+    A = A + sp.diags(np.r_[np.full(3 * m["nP2"], 1e2), np.zeros(A.shape[0] - 3 * m["nP2"])])
+    x, it, _ = O.minres(lambda v: A @ v, b, lu.solve, rtol=1e-12)
+    M = spla.LinearOperator(A.shape, matvec=lu.solve)
+    xs, info = spla.minres(A, b, M=M, rtol=1e-12, maxiter=5000)
+    xd = spla.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
+    # scipy stops on its own (looser) residual estimate
+    assert np.linalg.norm(x - xs) <= 1e-4 * np.linalg.norm(xd)
+
+
+def mms_run(n, params=CFG1, dt=0.05, T=0.1, theta=1.0):
+    mesh, space, m = _small_problem(n, int(params["J"]))
+    u, p, f, g = mms.exact_solutions(params)
+    A = int(params["J"])
+    mk = _all_dirichlet_markers(mesh)
+    prob = O.OracleProblem(
+        m, params, dict(dt=dt, theta=theta, T=T), f=mms.lambdify_vec(f),
+        g=[mms.lambdify(gi) for gi in g], u_bar=mms.lambdify_vec(u),
+        p_bar=[mms.lambdify(p[j + 1]) for j in range(A)], f_degree=3, g_degree=3,
+        momentum_markers=mk, continuity_markers=[mk] * A)
+    lay = prob.lay
+    up0 = np.zeros(lay.n)
+    uv = mms.lambdify_vec(u)(m["x2"], 0.0)
+    up0[: lay.nu] = uv.reshape(-1)
+    for j in range(A + 1):
+        up0[lay.pdofs(np.arange(m["nP1"]), j)] = mms.lambdify(p[j])(m["coords"], 0.0)
+    for up, t in prob.solve_direct(up0):
+        pass
+    errs = {}
+    uh = up[: lay.nu].reshape(-1, 3)
+    eu2 = eh1 = 0.0
+    for k in range(3):
+        l2, h1 = mms.error_norms(m["coords"], m["cells"], m["cell_p2"], m["x2"], uh[:, k],
+                                 u[k], t, 2)
+        eu2 += l2 ** 2
+        eh1 += h1 ** 2
+    errs["u"] = (np.sqrt(eu2), np.sqrt(eh1))
+    for j in range(A + 1):
+        vals = up[lay.pdofs(np.arange(m["nP1"]), j)]
+        errs[f"p{j}"] = mms.error_norms(m["coords"], m["cells"], m["cell_p1"], m["coords"],
+                                        vals, p[j], t, 1)
+    return errs, up, t
+
+
+def test_mms_convergence_rates_3d():
+    """Rates between n = 4 and n = 8 meet the reference thresholds
+    (u L2/H1 > 1.70, p_1 L2 > 1.85, p_2 L2 > 1.87, p H1 > 0.95)."""
+    e4, _, _ = mms_run(4)
+    e8, _, _ = mms_run(8)
+    rate = lambda a, b: np.log(a / b) / np.log(2.0)  # noqa: E731
+    ru2, ruh = rate(e4["u"][0], e8["u"][0]), rate(e4["u"][1], e8["u"][1])
+    rp1, rp2 = rate(e4["p1"][0], e8["p1"][0]), rate(e4["p2"][0], e8["p2"][0])
+    rh1, rh2 = rate(e4["p1"][1], e8["p1"][1]), rate(e4["p2"][1], e8["p2"][1])
+    assert ru2 > 1.70 and ruh > 1.70, (ru2, ruh)
+    assert rp1 > 1.85 and rp2 > 1.87, (rp1, rp2)
+    assert rh1 > 0.95 and rh2 > 0.95, (rh1, rh2)
+
+
+def test_library_exports_every_header_symbol():
+    import ctypes
+    from waterscapes_amd import _lib
+    if not os.path.exists(_lib.LIBPATH):
+        pytest.skip("library not built")
+    L = ctypes.CDLL(_lib.LIBPATH)
+    names = _lib.exported_symbols()
+    assert len(names) >= 15
+    for nm in names:
+        assert hasattr(L, nm), nm

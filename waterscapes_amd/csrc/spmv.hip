@@ -1,0 +1,212 @@
+// Sparse operator application for the MPET system.
+//
+//  * csr_spmv: scalar CSR (the assembled AIJ matrix of the plain FP64 path),
+//    LPR lanes per row (8/16/32/64 chosen per matrix by the autotuner).
+//  * op_apply: the structured total-pressure operator on the node patterns,
+//    values stored structure-of-arrays so that every value load of a wave is
+//    one contiguous 256-byte (FP32) or 512-byte (FP64) segment:
+//        y_u(a)  = sum_b Kuu_ab u_b + sum_c Dv_ac p0_c
+//        y_p0(c) = sum_b Dt_cb . u_b + sum_g C[0][g] (M1 p_g)(c)
+//        y_pf(c) = sum_g C[f][g] (M1 p_g)(c) + stiff[f] (St_f p_f)(c)
+//    The mass is applied once per field; only St differs per network.
+//
+// Dirichlet handling: the operator is stored un-eliminated.  Rows flagged in
+// dmask produce 0 (Krylov mode: iterates have zero Dirichlet components, so
+// Pi A Pi x + (I - Pi) x == Pi A x), and with `b` the kernel returns the
+// masked residual Pi (b - A x) used for the lifting r0 = Pi (b - A x0) and for
+// the FP64 outer residual of iterative refinement.
+#include "mpet_common.h"
+#include "mpet_ops.h"
+
+namespace mpet {
+
+template <typename T, int LPR>
+__global__ void __launch_bounds__(256)
+csr_spmv_kernel(long long nrows, const long long* __restrict__ rp, const int* __restrict__ ci,
+                const T* __restrict__ val, const T* __restrict__ x, const T* __restrict__ b,
+                const uint8_t* __restrict__ dmask, T* __restrict__ y) {
+  long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long r = gid / LPR;
+  int lane = (int)(gid % LPR);
+  if (r >= nrows) return;   // whole groups exit together (LPR divides 256)
+  long long kb = rp[r], ke = rp[r + 1];
+  T s = 0;
+  for (long long k = kb + lane; k < ke; k += LPR) s += val[k] * x[ci[k]];
+  s = group_sum<LPR>(s);
+  if (lane == 0) {
+    T out = b ? b[r] - s : s;
+    if (dmask && dmask[r]) out = 0;
+    y[r] = out;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// structured operator
+// ---------------------------------------------------------------------------
+constexpr int OP_G = 16;             // lanes per node row
+constexpr int OP_ROWS = 256 / OP_G;  // node rows per block
+
+template <typename T, int F>
+__global__ void __launch_bounds__(256)
+op_apply_kernel(MpetOpT<T> op, const T* __restrict__ x, const T* __restrict__ b,
+                T* __restrict__ y, int nbu) {
+  const int lane = threadIdx.x % OP_G;
+  const long long nu = 3ll * op.nP2;
+  if ((int)blockIdx.x < nbu) {
+    int a = blockIdx.x * OP_ROWS + threadIdx.x / OP_G;
+    if (a >= op.nP2) return;
+    T y0 = 0, y1 = 0, y2 = 0;
+    const int kb = op.rp22[a], ke = op.rp22[a + 1];
+    const long long n22 = op.nnz22;
+    for (int k = kb + lane; k < ke; k += OP_G) {
+      const int cb = 3 * op.ci22[k];
+      const T x0 = x[cb], x1 = x[cb + 1], x2 = x[cb + 2];
+      y0 += op.Kuu[k] * x0 + op.Kuu[n22 + k] * x1 + op.Kuu[2 * n22 + k] * x2;
+      y1 += op.Kuu[3 * n22 + k] * x0 + op.Kuu[4 * n22 + k] * x1 + op.Kuu[5 * n22 + k] * x2;
+      y2 += op.Kuu[6 * n22 + k] * x0 + op.Kuu[7 * n22 + k] * x1 + op.Kuu[8 * n22 + k] * x2;
+    }
+    const int qb = op.rp21[a], qe = op.rp21[a + 1];
+    const long long n21 = op.nnz21;
+    for (int k = qb + lane; k < qe; k += OP_G) {
+      const T p0 = x[nu + (long long)F * op.ci21[k]];
+      y0 += op.Dv[k] * p0;
+      y1 += op.Dv[n21 + k] * p0;
+      y2 += op.Dv[2 * n21 + k] * p0;
+    }
+    y0 = group_sum<OP_G>(y0);
+    y1 = group_sum<OP_G>(y1);
+    y2 = group_sum<OP_G>(y2);
+    if (lane == 0) {
+      const long long o = 3ll * a;
+      if (b) { y0 = b[o] - y0; y1 = b[o + 1] - y1; y2 = b[o + 2] - y2; }
+      if (op.dmask) {
+        if (op.dmask[o]) y0 = 0;
+        if (op.dmask[o + 1]) y1 = 0;
+        if (op.dmask[o + 2]) y2 = 0;
+      }
+      y[o] = y0; y[o + 1] = y1; y[o + 2] = y2;
+    }
+  } else {
+    int c = (blockIdx.x - nbu) * OP_ROWS + threadIdx.x / OP_G;
+    if (c >= op.nP1) return;
+    T yd = 0;
+    const int qb = op.rp12[c], qe = op.rp12[c + 1];
+    const long long n12 = op.nnz12;
+    for (int k = qb + lane; k < qe; k += OP_G) {
+      const int cb = 3 * op.ci12[k];
+      yd += op.Dt[k] * x[cb] + op.Dt[n12 + k] * x[cb + 1] + op.Dt[2 * n12 + k] * x[cb + 2];
+    }
+    T m[F], s[F];
+#pragma unroll
+    for (int g = 0; g < F; ++g) { m[g] = 0; s[g] = 0; }
+    const int sb = op.rp11[c], se = op.rp11[c + 1];
+    const long long n11 = op.nnz11;
+    for (int k = sb + lane; k < se; k += OP_G) {
+      const T* xp = x + nu + (long long)F * op.ci11[k];
+      const T mv = op.M1[k];
+#pragma unroll
+      for (int g = 0; g < F; ++g) {
+        const T xv = xp[g];
+        m[g] += mv * xv;
+        if (g >= 1) s[g] += op.St[(g - 1) * n11 + k] * xv;
+      }
+    }
+    yd = group_sum<OP_G>(yd);
+#pragma unroll
+    for (int g = 0; g < F; ++g) {
+      m[g] = group_sum<OP_G>(m[g]);
+      s[g] = group_sum<OP_G>(s[g]);
+    }
+    if (lane == 0) {
+      const long long o = nu + (long long)F * c;
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        T v = (f == 0) ? yd : (T)op.stiff[f] * s[f];
+#pragma unroll
+        for (int g = 0; g < F; ++g) v += (T)op.C[f * MPET_MAXF + g] * m[g];
+        if (b) v = b[o + f] - v;
+        if (op.dmask && op.dmask[o + f]) v = 0;
+        y[o + f] = v;
+      }
+    }
+  }
+}
+
+// AoS FP64 primitives -> SoA values of type T (structured operator storage).
+template <typename T>
+__global__ void soa_convert_kernel(const double* __restrict__ src, int ncomp, long long n,
+                                   T* __restrict__ dst) {
+  long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  for (int c = 0; c < ncomp; ++c) dst[c * n + k] = (T)src[k * ncomp + c];
+}
+
+template <typename T>
+__global__ void copy_convert_kernel(const double* __restrict__ src, long long n,
+                                    T* __restrict__ dst) {
+  long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dst[k] = (T)src[k];
+}
+
+// ---------------------------------------------------------------------------
+// launchers (host)
+// ---------------------------------------------------------------------------
+template <typename T>
+int launch_csr_spmv(long long nrows, const long long* rp, const int* ci, const T* val,
+                    const T* x, const T* b, const uint8_t* dmask, T* y, int lpr,
+                    hipStream_t s) {
+  long long threads = nrows * lpr;
+  int blocks = (int)((threads + 255) / 256);
+  switch (lpr) {
+    case 4: hipLaunchKernelGGL((csr_spmv_kernel<T, 4>), dim3(blocks), dim3(256), 0, s, nrows, rp, ci, val, x, b, dmask, y); break;
+    case 8: hipLaunchKernelGGL((csr_spmv_kernel<T, 8>), dim3(blocks), dim3(256), 0, s, nrows, rp, ci, val, x, b, dmask, y); break;
+    case 16: hipLaunchKernelGGL((csr_spmv_kernel<T, 16>), dim3(blocks), dim3(256), 0, s, nrows, rp, ci, val, x, b, dmask, y); break;
+    case 32: hipLaunchKernelGGL((csr_spmv_kernel<T, 32>), dim3(blocks), dim3(256), 0, s, nrows, rp, ci, val, x, b, dmask, y); break;
+    case 64: hipLaunchKernelGGL((csr_spmv_kernel<T, 64>), dim3(blocks), dim3(256), 0, s, nrows, rp, ci, val, x, b, dmask, y); break;
+    default: return ERR_INVALID;
+  }
+  return hipGetLastError() == hipSuccess ? OK : ERR_HIP;
+}
+template int launch_csr_spmv<double>(long long, const long long*, const int*, const double*,
+                                     const double*, const double*, const uint8_t*, double*, int,
+                                     hipStream_t);
+template int launch_csr_spmv<float>(long long, const long long*, const int*, const float*,
+                                    const float*, const float*, const uint8_t*, float*, int,
+                                    hipStream_t);
+
+template <typename T>
+int launch_op_apply(const MpetOpT<T>& op, const T* x, const T* b, T* y, hipStream_t s) {
+  int nbu = (op.nP2 + OP_ROWS - 1) / OP_ROWS;
+  int nbp = (op.nP1 + OP_ROWS - 1) / OP_ROWS;
+  dim3 g(nbu + nbp), blk(256);
+  switch (op.A + 1) {
+    case 2: hipLaunchKernelGGL((op_apply_kernel<T, 2>), g, blk, 0, s, op, x, b, y, nbu); break;
+    case 3: hipLaunchKernelGGL((op_apply_kernel<T, 3>), g, blk, 0, s, op, x, b, y, nbu); break;
+    case 4: hipLaunchKernelGGL((op_apply_kernel<T, 4>), g, blk, 0, s, op, x, b, y, nbu); break;
+    case 5: hipLaunchKernelGGL((op_apply_kernel<T, 5>), g, blk, 0, s, op, x, b, y, nbu); break;
+    case 6: hipLaunchKernelGGL((op_apply_kernel<T, 6>), g, blk, 0, s, op, x, b, y, nbu); break;
+    case 7: hipLaunchKernelGGL((op_apply_kernel<T, 7>), g, blk, 0, s, op, x, b, y, nbu); break;
+    case 8: hipLaunchKernelGGL((op_apply_kernel<T, 8>), g, blk, 0, s, op, x, b, y, nbu); break;
+    default: return ERR_INVALID;
+  }
+  return hipGetLastError() == hipSuccess ? OK : ERR_HIP;
+}
+template int launch_op_apply<double>(const MpetOpT<double>&, const double*, const double*,
+                                     double*, hipStream_t);
+template int launch_op_apply<float>(const MpetOpT<float>&, const float*, const float*, float*,
+                                    hipStream_t);
+
+template <typename T>
+int launch_soa_convert(const double* src, int ncomp, long long n, T* dst, hipStream_t s) {
+  int blocks = (int)((n + 255) / 256);
+  if (ncomp == 1)
+    hipLaunchKernelGGL((copy_convert_kernel<T>), dim3(blocks), dim3(256), 0, s, src, n, dst);
+  else
+    hipLaunchKernelGGL((soa_convert_kernel<T>), dim3(blocks), dim3(256), 0, s, src, ncomp, n,
+                       dst);
+  return hipGetLastError() == hipSuccess ? OK : ERR_HIP;
+}
+template int launch_soa_convert<double>(const double*, int, long long, double*, hipStream_t);
+template int launch_soa_convert<float>(const double*, int, long long, float*, hipStream_t);
+
+}  // namespace mpet

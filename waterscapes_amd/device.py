@@ -1,0 +1,201 @@
+"""Device-resident MPET system of one instance: mesh tables, assembled
+operators (FP64 primitives, structured FP64/FP32 copies, optional scalar
+CSR), Dirichlet masks and the per-step right-hand side.
+
+Everything lives in HBM as torch tensors; kernels are launched through the
+C ABI (include/mpet_hip.h) on torch's current stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, ptr
+from .fem import MixedSpace, cell_colouring
+from . import forms
+
+
+def _dev(a, dtype, device):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(device)
+
+
+class DeviceSystem:
+    """Assembled total-pressure system of one MPET instance on one GPU."""
+
+    def __init__(self, space: MixedSpace, params, dt, theta, K_cells=None, device="cuda",
+                 build_csr=False, need_fp32=True):
+        if not torch.cuda.is_available():
+            raise RuntimeError("DeviceSystem needs a GPU (no CPU fallback)")
+        self.device = torch.device(device)
+        _lib.ensure_tables(self.device.index or 0)
+        self.space = space
+        self.A = A = space.A
+        self.F = A + 1
+        self.nP2, self.nP1 = space.nP2, space.nP1
+        self.n = space.ndofs
+        self.nu = space.nu
+        mesh = space.mesh
+        dm = space.dofmap
+        self.mu, self.lmbda = forms.lame(params["E"], params["nu"])
+        alpha, c, S = params["alpha"], params["c"], params["S"]
+        self.dt, self.theta = float(dt), float(theta)
+        self.Cm, self.Cs = forms.matrix_coefficients(alpha, c, S, self.lmbda, dt, theta)
+        self.Bm, self.Bs = forms.rhs_coefficients(alpha, c, S, self.lmbda, dt, theta)
+        self.Pd, self.Ps = forms.preconditioner_coefficients(alpha, c, S, self.lmbda, dt, theta)
+        dev = self.device
+        i32, f64 = torch.int32, torch.float64
+        # mesh tables
+        self.coords = _dev(mesh.coordinates(), f64, dev)
+        self.cells = _dev(dm.cell_p1, i32, dev)
+        self.cell_p2 = _dev(dm.cell_p2, i32, dev)
+        perm, offs = cell_colouring(mesh)
+        self.colour_cells = _dev(perm, i32, dev)
+        self.colour_offsets = np.ascontiguousarray(offs, dtype=np.int64)
+        if K_cells is None:
+            K_cells = np.tile(np.asarray([float(k) for k in params["K"]]), (mesh.num_cells(), 1))
+        self.Kcell = _dev(np.asarray(K_cells, dtype=np.float64).reshape(mesh.num_cells(), A),
+                          f64, dev)
+        self.mesh_desc = _lib.MeshDesc(
+            mesh.num_vertices(), mesh.num_cells(), self.nP2, self.nP1, ptr(self.coords),
+            ptr(self.cells), ptr(self.cell_p2), ptr(self.colour_cells),
+            self.colour_offsets.ctypes.data, len(offs) - 1)
+        # node patterns
+        self.pat_host = {}
+        self.pat_dev = {}
+        for kind in ("22", "21", "12", "11"):
+            rp, ci = dm.pattern(kind)
+            if rp[-1] >= 2**31:
+                raise ValueError("pattern too large for int32 indexing")
+            self.pat_host[kind] = (rp, ci)
+            self.pat_dev[kind] = (_dev(rp.astype(np.int32), i32, dev), _dev(ci, i32, dev))
+        p = self.pat_dev
+        self.pat = _lib.Patterns(ptr(p["22"][0]), ptr(p["22"][1]), ptr(p["21"][0]),
+                                 ptr(p["21"][1]), ptr(p["12"][0]), ptr(p["12"][1]),
+                                 ptr(p["11"][0]), ptr(p["11"][1]),
+                                 int(self.pat_host["22"][1].size), int(self.pat_host["21"][1].size),
+                                 int(self.pat_host["12"][1].size), int(self.pat_host["11"][1].size))
+        self.nnz = {k: int(v[1].size) for k, v in self.pat_host.items()}
+        self.dmask = torch.zeros(self.n, dtype=torch.uint8, device=dev)
+        self._assemble()
+        self.op64 = self._structured(0)
+        self.op32 = self._structured(1) if need_fp32 else None
+        self.csr = self.expand_csr(0) if build_csr else None
+
+    # -- assembly --------------------------------------------------------------
+    def _assemble(self):
+        dev = self.device
+        f64 = torch.float64
+        nz = self.nnz
+        self.prim_t = {
+            "Kuu": torch.empty(9 * nz["22"], dtype=f64, device=dev),
+            "Lap": torch.empty(nz["22"], dtype=f64, device=dev),
+            "Dv": torch.empty(3 * nz["21"], dtype=f64, device=dev),
+            "Dt": torch.empty(3 * nz["12"], dtype=f64, device=dev),
+            "M1": torch.empty(nz["11"], dtype=f64, device=dev),
+            "St": torch.empty(self.A * nz["11"], dtype=f64, device=dev),
+        }
+        t = self.prim_t
+        self.prim = _lib.Primitives(ptr(t["Kuu"]), ptr(t["Lap"]), ptr(t["Dv"]), ptr(t["Dt"]),
+                                    ptr(t["M1"]), ptr(t["St"]))
+        check(_lib.lib().mpet_assemble_primitives(
+            C.byref(self.mesh_desc), C.byref(self.pat), ptr(self.Kcell), self.A, self.mu,
+            C.byref(self.prim), _lib.stream_ptr()))
+
+    def _structured(self, dtype):
+        """SoA copy of the primitives in float64 (dtype 0) or float32 (1)."""
+        tdt = torch.float64 if dtype == 0 else torch.float32
+        L = _lib.lib()
+        s = _lib.stream_ptr()
+        out = {}
+        for name, ncomp in (("Kuu", 9), ("Dv", 3), ("Dt", 3), ("M1", 1)):
+            src = self.prim_t[name]
+            dst = torch.empty_like(src, dtype=tdt)
+            check(L.mpet_soa_convert(dtype, ptr(src), ncomp, src.numel() // ncomp, ptr(dst), s))
+            out[name] = dst
+        out["St"] = self.prim_t["St"].to(tdt)
+        return out
+
+    def op_desc(self, dtype, masked=True):
+        vals = self.op64 if dtype == 0 else self.op32
+        coef = _lib.make_coef(self.Cm, self.Cs, self.mu)
+        return _lib.OpDesc(self.nP2, self.nP1, self.A, self.pat, dtype, ptr(vals["Kuu"]),
+                           ptr(vals["Dv"]), ptr(vals["Dt"]), ptr(vals["M1"]), ptr(vals["St"]),
+                           coef, ptr(self.dmask) if masked else None)
+
+    def row_lengths(self, which):
+        rp = {k: np.diff(v[0]) for k, v in self.pat_host.items()}
+        F = self.F
+        lu = (3 * rp["22"] + (rp["21"] if which == 0 else 0))
+        lens_u = np.repeat(lu, 3)
+        lp = np.empty((self.nP1, F), dtype=np.int64)
+        if which == 0:
+            lp[:, :] = F * rp["11"][:, None]
+            lp[:, 0] += 3 * rp["12"]
+        else:
+            lp[:, :] = rp["11"][:, None]
+        return np.concatenate([lens_u, lp.reshape(-1)])
+
+    def expand_csr(self, which):
+        """Scalar CSR (rowptr int64, cols int32, vals f64) of the
+        un-eliminated A (which=0) or P (which=1)."""
+        lens = self.row_lengths(which)
+        rowptr = np.zeros(self.n + 1, dtype=np.int64)
+        np.cumsum(lens, out=rowptr[1:])
+        nnz = int(rowptr[-1])
+        if nnz >= 2**31:
+            raise ValueError("scalar CSR too large for int32 column indices")
+        dev = self.device
+        rp = _dev(rowptr, torch.int64, dev)
+        cols = torch.empty(nnz, dtype=torch.int32, device=dev)
+        vals = torch.empty(nnz, dtype=torch.float64, device=dev)
+        if which == 0:
+            coef = _lib.make_coef(self.Cm, self.Cs, self.mu)
+        else:
+            coef = _lib.make_coef(np.diag(self.Pd), self.Ps, self.mu)
+        check(_lib.lib().mpet_expand_csr(C.byref(self.pat), C.byref(self.prim), self.nP2,
+                                         self.nP1, self.A, C.byref(coef), which, ptr(rp),
+                                         ptr(cols), ptr(vals), _lib.stream_ptr()))
+        return rp, cols, vals
+
+    # -- Dirichlet data ----------------------------------------------------------
+    def set_dirichlet(self, dofs):
+        m = torch.zeros(self.n, dtype=torch.uint8)
+        if len(dofs):
+            m[torch.as_tensor(np.asarray(dofs, dtype=np.int64))] = 1
+        self.dmask.copy_(m.to(self.device))
+
+    # -- operator applications ----------------------------------------------------
+    def apply(self, x, b=None, dtype=0, masked=True):
+        y = torch.empty_like(x)
+        d = self.op_desc(dtype, masked)
+        check(_lib.lib().mpet_op_apply(C.byref(d), ptr(x), ptr(b), ptr(y), _lib.stream_ptr()))
+        return y
+
+    def csr_apply(self, x, b=None, lpr=16, masked=True, csr=None):
+        rp, ci, vals = csr if csr is not None else self.csr
+        y = torch.empty_like(x)
+        check(_lib.lib().mpet_csr_spmv(0, self.n, ptr(rp), ptr(ci), ptr(vals), ptr(x), ptr(b),
+                                       ptr(self.dmask) if masked else None, ptr(y), lpr,
+                                       _lib.stream_ptr()))
+        return y
+
+    def cell_source(self, test_degree, Q, fval, ncomp, scale, dof_base, dof_stride, dof_off, b):
+        """b += scale * int f_h . v over all cells (f_h: per-cell P_k nodal
+        values fval (nC*nk*ncomp), Q: reference weights (ntest, nk))."""
+        test = self.cell_p2 if test_degree == 2 else self.cells
+        ntest = 10 if test_degree == 2 else 4
+        nk = Q.shape[1]
+        check(_lib.lib().mpet_cell_source(C.byref(self.mesh_desc), ptr(test), ntest, ptr(Q), nk,
+                                          ptr(fval), ncomp, float(scale), int(dof_base),
+                                          int(dof_stride), int(dof_off), ptr(b),
+                                          _lib.stream_ptr()))
+
+    def rhs_pressure(self, up_prev, b):
+        coef = _lib.make_coef(self.Bm, self.Bs, self.mu)
+        check(_lib.lib().mpet_rhs_pressure(C.byref(self.pat), C.byref(self.prim), self.nP2,
+                                           self.nP1, self.A, C.byref(coef), ptr(up_prev),
+                                           ptr(b), _lib.stream_ptr()))
+        return b

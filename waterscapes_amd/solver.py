@@ -1,0 +1,343 @@
+"""MPETTotalPressureSolver: drop-in for mpet/mpettotalpressuresolver.py on one
+MI355X.
+
+Same constructor, parameters (default_params, :146-162), attributes (F, L0,
+L1, L2, prec, up_, up, solver_monitor, params) and generator protocol
+(solve() yields (up, t) per step and updates up_ after the yield,
+:497-502).  The forms of :164-328 are not built symbolically: F, L0, L1, L2
+and prec are descriptors of the fixed forms of the total-pressure
+formulation, whose element tensors the HIP assembly kernel evaluates.
+
+Per step, on the GPU (StepEngine.step):
+  b  = rhs(F)(up_)             at t_n        (:461)   one P1-pattern kernel
+  b += dt g_i, dt I_i, Robin   at t_theta    (:470-476)
+  b += f, s                    at t_{n+1}    (:482-483)
+  x0 = up_ with the Dirichlet values at t_{n+1} in place   (:487-488)
+  x  = MINRES(A_elim, P_elim)(b_elim)        (:489-494)
+The symmetric elimination of :487-489 is folded into the operators (rows
+masked, lifting = the initial residual), so A is never copied (:457).
+
+direct_solver=True (the reference default, LU with MUMPS, :330-411) has no
+sparse LU on the GPU: it runs the same FP64 Krylov solve to rtol 1e-12,
+which reproduces the LU solution to round-off level; row replacement and
+symmetric elimination give the same solution (tested on the oracle).
+"""
+from __future__ import annotations
+
+import time as _time
+
+import numpy as np
+import torch
+
+from . import forms, refel
+from .device import DeviceSystem
+from .fem import Constant, Function, MixedSpace
+from .krylov import KrylovSolver, SolverConfig
+from .problem import resolved_params
+
+NEUMANN_MARKER = 1
+ROBIN_MARKER = 2
+
+
+class Parameters(dict):
+    """dolfin.Parameters look-alike: a dict with add()/update()."""
+
+    def __init__(self, name="", **kw):
+        super().__init__(**kw)
+        self.name = name
+
+    def add(self, key, value):
+        self[key] = value
+
+
+class FormDescriptor:
+    """Stand-in for a UFL form of the total-pressure formulation."""
+
+    def __init__(self, name, **info):
+        self.name = name
+        self.info = info
+
+    def __repr__(self):
+        return f"<form {self.name}>"
+
+
+class DirichletBC:
+    """DirichletBC(V.sub(i), value, markers, 0): the DOFs in the closure of
+    facets marked 0 and their nodal values at the time of apply."""
+
+    def __init__(self, space, field, value, markers, marker_value=0):
+        if marker_value != 0:
+            raise ValueError("only marker 0 denotes Dirichlet conditions")
+        self.space = space
+        self.field = field
+        self.value = value
+        nodes = forms.dirichlet_nodes(space, markers, field)
+        self.nodes = nodes
+        self.dofs = forms.field_dofs(space, field, nodes)
+        dm = space.dofmap
+        self.coords = dm.x2[nodes] if field == 0 else dm.x1[nodes]
+
+    def values(self, t):
+        if self.dofs.size == 0:
+            return np.zeros(0)
+        v = self.value.eval(self.coords, t)
+        if self.field == 0:
+            return np.broadcast_to(v, (self.nodes.size, 3)).reshape(-1)
+        return np.broadcast_to(v, (self.nodes.size,)).astype(np.float64)
+
+    def get_boundary_values(self, t=None):
+        return dict(zip(self.dofs.tolist(), self.values(t).tolist()))
+
+
+class _Source:
+    """A volume source interpolated into P_k on each cell (Expression
+    semantics) and integrated against P2 (u rows) or P1 (network rows)."""
+
+    def __init__(self, engine, expr, test_degree, ncomp, scale, dof_base, dof_stride, dof_off):
+        self.e = engine
+        self.expr = expr
+        self.k = int(getattr(expr, "degree", 0))
+        mesh = engine.space.mesh
+        x = mesh.coordinates()
+        c = mesh.cells()
+        bary = refel.pk_nodes_bary(self.k)                           # nk,4
+        self.X = np.einsum("nk,ckd->cnd", bary, x[c])                # nC,nk,3
+        self.Q = torch.as_tensor(refel.source_weights(test_degree, self.k)).to(engine.device)
+        self.test_degree = test_degree
+        self.ncomp = ncomp
+        self.scale = scale
+        self.dof = (dof_base, dof_stride, dof_off)
+        self._cache = None
+
+    def add(self, b, t):
+        if self._cache is None or self.expr.depends_on_time():
+            v = self.expr.eval(self.X.reshape(-1, 3), t)
+            v = np.broadcast_to(np.asarray(v, dtype=np.float64),
+                                (self.X.shape[0] * self.X.shape[1],) + ((self.ncomp,) if self.ncomp > 1 else ()))
+            self._cache = torch.as_tensor(np.array(v, dtype=np.float64)).to(self.e.device)
+        self.e.system.cell_source(self.test_degree, self.Q, self._cache, self.ncomp, self.scale,
+                                  *self.dof, b)
+
+
+def _is_zero(e):
+    return bool(getattr(e, "is_zero", lambda: False)())
+
+
+class StepEngine:
+    """Device-resident state of one MPET instance and its time step."""
+
+    def __init__(self, problem, space, dt, theta, config: SolverConfig, device="cuda"):
+        self.problem = problem
+        self.space = space
+        self.device = torch.device(device)
+        self.dt, self.theta = float(dt), float(theta)
+        params, cellK = resolved_params(problem)
+        self.params = params
+        A = space.A
+        for j in range(A):
+            for name, lst in (("I", problem.I), ("beta", problem.beta)):
+                if not _is_zero(lst[j]):
+                    raise NotImplementedError(
+                        f"facet terms ({name}) are not supported by the GPU backend yet")
+        if not _is_zero(problem.s):
+            raise NotImplementedError("traction s is not supported by the GPU backend yet")
+        if problem.u_has_nullspace or any(problem.p_has_nullspace):
+            raise NotImplementedError("nullspace (rigid-motion / constant) handling")
+        self.system = DeviceSystem(space, params, dt, theta, K_cells=cellK, device=device,
+                                   build_csr=(config.mode == "fp64" and config.use_csr is not False))
+        # Dirichlet conditions (create_dirichlet_bcs, :123-144)
+        self.bcs = [DirichletBC(space, 0, problem.u_bar, problem.momentum_boundary_markers, 0)]
+        for i in range(A):
+            self.bcs.append(DirichletBC(space, i + 2, problem.p_bar[i],
+                                        problem.continuity_boundary_markers[i], 0))
+        self.bc_dofs_host = np.concatenate([bc.dofs for bc in self.bcs]).astype(np.int64)
+        self.bc_dofs = torch.as_tensor(self.bc_dofs_host).to(self.device)
+        self.krylov = KrylovSolver(self.system, [bc.dofs for bc in self.bcs], config)
+        self.config = config
+        # sources
+        nu, F = space.nu, space.A + 1
+        self.f_src = None
+        if not problem.f.is_zero():
+            self.f_src = _Source(self, problem.f, 2, 3, 1.0, 0, 3, 0)
+        self.g_src = []
+        for i in range(A):
+            gi = problem.g[i]
+            if not gi.is_zero():
+                self.g_src.append(_Source(self, gi, 1, 1, self.dt, nu, F, i + 1))
+        self.b = torch.empty(space.ndofs, dtype=torch.float64, device=self.device)
+        self.g_full = torch.zeros(space.ndofs, dtype=torch.float64, device=self.device)
+        self.stats = []
+
+    def bc_values(self, t=None):
+        return np.concatenate([bc.values(t) for bc in self.bcs]) if self.bcs else np.zeros(0)
+
+    def assemble_rhs(self, up_prev, t_n):
+        """b at the reference's time levels (:461-483), moving the problem's
+        time Constant through t_theta to t_{n+1} exactly as :467-479 do, so
+        Expressions bound to it see the same values.  Returns t_{n+1}."""
+        b = self.b
+        time = self.problem.time
+        self.system.rhs_pressure(up_prev, b)
+        t_theta = float(t_n) + self.theta * float(self.dt)
+        time.assign(t_theta)
+        for src in self.g_src:
+            src.add(b, None)
+        t_np1 = float(time) + (1.0 - self.theta) * float(self.dt)
+        time.assign(t_np1)
+        if self.f_src is not None:
+            self.f_src.add(b, None)
+        return t_np1
+
+    def step(self, up_prev, x, t_n, testing_guess=False):
+        """One implicit step from up_prev (device FP64) into x; returns t_{n+1}."""
+        t_np1 = self.assemble_rhs(up_prev, t_n)
+        g = torch.as_tensor(self.bc_values(None)).to(self.device)
+        if not testing_guess:
+            x.copy_(up_prev)
+        if self.bc_dofs.numel():
+            x.index_copy_(0, self.bc_dofs, g)
+        bnorm = 0.0
+        if self.config.mode == "mixed":
+            gf = self.g_full
+            gf.zero_()
+            if self.bc_dofs.numel():
+                gf.index_copy_(0, self.bc_dofs, g)
+            lifted = self.system.apply(gf, b=self.b)          # Pi (b - A g)
+            bnorm = float(torch.sqrt(torch.dot(lifted, lifted) + torch.dot(g, g)))
+        st = self.krylov.solve(self.b, x, bnorm=bnorm)
+        self.stats.append(st)
+        return t_np1, st
+
+
+class MPETTotalPressureSolver(object):
+    """Total-pressure MPET solver (mpet/mpettotalpressuresolver.py:15-512).
+
+    Backend parameters added to default_params: "precision" ("fp64" or
+    "mixed"), "krylov_rtol", "outer_tol", "inner_rtol", "maxit",
+    "p0_weight" ("robust" / "reference"), "device"."""
+
+    def __init__(self, problem, params=None):
+        self.problem = problem
+        self.monitor = False
+        self.solver_monitor = {}
+        self.params = self.default_params()
+        if params is not None:
+            self.params.update(params)
+        F, L0, L1, L2, prec, up_, up = self.create_variational_forms()
+        self.F = F
+        self.L0 = L0
+        self.L1 = L1
+        self.L2 = L2
+        self.prec = prec
+        self.up_ = up_
+        self.up = up
+        self._engine = None
+
+    @staticmethod
+    def default_params():
+        """mpet/mpettotalpressuresolver.py:146-162 plus backend keys."""
+        params = Parameters("MPETSolver")
+        params.add("dt", 0.05)
+        params.add("t", 0.0)
+        params.add("T", 1.0)
+        params.add("theta", 0.5)
+        params.add("u_degree", 2)
+        params.add("p_degree", 1)
+        params.add("direct_solver", True)
+        params.add("testing", False)
+        params.add("precision", "fp64")
+        params.add("krylov_rtol", 1e-10)
+        params.add("outer_tol", 1e-8)
+        params.add("inner_rtol", 1e-4)
+        params.add("maxit", 5000)
+        params.add("p0_weight", "robust")
+        params.add("device", "cuda")
+        return params
+
+    def create_dirichlet_bcs(self):
+        """[bcs0, bcs1] as in :123-144 (DirichletBC look-alikes)."""
+        VP = self.up.function_space()
+        bcs0 = [DirichletBC(VP, 0, self.problem.u_bar, self.problem.momentum_boundary_markers, 0)]
+        bcs1 = []
+        for i in range(int(self.problem.params["J"])):
+            bcs1 += [DirichletBC(VP, i + 2, self.problem.p_bar[i],
+                                 self.problem.continuity_boundary_markers[i], 0)]
+        return [bcs0, bcs1]
+
+    def create_variational_forms(self):
+        if self.params["u_degree"] != 2 or self.params["p_degree"] != 1:
+            raise NotImplementedError("the GPU backend implements P2 displacement / P1 pressures")
+        mesh = self.problem.mesh
+        A = int(self.problem.params["J"])
+        VW = MixedSpace(mesh, A)
+        dev = torch.device(self.params["device"])
+        up_ = Function(VW, torch.zeros(VW.ndofs, dtype=torch.float64, device=dev))
+        up = Function(VW, torch.zeros(VW.ndofs, dtype=torch.float64, device=dev))
+        F = FormDescriptor("F", terms=":267-274")
+        L0 = FormDescriptor("L0", terms=":313")
+        L1 = [FormDescriptor(f"L1[{i}]", terms=":322") for i in range(A)]
+        L2 = [FormDescriptor(f"L2[{i}]", terms=":324") for i in range(A)]
+        prec = FormDescriptor("prec", terms=":279-283") if not self.params["direct_solver"] else 0
+        return F, L0, L1, L2, prec, up_, up
+
+    def _config(self, direct):
+        p = self.params
+        if direct:
+            return SolverConfig(mode="fp64", rtol=1e-12, maxit=int(p["maxit"]),
+                                p0_weight=p["p0_weight"])
+        return SolverConfig(mode=p["precision"], rtol=float(p["krylov_rtol"]),
+                            outer_tol=float(p["outer_tol"]), inner_rtol=float(p["inner_rtol"]),
+                            maxit=int(p["maxit"]), p0_weight=p["p0_weight"])
+
+    def engine(self, direct=None):
+        direct = self.params["direct_solver"] if direct is None else direct
+        if self._engine is None:
+            self._engine = StepEngine(self.problem, self.up_.function_space(),
+                                      self.params["dt"], self.params["theta"],
+                                      self._config(direct), self.params["device"])
+        return self._engine
+
+    def _run(self, direct):
+        dt = self.params["dt"]
+        T = self.params["T"]
+        time = self.problem.time
+        eng = self.engine(direct)
+        self.solver_monitor["niter"] = []
+        self.solver_monitor["outer"] = []
+        self.solver_monitor["residual"] = []
+        self.solver_monitor["step_seconds"] = []
+        x = self.up.data
+        testing = bool(self.params["testing"]) and not direct
+        if testing:
+            x.copy_(torch.randn(x.shape[0], dtype=torch.float64, device=x.device))
+        else:
+            self.up.assign(self.up_)
+        while float(time) < (T - 1.0e-9):
+            t0 = _time.perf_counter()
+            t = eng.step(self.up_.data, x, float(time), testing_guess=testing)[0]
+            testing = False
+            torch.cuda.synchronize()
+            st = eng.stats[-1]
+            self.solver_monitor["niter"] += [int(st.inner_iterations)]
+            self.solver_monitor["outer"] += [int(st.outer_iterations)]
+            self.solver_monitor["residual"] += [float(st.residual_norm)]
+            self.solver_monitor["step_seconds"] += [_time.perf_counter() - t0]
+            yield self.up, float(time)
+            self.up_.assign(self.up)
+            time.assign(t)
+        self.solver_monitor["P"] = eng.krylov
+        self.solver_monitor["A"] = eng.system
+
+    def solve_direct(self):
+        """Reference :330-411 (see module docstring for the GPU realisation)."""
+        return self._run(True)
+
+    def solve_iterative(self):
+        """Reference :414-505."""
+        return self._run(False)
+
+    def solve(self):
+        if self.params["direct_solver"]:
+            return self.solve_direct()
+        else:
+            return self.solve_iterative()
