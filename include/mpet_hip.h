@@ -97,6 +97,7 @@ typedef struct {
   const void *Kuu, *Dv, *Dt, *M1, *St;
   MpetCoef coef;
   const uint8_t* dmask;
+  int32_t lanes_u, lanes_p;   /* lanes per P2 / P1 node row: 4/8/16/32, 0 = 16 */
 } MpetOpDesc;
 
 /* One level of the displacement-block hierarchy (device arrays, int32
@@ -115,25 +116,25 @@ typedef struct {
 
 #define MPET_MAXLEV 16
 
-/* Block-diagonal preconditioner of prec (:279-283) with Dirichlet rows and
- * columns eliminated (apply_symmetric(bc, P), :445-446). */
+/* One AMG hierarchy (SA setup, Chebyshev smoothing, dense coarsest solve). */
 typedef struct {
   int32_t nlev;
   MpetAmgLevelDesc lev[MPET_MAXLEV];
-  const void* coarse_inv;        /* dense n_coarse^2 */
+  const void* coarse_inv;        /* dense n_coarse^2, row-major */
   int32_t degree;                /* Chebyshev degree of the smoother */
   double eig_ratio, eig_safety;  /* interval [safety*rho/ratio, safety*rho] */
-  double inv_mu;                 /* the hierarchy is built on grad:grad */
-  const uint8_t* mask0;          /* level-0 rows (P2 nodes) that are Dirichlet */
-  /* pressure blocks: field f matrix = Pd[f] M1 + Ps[f] St[f-1] */
+  double scale;                  /* V-cycle approximates (scale * A0)^-1 */
+  const uint8_t* mask0;          /* level-0 rows that are Dirichlet (may be NULL) */
+} MpetHierDesc;
+
+/* Block-diagonal preconditioner of prec (:279-283) with Dirichlet rows and
+ * columns eliminated (apply_symmetric(bc, P), :445-446):
+ *   u: hierarchy on the scalar P2 Laplacian, 3 components at once, scale mu;
+ *   p: one hierarchy on the block-diagonal pressure matrix diag(P_0..P_A)
+ *      in field-major order (F*nP1 unknowns). */
+typedef struct {
+  MpetHierDesc u, p;
   int32_t nP1, F;
-  const int32_t *rp11, *ci11;
-  long long nnz11;
-  const void *M1, *St, *dinvp;   /* dinvp: F*nP1 inverse diagonal */
-  const uint8_t* pmask;          /* F*nP1 Dirichlet flags */
-  double Pd[MPET_MAXF], Ps[MPET_MAXF];
-  int32_t pdeg;
-  double plmin[MPET_MAXF], plmax[MPET_MAXF];
 } MpetPrecDesc;
 
 typedef struct {
@@ -145,17 +146,20 @@ typedef struct {
 } MpetCsrDesc;
 
 typedef struct {
-  int32_t mode;         /* 0: FP64 MINRES; 1: FP32 MINRES in FP64 refinement */
+  int32_t mode;         /* 0: FP64 MINRES; 1: FP32 MINRES in FP64 refinement;
+                           2: FP64 MINRES with the FP32 preconditioner */
   int32_t use_csr;      /* mode 0: use the scalar CSR A instead of the structured one */
-  double rtol;          /* mode 0: relative preconditioned residual */
+  double rtol;          /* modes 0, 2: ||r||_{M^-1} <= rtol ||b_I||_{M^-1} */
   int32_t maxit;        /* per MINRES solve */
   int32_t check_every;  /* host reads the device state every k iterations */
   int32_t use_graphs;   /* replay k-iteration chunks from a captured hipGraph */
-  double outer_tol;     /* mode 1: ||Pi(b - A x)|| <= outer_tol * bnorm */
+  double outer_tol;     /* mode 1: ||r||_{M^-1} <= outer_tol ||b_I||_{M^-1}, r = Pi(b - A x) in FP64 */
   double inner_rtol;    /* mode 1: FP32 MINRES relative tolerance */
   int32_t max_outer;
 } MpetSolverOptions;
 
+/* residual_norm: final ||r||_{M^-1} / ||b_I||_{M^-1} (preconditioned norms;
+ * b_I = Pi (b - A g), the lifted right-hand side of the free rows). */
 typedef struct {
   int32_t inner_iterations, outer_iterations, converged;
   double residual_norm;
@@ -217,6 +221,12 @@ int mpet_csr_spmv(int dtype, long long nrows, const long long* rp, const int* ci
 int mpet_soa_convert(int dtype, const double* src, int ncomp, long long n, void* dst,
                      hipStream_t stream);
 
+/* Node-pair sparsity pattern (host arrays) of all (r, c) sharing a cell:
+ * the sparsity DOLFIN builds before `assemble(a)` (:437).  Call once with
+ * cols == NULL to fill rowptr (nrow + 1), then with cols sized rowptr[nrow]. */
+int mpet_build_pattern(int ncell, int nr, const int32_t* rmap, int nc, const int32_t* cmap,
+                       int nrow, int64_t* rowptr, int32_t* cols);
+
 /* Smoothed-aggregation setup (host arrays, FP64) of the displacement-block
  * hierarchy: replaces hypre BoomerAMG's setup inside
  * PETScKrylovSolver("minres", "hypre_amg") (:448).  P0 optional (n1 > 0):
@@ -242,7 +252,7 @@ void mpet_amg_destroy(void* h);
  * (or CSR) operator — PETScKrylovSolver("minres","hypre_amg") (:448).
  * Borrows every device array of the descriptors (caller keeps them alive);
  * owns its workspaces.  op32/prec32 are needed for mode 1, prec64 for
- * mode 0. */
+ * mode 0, prec32 for mode 2. */
 int mpet_solver_create(const MpetOpDesc* op64, const MpetOpDesc* op32,
                        const MpetCsrDesc* csr64, const MpetPrecDesc* prec64,
                        const MpetPrecDesc* prec32, const MpetSolverOptions* opt,
@@ -256,9 +266,6 @@ int mpet_solver_destroy(void* h);
  * Returns MPET_ERR_NOT_CONVERGED (stats filled) when maxit is reached. */
 int mpet_solver_solve(void* h, const double* b, double* x, MpetSolveStats* stats,
                       hipStream_t stream);
-
-/* Reference norm for the refinement test (||b_elim||); 0 = use ||r_0||. */
-int mpet_solver_set_bnorm(void* h, double bnorm);
 
 /* z = P^-1 r with the FP64 (dtype 0) or FP32 (1) preconditioner. */
 int mpet_precond_apply(void* h, int dtype, const void* r, void* z, hipStream_t stream);

@@ -735,7 +735,7 @@ class OracleProblem:
         A = self.A_mat.copy()
         alld = np.concatenate(self.bc_dofs)
         A = apply_row_replace(A, alld)
-        lu = spla.splu(A.tocsc())
+        lu = spla.splu(A.tocsc(), permc_spec="MMD_AT_PLUS_A")
         up_prev = up0.copy()
         t = self.t0
         while t < self.T - 1e-9:

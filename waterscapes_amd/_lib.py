@@ -47,7 +47,7 @@ class OpDesc(C.Structure):
     _fields_ = [("nP2", C.c_int32), ("nP1", C.c_int32), ("A", C.c_int32), ("pat", Patterns),
                 ("dtype", C.c_int32), ("Kuu", C.c_void_p), ("Dv", C.c_void_p),
                 ("Dt", C.c_void_p), ("M1", C.c_void_p), ("St", C.c_void_p), ("coef", Coef),
-                ("dmask", C.c_void_p)]
+                ("dmask", C.c_void_p), ("lanes_u", C.c_int32), ("lanes_p", C.c_int32)]
 
 
 def make_coef(Cm, stiff, mu=0.0):

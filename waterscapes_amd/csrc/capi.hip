@@ -19,6 +19,8 @@ static MpetOpT<T> make_op(const MpetOpDesc* d) {
   for (int i = 0; i < MPET_MAXF * MPET_MAXF; ++i) op.C[i] = d->coef.C[i];
   for (int i = 0; i < MPET_MAXF; ++i) op.stiff[i] = d->coef.stiff[i];
   op.dmask = d->dmask;
+  op.lanes_u = d->lanes_u;
+  op.lanes_p = d->lanes_p;
   return op;
 }
 

@@ -19,6 +19,7 @@ struct MpetOpT {
   double C[MPET_MAXF * MPET_MAXF];
   double stiff[MPET_MAXF];
   const uint8_t* dmask;   // per DOF, 1 = Dirichlet (may be null)
+  int lanes_u, lanes_p;   // lanes per P2 / P1 node row (0 = default)
 };
 
 template <typename T>

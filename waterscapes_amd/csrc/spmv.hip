@@ -41,67 +41,66 @@ csr_spmv_kernel(long long nrows, const long long* __restrict__ rp, const int* __
 }
 
 // ---------------------------------------------------------------------------
-// structured operator
+// structured operator: GU lanes per P2 node row, GP lanes per P1 node row,
+// 256-thread blocks; the first nbu blocks take u rows, the rest p rows.
 // ---------------------------------------------------------------------------
-constexpr int OP_G = 16;             // lanes per node row
-constexpr int OP_ROWS = 256 / OP_G;  // node rows per block
+template <int G, typename T>
+__device__ __forceinline__ T op_allsum(T v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
 
-template <typename T, int F>
+template <typename T, int F, int GU, int GP>
 __global__ void __launch_bounds__(256)
 op_apply_kernel(MpetOpT<T> op, const T* __restrict__ x, const T* __restrict__ b,
                 T* __restrict__ y, int nbu) {
-  const int lane = threadIdx.x % OP_G;
   const long long nu = 3ll * op.nP2;
   if ((int)blockIdx.x < nbu) {
-    int a = blockIdx.x * OP_ROWS + threadIdx.x / OP_G;
+    const int lane = threadIdx.x % GU;
+    const int a = blockIdx.x * (256 / GU) + threadIdx.x / GU;
     if (a >= op.nP2) return;
     T y0 = 0, y1 = 0, y2 = 0;
-    const int kb = op.rp22[a], ke = op.rp22[a + 1];
     const long long n22 = op.nnz22;
-    for (int k = kb + lane; k < ke; k += OP_G) {
+    for (int k = op.rp22[a] + lane; k < op.rp22[a + 1]; k += GU) {
       const int cb = 3 * op.ci22[k];
       const T x0 = x[cb], x1 = x[cb + 1], x2 = x[cb + 2];
       y0 += op.Kuu[k] * x0 + op.Kuu[n22 + k] * x1 + op.Kuu[2 * n22 + k] * x2;
       y1 += op.Kuu[3 * n22 + k] * x0 + op.Kuu[4 * n22 + k] * x1 + op.Kuu[5 * n22 + k] * x2;
       y2 += op.Kuu[6 * n22 + k] * x0 + op.Kuu[7 * n22 + k] * x1 + op.Kuu[8 * n22 + k] * x2;
     }
-    const int qb = op.rp21[a], qe = op.rp21[a + 1];
     const long long n21 = op.nnz21;
-    for (int k = qb + lane; k < qe; k += OP_G) {
+    for (int k = op.rp21[a] + lane; k < op.rp21[a + 1]; k += GU) {
       const T p0 = x[nu + (long long)F * op.ci21[k]];
       y0 += op.Dv[k] * p0;
       y1 += op.Dv[n21 + k] * p0;
       y2 += op.Dv[2 * n21 + k] * p0;
     }
-    y0 = group_sum<OP_G>(y0);
-    y1 = group_sum<OP_G>(y1);
-    y2 = group_sum<OP_G>(y2);
-    if (lane == 0) {
-      const long long o = 3ll * a;
-      if (b) { y0 = b[o] - y0; y1 = b[o + 1] - y1; y2 = b[o + 2] - y2; }
-      if (op.dmask) {
-        if (op.dmask[o]) y0 = 0;
-        if (op.dmask[o + 1]) y1 = 0;
-        if (op.dmask[o + 2]) y2 = 0;
-      }
-      y[o] = y0; y[o + 1] = y1; y[o + 2] = y2;
+    y0 = op_allsum<GU>(y0);
+    y1 = op_allsum<GU>(y1);
+    y2 = op_allsum<GU>(y2);
+    if (lane < 3) {
+      const long long o = 3ll * a + lane;
+      T v = lane == 0 ? y0 : (lane == 1 ? y1 : y2);
+      if (b) v = b[o] - v;
+      if (op.dmask && op.dmask[o]) v = 0;
+      y[o] = v;
     }
   } else {
-    int c = (blockIdx.x - nbu) * OP_ROWS + threadIdx.x / OP_G;
+    const int lane = threadIdx.x % GP;
+    const int c = (blockIdx.x - nbu) * (256 / GP) + threadIdx.x / GP;
     if (c >= op.nP1) return;
     T yd = 0;
-    const int qb = op.rp12[c], qe = op.rp12[c + 1];
     const long long n12 = op.nnz12;
-    for (int k = qb + lane; k < qe; k += OP_G) {
+    for (int k = op.rp12[c] + lane; k < op.rp12[c + 1]; k += GP) {
       const int cb = 3 * op.ci12[k];
       yd += op.Dt[k] * x[cb] + op.Dt[n12 + k] * x[cb + 1] + op.Dt[2 * n12 + k] * x[cb + 2];
     }
     T m[F], s[F];
 #pragma unroll
     for (int g = 0; g < F; ++g) { m[g] = 0; s[g] = 0; }
-    const int sb = op.rp11[c], se = op.rp11[c + 1];
     const long long n11 = op.nnz11;
-    for (int k = sb + lane; k < se; k += OP_G) {
+    for (int k = op.rp11[c] + lane; k < op.rp11[c + 1]; k += GP) {
       const T* xp = x + nu + (long long)F * op.ci11[k];
       const T mv = op.M1[k];
 #pragma unroll
@@ -111,23 +110,21 @@ op_apply_kernel(MpetOpT<T> op, const T* __restrict__ x, const T* __restrict__ b,
         if (g >= 1) s[g] += op.St[(g - 1) * n11 + k] * xv;
       }
     }
-    yd = group_sum<OP_G>(yd);
+    yd = op_allsum<GP>(yd);
+    T mine = 0;
 #pragma unroll
     for (int g = 0; g < F; ++g) {
-      m[g] = group_sum<OP_G>(m[g]);
-      s[g] = group_sum<OP_G>(s[g]);
+      m[g] = op_allsum<GP>(m[g]);
+      T sg = op_allsum<GP>(s[g]);
+      if (lane == g) mine = (g == 0) ? yd : (T)op.stiff[g] * sg;
     }
-    if (lane == 0) {
-      const long long o = nu + (long long)F * c;
+    if (lane < F) {
+      const long long o = nu + (long long)F * c + lane;
 #pragma unroll
-      for (int f = 0; f < F; ++f) {
-        T v = (f == 0) ? yd : (T)op.stiff[f] * s[f];
-#pragma unroll
-        for (int g = 0; g < F; ++g) v += (T)op.C[f * MPET_MAXF + g] * m[g];
-        if (b) v = b[o + f] - v;
-        if (op.dmask && op.dmask[o + f]) v = 0;
-        y[o + f] = v;
-      }
+      for (int g = 0; g < F; ++g) mine += (T)op.C[lane * MPET_MAXF + g] * m[g];
+      if (b) mine = b[o] - mine;
+      if (op.dmask && op.dmask[o]) mine = 0;
+      y[o] = mine;
     }
   }
 }
@@ -174,19 +171,40 @@ template int launch_csr_spmv<float>(long long, const long long*, const int*, con
                                     const float*, const float*, const uint8_t*, float*, int,
                                     hipStream_t);
 
+template <typename T, int F, int GU>
+static void launch_op_gp(const MpetOpT<T>& op, const T* x, const T* b, T* y, int gp,
+                         hipStream_t s) {
+  int nbu = (op.nP2 + 256 / GU - 1) / (256 / GU);
+  int gpp = gp < F ? 8 : gp;
+  int nbp = (op.nP1 + 256 / gpp - 1) / (256 / gpp);
+  dim3 g(nbu + nbp), blk(256);
+  switch (gpp) {
+    case 8: hipLaunchKernelGGL((op_apply_kernel<T, F, GU, 8>), g, blk, 0, s, op, x, b, y, nbu); break;
+    case 16: hipLaunchKernelGGL((op_apply_kernel<T, F, GU, 16>), g, blk, 0, s, op, x, b, y, nbu); break;
+    default: hipLaunchKernelGGL((op_apply_kernel<T, F, GU, 32>), g, blk, 0, s, op, x, b, y, nbu); break;
+  }
+}
+
+template <typename T, int F>
+static void launch_op_f(const MpetOpT<T>& op, const T* x, const T* b, T* y, hipStream_t s) {
+  switch (op.lanes_u) {
+    case 4: launch_op_gp<T, F, 4>(op, x, b, y, op.lanes_p, s); break;
+    case 8: launch_op_gp<T, F, 8>(op, x, b, y, op.lanes_p, s); break;
+    case 32: launch_op_gp<T, F, 32>(op, x, b, y, op.lanes_p, s); break;
+    default: launch_op_gp<T, F, 16>(op, x, b, y, op.lanes_p, s); break;
+  }
+}
+
 template <typename T>
 int launch_op_apply(const MpetOpT<T>& op, const T* x, const T* b, T* y, hipStream_t s) {
-  int nbu = (op.nP2 + OP_ROWS - 1) / OP_ROWS;
-  int nbp = (op.nP1 + OP_ROWS - 1) / OP_ROWS;
-  dim3 g(nbu + nbp), blk(256);
   switch (op.A + 1) {
-    case 2: hipLaunchKernelGGL((op_apply_kernel<T, 2>), g, blk, 0, s, op, x, b, y, nbu); break;
-    case 3: hipLaunchKernelGGL((op_apply_kernel<T, 3>), g, blk, 0, s, op, x, b, y, nbu); break;
-    case 4: hipLaunchKernelGGL((op_apply_kernel<T, 4>), g, blk, 0, s, op, x, b, y, nbu); break;
-    case 5: hipLaunchKernelGGL((op_apply_kernel<T, 5>), g, blk, 0, s, op, x, b, y, nbu); break;
-    case 6: hipLaunchKernelGGL((op_apply_kernel<T, 6>), g, blk, 0, s, op, x, b, y, nbu); break;
-    case 7: hipLaunchKernelGGL((op_apply_kernel<T, 7>), g, blk, 0, s, op, x, b, y, nbu); break;
-    case 8: hipLaunchKernelGGL((op_apply_kernel<T, 8>), g, blk, 0, s, op, x, b, y, nbu); break;
+    case 2: launch_op_f<T, 2>(op, x, b, y, s); break;
+    case 3: launch_op_f<T, 3>(op, x, b, y, s); break;
+    case 4: launch_op_f<T, 4>(op, x, b, y, s); break;
+    case 5: launch_op_f<T, 5>(op, x, b, y, s); break;
+    case 6: launch_op_f<T, 6>(op, x, b, y, s); break;
+    case 7: launch_op_f<T, 7>(op, x, b, y, s); break;
+    case 8: launch_op_f<T, 8>(op, x, b, y, s); break;
     default: return ERR_INVALID;
   }
   return hipGetLastError() == hipSuccess ? OK : ERR_HIP;

@@ -118,12 +118,15 @@ class DeviceSystem:
         out["St"] = self.prim_t["St"].to(tdt)
         return out
 
-    def op_desc(self, dtype, masked=True):
+    lanes = {0: (16, 16), 1: (16, 16)}     # lanes per (P2, P1) node row, per dtype
+
+    def op_desc(self, dtype, masked=True, lanes=None):
         vals = self.op64 if dtype == 0 else self.op32
         coef = _lib.make_coef(self.Cm, self.Cs, self.mu)
+        lu, lp = lanes or self.lanes[dtype]
         return _lib.OpDesc(self.nP2, self.nP1, self.A, self.pat, dtype, ptr(vals["Kuu"]),
                            ptr(vals["Dv"]), ptr(vals["Dt"]), ptr(vals["M1"]), ptr(vals["St"]),
-                           coef, ptr(self.dmask) if masked else None)
+                           coef, ptr(self.dmask) if masked else None, lu, lp)
 
     def row_lengths(self, which):
         rp = {k: np.diff(v[0]) for k, v in self.pat_host.items()}

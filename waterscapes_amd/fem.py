@@ -72,7 +72,31 @@ class DofMap:
 
 
 def build_pattern(rmap, cmap, nrow, ncol):
-    """CSR (rowptr int64, cols int32) of all (r, c) with r, c in one cell."""
+    """CSR (rowptr int64, cols int32) of all (r, c) with r, c in one cell
+    (native builder in csrc/setup_host.cpp; numpy fallback for hosts that
+    have not built the library, used only by host-side tests)."""
+    try:
+        from . import _lib
+        L = _lib.lib()
+    except RuntimeError:
+        return _build_pattern_np(rmap, cmap, nrow, ncol)
+    import ctypes as C
+    rmap = np.ascontiguousarray(rmap, dtype=np.int32)
+    cmap = np.ascontiguousarray(cmap, dtype=np.int32)
+    rowptr = np.zeros(nrow + 1, dtype=np.int64)
+    fn = L.mpet_build_pattern
+    fn.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                   C.c_void_p]
+    fn.restype = C.c_int
+    _lib.check(fn(rmap.shape[0], rmap.shape[1], rmap.ctypes.data, cmap.shape[1],
+                  cmap.ctypes.data, nrow, rowptr.ctypes.data, None))
+    cols = np.empty(int(rowptr[-1]), dtype=np.int32)
+    _lib.check(fn(rmap.shape[0], rmap.shape[1], rmap.ctypes.data, cmap.shape[1],
+                  cmap.ctypes.data, nrow, rowptr.ctypes.data, cols.ctypes.data))
+    return rowptr, cols
+
+
+def _build_pattern_np(rmap, cmap, nrow, ncol):
     nC = rmap.shape[0]
     nr, nc = rmap.shape[1], cmap.shape[1]
     keys = (rmap.astype(np.int64)[:, :, None] * ncol

@@ -16,6 +16,7 @@ import scipy.sparse as sp
 import torch
 
 from . import _lib, amg, forms
+from .amg import AmgParams
 from ._lib import check, ptr
 
 MAXLEV = 16
@@ -29,16 +30,14 @@ class AmgLevelDesc(C.Structure):
                 ("lpr", C.c_int32), ("lprP", C.c_int32), ("lprR", C.c_int32)]
 
 
-class PrecDesc(C.Structure):
+class HierDesc(C.Structure):
     _fields_ = [("nlev", C.c_int32), ("lev", AmgLevelDesc * MAXLEV), ("coarse_inv", C.c_void_p),
                 ("degree", C.c_int32), ("eig_ratio", C.c_double), ("eig_safety", C.c_double),
-                ("inv_mu", C.c_double), ("mask0", C.c_void_p), ("nP1", C.c_int32),
-                ("F", C.c_int32), ("rp11", C.c_void_p), ("ci11", C.c_void_p),
-                ("nnz11", C.c_longlong), ("M1", C.c_void_p), ("St", C.c_void_p),
-                ("dinvp", C.c_void_p), ("pmask", C.c_void_p),
-                ("Pd", C.c_double * _lib.MAXF), ("Ps", C.c_double * _lib.MAXF),
-                ("pdeg", C.c_int32), ("plmin", C.c_double * _lib.MAXF),
-                ("plmax", C.c_double * _lib.MAXF)]
+                ("scale", C.c_double), ("mask0", C.c_void_p)]
+
+
+class PrecDesc(C.Structure):
+    _fields_ = [("u", HierDesc), ("p", HierDesc), ("nP1", C.c_int32), ("F", C.c_int32)]
 
 
 class CsrDesc(C.Structure):
@@ -66,8 +65,6 @@ def _bind():
     L.mpet_solver_destroy.restype = C.c_int
     L.mpet_solver_solve.argtypes = [P, P, P, P, P]
     L.mpet_solver_solve.restype = C.c_int
-    L.mpet_solver_set_bnorm.argtypes = [P, C.c_double]
-    L.mpet_solver_set_bnorm.restype = C.c_int
     L.mpet_precond_apply.argtypes = [P, C.c_int, P, P, P]
     L.mpet_precond_apply.restype = C.c_int
     return L
@@ -83,15 +80,14 @@ class SolverConfig:
     mode: str = "fp64"
     rtol: float = 1e-10
     maxit: int = 5000
-    outer_tol: float = 1e-8
-    inner_rtol: float = 1e-4
+    outer_tol: float = 1e-10
+    inner_rtol: float = 3e-4
     max_outer: int = 30
     check_every: int = 0              # 0: 1 for fp64, 10 for mixed
     use_graphs: bool = None           # default: mixed only
     use_csr: bool = None              # default: fp64 only
-    amg: amg.AmgParams = field(default_factory=amg.AmgParams)
-    p_reduction: float = 0.05         # Chebyshev error target of the pressure blocks
-    max_pdeg: int = 12
+    amg: AmgParams = field(default_factory=AmgParams)
+    pamg: AmgParams = field(default_factory=AmgParams)
     # weight of the total-pressure mass in the preconditioner.  The reference
     # form prec uses the bare mass p0 w0 dx (mpettotalpressuresolver.py:282),
     # which is off from the p0 Schur complement ~ (1/(2 mu) + 1/lambda) M by
@@ -106,11 +102,14 @@ def _csr_i32(rp):
     return rp.astype(np.int32)
 
 
-def _pick_lpr(avg):
-    for l in (4, 8, 16, 32):
-        if avg <= 1.5 * l:
-            return l
-    return 64
+def _pick_lpr(avg, nc=1):
+    """Lanes per row so that each lane handles about two nonzeros (at least
+    one lane per vector component for the epilogue)."""
+    target = max(avg / 2.0, 1.0)
+    lanes = 1
+    while lanes < target and lanes < 64:
+        lanes *= 2
+    return max(lanes, 4 if nc == 3 else 1)
 
 
 class KrylovSolver:
@@ -121,7 +120,10 @@ class KrylovSolver:
         self.cfg = cfg = config or SolverConfig()
         dev = system.device
         self.keep = []                       # tensors the native handle borrows
+        if cfg.mode not in ("fp64", "mixed", "fp64-pc32"):
+            raise ValueError("mode must be 'fp64', 'mixed' or 'fp64-pc32'")
         mixed = cfg.mode == "mixed"
+        pc32 = cfg.mode in ("mixed", "fp64-pc32")
         # Dirichlet masks
         dofs = np.unique(np.concatenate([np.asarray(d, dtype=np.int64) for d in bc_dofs])) \
             if len(bc_dofs) else np.zeros(0, np.int64)
@@ -148,19 +150,20 @@ class KrylovSolver:
         self._build_hierarchy()
         self._build_pressure()
         L = _bind()
-        self.prec64 = self._prec_desc(0) if not mixed else None
-        self.prec32 = self._prec_desc(1) if mixed else None
-        use_csr = cfg.use_csr if cfg.use_csr is not None else (not mixed)
+        self.prec64 = self._prec_desc(0) if not pc32 else None
+        self.prec32 = self._prec_desc(1) if pc32 else None
+        use_csr = cfg.use_csr if cfg.use_csr is not None else (cfg.mode == "fp64")
         self.csr_desc = None
-        if use_csr and not mixed:
+        if use_csr and cfg.mode == "fp64":
             if system.csr is None:
                 system.csr = system.expand_csr(0)
             rp, ci, v = system.csr
             avg = (system.nnz_csr if hasattr(system, "nnz_csr") else v.numel()) / system.n
-            self.csr_desc = CsrDesc(system.n, ptr(rp), ptr(ci), ptr(v), _pick_lpr(avg))
-        check_every = cfg.check_every or (1 if not mixed else 10)
-        graphs = cfg.use_graphs if cfg.use_graphs is not None else mixed
-        self.opt = SolverOptions(1 if mixed else 0, 1 if self.csr_desc is not None else 0,
+            self.csr_desc = CsrDesc(system.n, ptr(rp), ptr(ci), ptr(v), min(32, _pick_lpr(avg)))
+        check_every = cfg.check_every or (1 if cfg.mode == "fp64" else 10)
+        graphs = cfg.use_graphs if cfg.use_graphs is not None else (cfg.mode != "fp64")
+        mode_id = {"fp64": 0, "mixed": 1, "fp64-pc32": 2}[cfg.mode]
+        self.opt = SolverOptions(mode_id, 1 if self.csr_desc is not None else 0,
                                  cfg.rtol, cfg.maxit, check_every, 1 if graphs else 0,
                                  cfg.outer_tol, cfg.inner_rtol, cfg.max_outer)
         self.op64 = system.op_desc(0)
@@ -192,18 +195,16 @@ class KrylovSolver:
         # level 0 on the device is the un-eliminated Lap (output masked)
         self.level_dev = {}
 
-    def _level_arrays(self, dtype):
-        if dtype in self.level_dev:
-            return self.level_dev[dtype]
-        s = self.sys
-        dev = s.device
+    def _upload_levels(self, levels, cinv, dtype, level0=None):
+        """Device copies of a host hierarchy; level0 = (rp, ci, v) device
+        tensors to use instead of the host level-0 matrix."""
+        dev = self.sys.device
         tdt = torch.float64 if dtype == 0 else torch.float32
         out = []
-        for l, lev in enumerate(self.levels):
+        for l, lev in enumerate(levels):
             d = {}
-            if l == 0:
-                d["A_rp"], d["A_ci"] = s.pat_dev["22"]
-                d["A_v"] = s.prim_t["Lap"] if dtype == 0 else s.prim_t["Lap"].to(tdt)
+            if l == 0 and level0 is not None:
+                d["A_rp"], d["A_ci"], d["A_v"] = level0
             else:
                 d["A_rp"] = torch.as_tensor(_csr_i32(lev["A_rp"])).to(dev)
                 d["A_ci"] = torch.as_tensor(lev["A_ci"]).to(dev)
@@ -215,13 +216,23 @@ class KrylovSolver:
                     d[k + "_ci"] = torch.as_tensor(lev[k + "_ci"]).to(dev)
                     d[k + "_v"] = torch.as_tensor(lev[k + "_v"]).to(dev, tdt)
             out.append(d)
-        cinv = torch.as_tensor(self.cinv).to(dev, tdt)
-        self.level_dev[dtype] = (out, cinv)
-        return out, cinv
+        return out, torch.as_tensor(cinv).to(dev, tdt)
+
+    def _level_arrays(self, dtype):
+        if dtype in self.level_dev:
+            return self.level_dev[dtype]
+        s = self.sys
+        lap = s.prim_t["Lap"] if dtype == 0 else s.prim_t["Lap"].to(torch.float32)
+        u = self._upload_levels(self.levels, self.cinv, dtype,
+                                level0=(s.pat_dev["22"][0], s.pat_dev["22"][1], lap))
+        p = self._upload_levels(self.plevels, self.pcinv, dtype)
+        self.level_dev[dtype] = (u, p)
+        return u, p
 
     def _build_pressure(self):
-        """Inverse diagonals and Chebyshev intervals of the pressure blocks
-        (Pd_f M1 + Ps_f St_f, Dirichlet rows eliminated)."""
+        """Hierarchy of the block-diagonal pressure preconditioner
+        diag(P_0, .., P_A), P_f = Pd_f M1 + Ps_f St_f (Dirichlet rows and
+        columns eliminated), in field-major order."""
         s = self.sys
         rp, ci = s.pat_host["11"]
         M1 = s.prim_t["M1"].cpu().numpy()
@@ -230,107 +241,61 @@ class KrylovSolver:
         F = s.F
         Mm = sp.csr_matrix((M1, ci, rp), shape=(n1, n1))
         pmask = self.pmask.cpu().numpy().reshape(n1, F).astype(bool)
-        dinv = np.ones((n1, F))
-        lmin = np.zeros(F)
-        lmax = np.zeros(F)
-        dM = Mm.diagonal()
-        Pdw = self.Pd
+        blocks = []
         for f in range(F):
-            Pf = Pdw[f] * Mm
-            dS = np.zeros(n1)
+            Pf = self.Pd[f] * Mm
             if f >= 1:
-                Sf = sp.csr_matrix((St[f - 1], ci, rp), shape=(n1, n1))
-                Pf = Pf + s.Ps[f] * Sf
-                dS = Sf.diagonal()
+                Pf = Pf + s.Ps[f] * sp.csr_matrix((St[f - 1], ci, rp), shape=(n1, n1))
             free = ~pmask[:, f]
             keep = sp.diags(free.astype(np.float64))
-            Pe = (keep @ Pf @ keep + sp.diags((~free).astype(np.float64))).tocsr()
-            d = Pe.diagonal()
-            dinv[:, f] = 1.0 / d
-            # largest eigenvalue of D^-1 P: power iteration (D-inner product)
-            x = np.random.default_rng(7).random(n1) * free
-            lam = 1.0
-            for _ in range(40):
-                y = (Pe @ x) / d
-                lam = float(x @ (d * y)) / float(x @ (d * x))
-                x = y / np.linalg.norm(y)
-            lmax[f] = 1.05 * lam
-            # lower bound: P >= Pd M >= Pd/2 D_M  (P1 mass, Wathen) and
-            # D = Pd D_M + Ps D_S  =>  lambda_min >= 1/2 min_i Pd D_M/(Pd D_M + Ps D_S)
-            ratio = (Pdw[f] * dM) / (Pdw[f] * dM + (s.Ps[f] * dS if f >= 1 else 0.0))
-            lmin[f] = 0.5 * float(ratio[free].min()) if free.any() else 1.0
-        self.dinvp_host = dinv.reshape(-1)
-        self.plmin, self.plmax = lmin, lmax
-        # common degree reaching the error target in every field
-        deg = 1
-        for f in range(F):
-            kappa = lmax[f] / lmin[f]
-            rho = (np.sqrt(kappa) - 1) / (np.sqrt(kappa) + 1)
-            k = 1
-            while 2 * rho ** k / (1 + rho ** (2 * k)) > self.cfg.p_reduction and k < self.cfg.max_pdeg:
-                k += 1
-            deg = max(deg, k)
-        self.pdeg = deg
+            blocks.append((keep @ Pf @ keep + sp.diags((~free).astype(np.float64))).tocsr())
+        Pp = sp.block_diag(blocks, format="csr")
+        Pp.sort_indices()
+        self.Pp_host = Pp
+        self.plevels, self.pcinv = amg.setup_host(Pp.indptr, Pp.indices, Pp.data, self.cfg.pamg)
 
-    def _prec_desc(self, dtype):
-        s = self.sys
-        cfg = self.cfg
-        levs, cinv = self._level_arrays(dtype)
-        tdt = torch.float64 if dtype == 0 else torch.float32
-        self.keep += [levs, cinv]
-        d = PrecDesc()
-        d.nlev = len(self.levels)
-        if d.nlev > MAXLEV:
+    def _hier_desc(self, h, levels, levs, cinv, params, scale, mask0, nc):
+        h.nlev = len(levels)
+        if h.nlev > MAXLEV:
             raise ValueError("hierarchy too deep")
-        for l, (lev, ld) in enumerate(zip(self.levels, levs)):
-            L = d.lev[l]
+        for l, (lev, ld) in enumerate(zip(levels, levs)):
+            L = h.lev[l]
             L.n = lev["n"]
             L.nc = lev["nc"]
             L.A_rp, L.A_ci, L.A_v = ptr(ld["A_rp"]), ptr(ld["A_ci"]), ptr(ld["A_v"])
             L.dinv = ptr(ld["dinv"])
             L.rho = lev["rho"]
-            nnz = ld["A_ci"].numel()
-            L.lpr = _pick_lpr(nnz / max(lev["n"], 1))
+            L.lpr = _pick_lpr(ld["A_ci"].numel() / max(lev["n"], 1), nc)
             if lev["nc"]:
                 L.P_rp, L.P_ci, L.P_v = ptr(ld["P_rp"]), ptr(ld["P_ci"]), ptr(ld["P_v"])
                 L.R_rp, L.R_ci, L.R_v = ptr(ld["R_rp"]), ptr(ld["R_ci"]), ptr(ld["R_v"])
                 nnzp = ld["P_ci"].numel()
-                L.lprP = _pick_lpr(nnzp / max(lev["n"], 1))
-                L.lprR = _pick_lpr(nnzp / max(lev["nc"], 1))
-        d.coarse_inv = ptr(cinv)
-        d.degree = cfg.amg.smoother_degree
-        d.eig_ratio = cfg.amg.eig_ratio
-        d.eig_safety = cfg.amg.eig_safety
-        d.inv_mu = 1.0 / s.mu
-        d.mask0 = ptr(self.mask0)
+                L.lprP = _pick_lpr(nnzp / max(lev["n"], 1), nc)
+                L.lprR = _pick_lpr(nnzp / max(lev["nc"], 1), nc)
+        h.coarse_inv = ptr(cinv)
+        h.degree = params.smoother_degree
+        h.eig_ratio = params.eig_ratio
+        h.eig_safety = params.eig_safety
+        h.scale = scale
+        h.mask0 = ptr(mask0) if mask0 is not None else None
+
+    def _prec_desc(self, dtype):
+        s = self.sys
+        (ulevs, ucinv), (plevs, pcinv) = self._level_arrays(dtype)
+        self.keep += [ulevs, ucinv, plevs, pcinv]
+        d = PrecDesc()
+        self._hier_desc(d.u, self.levels, ulevs, ucinv, self.cfg.amg, s.mu, self.mask0, 3)
+        self._hier_desc(d.p, self.plevels, plevs, pcinv, self.cfg.pamg, 1.0, None, 1)
         d.nP1 = s.nP1
         d.F = s.F
-        rp11, ci11 = s.pat_dev["11"]
-        d.rp11, d.ci11 = ptr(rp11), ptr(ci11)
-        d.nnz11 = s.nnz["11"]
-        vals = s.op64 if dtype == 0 else s.op32
-        d.M1 = ptr(vals["M1"])
-        d.St = ptr(vals["St"])
-        dinvp = torch.as_tensor(self.dinvp_host).to(s.device, tdt)
-        self.keep.append(dinvp)
-        d.dinvp = ptr(dinvp)
-        d.pmask = ptr(self.pmask)
-        for f in range(s.F):
-            d.Pd[f] = self.Pd[f]
-            d.Ps[f] = s.Ps[f]
-            d.plmin[f] = self.plmin[f]
-            d.plmax[f] = self.plmax[f]
-        d.pdeg = self.pdeg
         return d
 
     # -- use ------------------------------------------------------------------
-    def solve(self, b, x, bnorm=0.0, raise_on_fail=True):
+    def solve(self, b, x, raise_on_fail=True):
         """Solve one step: b full right-hand side (FP64 device), x initial
         guess with Dirichlet values in place (overwritten by the solution)."""
         L = _bind()
         st = SolveStats()
-        if bnorm:
-            L.mpet_solver_set_bnorm(self.handle, float(bnorm))
         rc = L.mpet_solver_solve(self.handle, ptr(b), ptr(x), C.byref(st), _lib.stream_ptr())
         self.last_stats = st
         if rc != 0 and (raise_on_fail or rc != 3):
