@@ -125,6 +125,13 @@ typedef struct {
   double eig_ratio, eig_safety;  /* interval [safety*rho/ratio, safety*rho] */
   double scale;                  /* V-cycle approximates (scale * A0)^-1 */
   const uint8_t* mask0;          /* level-0 rows that are Dirichlet (may be NULL) */
+  /* Block-diagonal coarsest level (a batch of instances), or all NULL for one
+   * dense block: row r of the coarsest level belongs to block coarse_blk[r],
+   * which spans rows coarse_off[b] .. coarse_off[b+1]-1 and whose dense
+   * inverse starts at coarse_inv + coarse_ioff[b]. */
+  const int32_t* coarse_blk;
+  const int32_t* coarse_off;
+  const int64_t* coarse_ioff;
 } MpetHierDesc;
 
 /* Block-diagonal preconditioner of prec (:279-283) with Dirichlet rows and
@@ -135,6 +142,10 @@ typedef struct {
 typedef struct {
   MpetHierDesc u, p;
   int32_t nP1, F;
+  /* NULL: pressure DOF F*c + f (node-interleaved) sits at f*nP1 + c of the
+   * p hierarchy.  Otherwise pressure DOF i sits at pmap[i] (a batch: set
+   * nP1 to the number of pressure DOFs and F to 1). */
+  const int32_t* pmap;
 } MpetPrecDesc;
 
 typedef struct {
@@ -269,6 +280,78 @@ int mpet_solver_solve(void* h, const double* b, double* x, MpetSolveStats* stats
 
 /* z = P^-1 r with the FP64 (dtype 0) or FP32 (1) preconditioner. */
 int mpet_precond_apply(void* h, int dtype, const void* r, void* z, hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Batches: many independent instances (different meshes, network counts,
+ * parameters and boundary conditions) stepped together.  No counterpart in
+ * the reference, which steps one MPETTotalPressureSolver at a time (:456-502);
+ * each instance's step is the one mpet_solver_solve performs.
+ *
+ * Packed numbering of a batch of K instances:
+ *   u DOFs of all instances first: instance k's P2 nodes are the global P2
+ *   nodes p2_off[k] .. p2_off[k+1]-1, u DOF = 3a + j;
+ *   then the pressure DOFs: instance k's are 3*nP2 + pd_off[k] + F_k*c' + f
+ *   (c' = local P1 node, node-interleaved as for one instance).
+ * ------------------------------------------------------------------------- */
+
+/* Per-instance constants of a packed operator (device array, K entries). */
+typedef struct {
+  double C[MPET_MAXF * MPET_MAXF];   /* pressure coupling, as MpetCoef */
+  double stiff[MPET_MAXF];
+  int64_t st_base;    /* offset of the instance's St arrays in the packed St */
+  int64_t k11_base;   /* first P11 nonzero of the instance */
+  int64_t nnz11;      /* its P11 nonzeros (stride of its St arrays) */
+  int32_t F, pad;     /* fields A+1 */
+} MpetBatchCoef;
+
+/* Packed structured operator: the node patterns of all instances
+ * concatenated (row pointers global), values structure-of-arrays with the
+ * packed nonzero counts as strides (Kuu[c*nnz22 + k] etc., St per instance
+ * at st_base).  Column indices: ci22, ci12 = global P2 node; ci21, ci11 =
+ * packed DOF of the column node's p_0 (fields f follow at +f).  p_inst /
+ * p_base: instance and packed p_0 DOF of every global P1 node. */
+typedef struct {
+  int32_t ninst, nP2, nP1;
+  const int32_t *rp22, *ci22, *rp21, *ci21, *rp12, *ci12, *rp11, *ci11;
+  int64_t nnz22, nnz21, nnz12, nnz11;
+  int32_t dtype;                      /* 0 float64, 1 float32 */
+  const void *Kuu, *Dv, *Dt, *M1, *St;
+  const int32_t *p_inst, *p_base;
+  const MpetBatchCoef* coef;
+  const uint8_t* dmask;               /* packed Dirichlet mask (required) */
+} MpetBatchOpDesc;
+
+/* Instance ranges (host arrays of K+1 prefix sums). */
+typedef struct {
+  int32_t ninst;
+  const int64_t* p2_off;   /* P2 nodes */
+  const int64_t* p1_off;   /* P1 nodes */
+  const int64_t* pd_off;   /* pressure DOFs, relative to the first one */
+} MpetBatchLayout;
+
+/* Batched solver handle over packed operators and preconditioners (the
+ * hierarchies of the instances concatenated level by level, block-diagonal
+ * coarsest level, pmap set).  opt->mode 0 (FP64) or 1 (mixed); one MINRES
+ * state per instance.  Borrows the descriptors' arrays; owns workspaces. */
+int mpet_batch_create(const MpetBatchOpDesc* op64, const MpetBatchOpDesc* op32,
+                      const MpetBatchLayout* layout, const MpetPrecDesc* prec64,
+                      const MpetPrecDesc* prec32, const MpetSolverOptions* opt,
+                      void** handle);
+int mpet_batch_destroy(void* h);
+
+/* One time step of the instances with active[k] != 0 (NULL: all): for each,
+ * the solve of mpet_solver_solve on its part of the packed b and x
+ * (packed numbering).  stats: host array of K entries.  Synchronises. */
+int mpet_batch_solve(void* h, const double* b, double* x, const int32_t* active,
+                     MpetSolveStats* stats, hipStream_t stream);
+
+/* y = A x or y = b - A x of every instance in the packed operator of the
+ * handle (dtype 0 FP64, 1 FP32), Dirichlet rows 0 (as mpet_op_apply). */
+int mpet_batch_apply(void* h, int dtype, const void* x, const void* b, void* y,
+                     hipStream_t stream);
+
+/* z = P^-1 r with the batch's block preconditioner (as mpet_precond_apply). */
+int mpet_batch_precond_apply(void* h, int dtype, const void* r, void* z, hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,525 @@
+"""Batched MPET stepping: many MPETTotalPressureSolver instances (uneven
+meshes, network counts, parameters, boundary conditions and time steps)
+advanced together on one GPU (csrc/batch.hip).
+
+The reference steps one problem per MPETTotalPressureSolver
+(mpet/mpettotalpressuresolver.py:456-502); a batch runs that per-instance
+step for every active instance in one set of kernel launches:
+
+  * each instance keeps its own device-resident system (assembled
+    primitives, Dirichlet data, right-hand-side kernels) and its own host
+    AMG setup, exactly as a single solver would;
+  * the operator of all instances is packed into one block-diagonal
+    structured operator (u rows of all instances first, then the pressure
+    rows), and the hierarchies are concatenated level by level (instances
+    with fewer levels are padded with identity-prolongation levels ending in
+    their own coarsest matrix; the coarsest level is a block-diagonal dense
+    solve);
+  * MINRES keeps one state per instance (each instance converges to its own
+    tolerance with its own iteration count; finished instances are skipped).
+
+Per-instance results equal the single-instance solver's to the solver
+tolerance (not bitwise: padding and packing change rounding order).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time as _time
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, ptr, MAXF
+from .krylov import (MAXLEV, HierDesc, PrecDesc, SolveStats, SolverConfig, SolverOptions,
+                     _pick_lpr)
+from .solver import StepEngine
+
+
+class BatchCoef(C.Structure):
+    _fields_ = [("C", C.c_double * (MAXF * MAXF)), ("stiff", C.c_double * MAXF),
+                ("st_base", C.c_int64), ("k11_base", C.c_int64), ("nnz11", C.c_int64),
+                ("F", C.c_int32), ("pad", C.c_int32)]
+
+
+class BatchOpDesc(C.Structure):
+    _fields_ = [("ninst", C.c_int32), ("nP2", C.c_int32), ("nP1", C.c_int32),
+                ("rp22", C.c_void_p), ("ci22", C.c_void_p), ("rp21", C.c_void_p),
+                ("ci21", C.c_void_p), ("rp12", C.c_void_p), ("ci12", C.c_void_p),
+                ("rp11", C.c_void_p), ("ci11", C.c_void_p), ("nnz22", C.c_int64),
+                ("nnz21", C.c_int64), ("nnz12", C.c_int64), ("nnz11", C.c_int64),
+                ("dtype", C.c_int32), ("Kuu", C.c_void_p), ("Dv", C.c_void_p),
+                ("Dt", C.c_void_p), ("M1", C.c_void_p), ("St", C.c_void_p),
+                ("p_inst", C.c_void_p), ("p_base", C.c_void_p), ("coef", C.c_void_p),
+                ("dmask", C.c_void_p)]
+
+
+class BatchLayout(C.Structure):
+    _fields_ = [("ninst", C.c_int32), ("p2_off", C.c_void_p), ("p1_off", C.c_void_p),
+                ("pd_off", C.c_void_p)]
+
+
+def _bind():
+    L = _lib.lib()
+    P = C.c_void_p
+    for name, args in (("mpet_batch_create", [P, P, P, P, P, P, C.POINTER(C.c_void_p)]),
+                       ("mpet_batch_destroy", [P]),
+                       ("mpet_batch_solve", [P, P, P, P, P, P]),
+                       ("mpet_batch_apply", [P, C.c_int, P, P, P, P]),
+                       ("mpet_batch_precond_apply", [P, C.c_int, P, P, P])):
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    return L
+
+
+def _prefix(sizes):
+    out = np.zeros(len(sizes) + 1, dtype=np.int64)
+    np.cumsum(np.asarray(sizes, dtype=np.int64), out=out[1:])
+    return out
+
+
+def _i32(a):
+    a = np.asarray(a)
+    if a.size and (a.max() >= 2**31 or a.min() < -2**31):
+        raise ValueError("batch too large for 32-bit indices")
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def pad_levels(levels, cinv, depth):
+    """The levels of one instance extended to `depth`: after its coarsest
+    level come copies of that level joined by identity prolongators, so the
+    V-cycle of the padded hierarchy still ends in the exact coarsest solve
+    (smoothing an exact solution leaves it unchanged)."""
+    levels = [dict(lev) for lev in levels]
+    last = levels[-1]
+    n = last["n"]
+    eye_rp = np.arange(n + 1, dtype=np.int64)
+    eye_ci = np.arange(n, dtype=np.int32)
+    eye_v = np.ones(n)
+    while len(levels) < depth:
+        levels[-1].update(nc=n, P_rp=eye_rp, P_ci=eye_ci, P_v=eye_v, R_rp=eye_rp, R_ci=eye_ci,
+                          R_v=eye_v)
+        levels.append(dict(n=n, nc=0, A_rp=last["A_rp"], A_ci=last["A_ci"], A_v=last["A_v"],
+                           dinv=last["dinv"], rho=last["rho"]))
+    return levels, cinv
+
+
+def csr_blockdiag(blocks):
+    """Block-diagonal CSR of blocks (rp, ci, v, nrows, ncols) given as host
+    arrays: row pointers int64, columns int32."""
+    roff = _prefix([b[3] for b in blocks])
+    coff = _prefix([b[4] for b in blocks])
+    voff = _prefix([b[0][-1] for b in blocks])
+    rp = np.concatenate([np.asarray(b[0][:-1], dtype=np.int64) + voff[k]
+                         for k, b in enumerate(blocks)] + [voff[-1:]])
+    ci = np.concatenate([np.asarray(b[1], dtype=np.int64) + coff[k]
+                         for k, b in enumerate(blocks)])
+    v = np.concatenate([np.asarray(b[2], dtype=np.float64) for b in blocks])
+    return rp, _i32(ci), v, int(roff[-1]), int(coff[-1])
+
+
+def pack_hierarchy(hiers, scales, skip_A0=False):
+    """Concatenate instance hierarchies level by level (host arrays).
+
+    hiers: list of (levels, cinv) as returned by amg.setup_host; scales:
+    per-instance factor s_k (the packed hierarchy approximates
+    blockdiag(s_k A_k)^-1: matrices times s_k, inverses divided by s_k).
+    Each level's diagonal inverse is also divided by its spectral-radius
+    estimate so that one Chebyshev interval (rho = 1) fits every instance.
+    skip_A0: leave out the level-0 matrix (the caller supplies it on the
+    device).  Returns (levels, coarse) with coarse = (inv, blk, off, ioff)."""
+    depth = max(len(h[0]) for h in hiers)
+    padded = [pad_levels(lv, ci, depth) for lv, ci in hiers]
+    s = np.asarray(scales, dtype=np.float64)
+    out = []
+    for l in range(depth):
+        lev_k = [p[0][l] for p in padded]
+        rows = _prefix([lv["n"] for lv in lev_k])
+        d = dict(n=int(rows[-1]), rho=1.0, nc=0,
+                 dinv=np.concatenate([lv["dinv"] / (s[k] * lv["rho"])
+                                      for k, lv in enumerate(lev_k)]))
+        if not (l == 0 and skip_A0):
+            d["A_rp"], d["A_ci"], d["A_v"], _, _ = csr_blockdiag(
+                [(lv["A_rp"], lv["A_ci"], lv["A_v"] * s[k], lv["n"], lv["n"])
+                 for k, lv in enumerate(lev_k)])
+        if l < depth - 1:
+            P = csr_blockdiag([(lv["P_rp"], lv["P_ci"], lv["P_v"], lv["n"], lv["nc"])
+                               for lv in lev_k])
+            R = csr_blockdiag([(lv["R_rp"], lv["R_ci"], lv["R_v"], lv["nc"], lv["n"])
+                               for lv in lev_k])
+            d.update(nc=P[4], P_rp=P[0], P_ci=P[1], P_v=P[2], R_rp=R[0], R_ci=R[1], R_v=R[2])
+        out.append(d)
+    sizes = [p[0][-1]["n"] for p in padded]
+    off = _prefix(sizes)
+    inv = np.concatenate([(np.asarray(p[1]) / s[k]).reshape(-1) for k, p in enumerate(padded)])
+    ioff = _prefix([n * n for n in sizes])
+    blk = np.repeat(np.arange(len(sizes), dtype=np.int32), sizes)
+    return out, (inv, blk, _i32(off), ioff)
+
+
+class MPETBatch:
+    """Advance many MPETTotalPressureSolver instances together.
+
+    solvers: MPETTotalPressureSolver objects (each with its own problem and
+    params); their "precision" is overridden by `precision` ("mixed" or
+    "fp64").  The solvers' up_ / up become views into the batch's packed
+    state, so initial conditions set on them before construction are kept
+    and results are read from them as usual.
+    """
+
+    def __init__(self, solvers, precision="mixed", rtol=1e-10, outer_tol=1e-10,
+                 inner_rtol=3e-4, maxit=5000, max_outer=30, check_every=10, use_graphs=True):
+        if not torch.cuda.is_available():
+            raise RuntimeError("MPETBatch needs a GPU (no CPU fallback)")
+        if precision not in ("mixed", "fp64"):
+            raise ValueError("precision must be 'mixed' or 'fp64'")
+        self.solvers = list(solvers)
+        if not self.solvers:
+            raise ValueError("empty batch")
+        self.precision = precision
+        self.K = K = len(self.solvers)
+        dev = torch.device(self.solvers[0].params["device"])
+        self.device = dev
+        cfg = SolverConfig(mode=precision, rtol=rtol, outer_tol=outer_tol, inner_rtol=inner_rtol,
+                           maxit=maxit, max_outer=max_outer)
+        self.engines = []
+        for s in self.solvers:
+            if str(torch.device(s.params["device"])) != str(dev):
+                raise ValueError("all instances of a batch must be on one device")
+            cfg_k = SolverConfig(mode=precision, rtol=rtol, outer_tol=outer_tol,
+                                 inner_rtol=inner_rtol, maxit=maxit, max_outer=max_outer,
+                                 p0_weight=s.params["p0_weight"])
+            eng = StepEngine(s.problem, s.up_.function_space(), s.params["dt"],
+                             s.params["theta"], cfg_k, s.params["device"], batched=True)
+            s._engine = eng
+            self.engines.append(eng)
+        self.keep = []
+        self._layout()
+        self._pack_operator()
+        self._pack_preconditioner()
+        L = _bind()
+        self.opt = SolverOptions({"fp64": 0, "mixed": 1}[precision], 0, rtol, maxit,
+                                 check_every, 1 if use_graphs else 0, outer_tol, inner_rtol,
+                                 max_outer)
+        h = C.c_void_p()
+        pc = C.byref(self.prec)
+        check(L.mpet_batch_create(C.byref(self.op64),
+                                  C.byref(self.op32) if self.op32 is not None else None,
+                                  C.byref(self.layout), pc if precision == "fp64" else None,
+                                  pc if precision == "mixed" else None, C.byref(self.opt),
+                                  C.byref(h)))
+        self.handle = h
+        self._state_views()
+        self.stats = None
+
+    # -- layout and packing ----------------------------------------------------
+    def _layout(self):
+        sys_ = [e.system for e in self.engines]
+        self.p2_off = _prefix([s.nP2 for s in sys_])
+        self.p1_off = _prefix([s.nP1 for s in sys_])
+        self.pd_off = _prefix([s.F * s.nP1 for s in sys_])
+        self.n_off = _prefix([s.n for s in sys_])          # instance-major offsets
+        self.nP2 = int(self.p2_off[-1])
+        self.nP1 = int(self.p1_off[-1])
+        self.nu = 3 * self.nP2
+        self.n = self.nu + int(self.pd_off[-1])
+        if self.n >= 2**31:
+            raise ValueError("batch exceeds 2^31 DOFs")
+        self.layout = BatchLayout(self.K, self.p2_off.ctypes.data, self.p1_off.ctypes.data,
+                                  self.pd_off.ctypes.data)
+        # packed index -> instance-major index
+        perm = np.empty(self.n, dtype=np.int64)
+        for k, s in enumerate(sys_):
+            base = self.n_off[k]
+            perm[3 * self.p2_off[k]:3 * self.p2_off[k + 1]] = base + np.arange(s.nu)
+            p0 = self.nu + self.pd_off[k]
+            perm[p0:p0 + s.F * s.nP1] = base + s.nu + np.arange(s.F * s.nP1)
+        self.perm = torch.as_tensor(perm).to(self.device)
+
+    def _pack_operator(self):
+        dev = self.device
+        sys_ = [e.system for e in self.engines]
+        i32 = torch.int32
+        nz = {k: _prefix([s.nnz[k] for s in sys_]) for k in ("22", "21", "12", "11")}
+        self.nnz_off = nz
+        pats = {}
+        for kind in ("22", "21", "12", "11"):
+            rps, cis = [], []
+            for k, s in enumerate(sys_):
+                rp, ci = s.pat_dev[kind]
+                rps.append(rp[:-1].to(torch.int64) + int(nz[kind][k]))
+                if kind in ("22", "12"):     # columns are P2 nodes
+                    cis.append(ci.to(torch.int64) + int(self.p2_off[k]))
+                else:                        # columns are P1 nodes -> packed p_0 DOF
+                    cis.append(ci.to(torch.int64) * s.F + int(self.nu + self.pd_off[k]))
+            rps.append(torch.tensor([int(nz[kind][-1])], dtype=torch.int64, device=dev))
+            rp = torch.cat(rps)
+            ci = torch.cat(cis)
+            if int(rp[-1]) >= 2**31 or (ci.numel() and int(ci.max()) >= 2**31):
+                raise ValueError("batch too large for 32-bit pattern indices")
+            pats[kind] = (rp.to(i32), ci.to(i32))
+        self.pats = pats
+        # per-P1-node instance and packed p_0 DOF
+        p_inst = np.repeat(np.arange(self.K, dtype=np.int32), np.diff(self.p1_off))
+        p_base = np.concatenate([self.nu + self.pd_off[k] + s.F * np.arange(s.nP1)
+                                 for k, s in enumerate(sys_)])
+        self.p_inst = torch.as_tensor(p_inst).to(dev)
+        self.p_base = torch.as_tensor(_i32(p_base)).to(dev)
+        # per-instance coefficients
+        st_base = _prefix([s.A * s.nnz["11"] for s in sys_])
+        coefs = (BatchCoef * self.K)()
+        for k, s in enumerate(sys_):
+            c = coefs[k]
+            for f in range(s.F):
+                for g in range(s.F):
+                    c.C[f * MAXF + g] = float(s.Cm[f, g])
+                c.stiff[f] = float(s.Cs[f])
+            c.st_base = int(st_base[k])
+            c.k11_base = int(nz["11"][k])
+            c.nnz11 = int(s.nnz["11"])
+            c.F = s.F
+        raw = np.frombuffer(coefs, dtype=np.uint8).copy()
+        self.coef_dev = torch.as_tensor(raw).to(dev)
+        # Dirichlet mask (packed)
+        m = torch.cat([s.dmask[:s.nu] for s in sys_] + [s.dmask[s.nu:] for s in sys_])
+        self.dmask = m.contiguous()
+        self.op64 = self._pack_values(0, st_base)
+        self.op32 = self._pack_values(1, st_base) if self.precision == "mixed" else None
+
+    def _pack_values(self, dtype, st_base):
+        """Structure-of-arrays values of every instance in one packed array
+        per primitive (strides = packed nonzero counts)."""
+        dev = self.device
+        tdt = torch.float64 if dtype == 0 else torch.float32
+        sys_ = [e.system for e in self.engines]
+        nz = self.nnz_off
+        L = _lib.lib()
+        sptr = _lib.stream_ptr()
+        vals = {}
+        for name, kind, ncomp in (("Kuu", "22", 9), ("Dv", "21", 3), ("Dt", "12", 3),
+                                  ("M1", "11", 1)):
+            tot = int(nz[kind][-1])
+            out = torch.empty(ncomp * tot, dtype=tdt, device=dev)
+            view = out.view(ncomp, tot)
+            for k, s in enumerate(sys_):
+                src = s.prim_t[name]
+                n = src.numel() // ncomp
+                tmp = torch.empty(ncomp * n, dtype=tdt, device=dev)
+                check(L.mpet_soa_convert(dtype, ptr(src), ncomp, n, ptr(tmp), sptr))
+                view[:, int(nz[kind][k]):int(nz[kind][k]) + n].copy_(tmp.view(ncomp, n))
+                del tmp
+            vals[name] = out
+        vals["St"] = torch.cat([s.prim_t["St"].to(tdt) for s in sys_]) if sys_ else None
+        assert vals["St"].numel() == int(st_base[-1])
+        self.keep.append(vals)
+        p = self.pats
+        return BatchOpDesc(self.K, self.nP2, self.nP1, ptr(p["22"][0]), ptr(p["22"][1]),
+                           ptr(p["21"][0]), ptr(p["21"][1]), ptr(p["12"][0]), ptr(p["12"][1]),
+                           ptr(p["11"][0]), ptr(p["11"][1]), int(nz["22"][-1]),
+                           int(nz["21"][-1]), int(nz["12"][-1]), int(nz["11"][-1]), dtype,
+                           ptr(vals["Kuu"]), ptr(vals["Dv"]), ptr(vals["Dt"]), ptr(vals["M1"]),
+                           ptr(vals["St"]), ptr(self.p_inst), ptr(self.p_base),
+                           ptr(self.coef_dev), ptr(self.dmask))
+
+    def _upload_hier(self, h, levels, coarse, params, mask0, nc, level0=None):
+        dev = self.device
+        tdt = torch.float64 if self.precision == "fp64" else torch.float32
+        keep = []
+        h.nlev = len(levels)
+        if h.nlev > MAXLEV:
+            raise ValueError("hierarchy too deep")
+        for l, lev in enumerate(levels):
+            L = h.lev[l]
+            L.n = lev["n"]
+            L.nc = lev["nc"]
+            if l == 0 and level0 is not None:
+                rp, ci, v = level0
+            else:
+                rp = torch.as_tensor(_i32(lev["A_rp"])).to(dev)
+                ci = torch.as_tensor(lev["A_ci"]).to(dev)
+                v = torch.as_tensor(lev["A_v"]).to(dev, tdt)
+            dinv = torch.as_tensor(lev["dinv"]).to(dev, tdt)
+            keep += [rp, ci, v, dinv]
+            L.A_rp, L.A_ci, L.A_v, L.dinv = ptr(rp), ptr(ci), ptr(v), ptr(dinv)
+            L.rho = lev["rho"]
+            L.lpr = _pick_lpr(ci.numel() / max(lev["n"], 1), nc)
+            if lev["nc"]:
+                arrs = []
+                for k in ("P", "R"):
+                    a = (torch.as_tensor(_i32(lev[k + "_rp"])).to(dev),
+                         torch.as_tensor(lev[k + "_ci"]).to(dev),
+                         torch.as_tensor(lev[k + "_v"]).to(dev, tdt))
+                    arrs.append(a)
+                    keep += list(a)
+                (L.P_rp, L.P_ci, L.P_v), (L.R_rp, L.R_ci, L.R_v) = \
+                    [tuple(ptr(t) for t in a) for a in arrs]
+                nnzp = arrs[0][1].numel()
+                L.lprP = _pick_lpr(nnzp / max(lev["n"], 1), nc)
+                L.lprR = _pick_lpr(nnzp / max(lev["nc"], 1), nc)
+        inv, blk, off, ioff = coarse
+        cinv = torch.as_tensor(inv).to(dev, tdt)
+        cblk = torch.as_tensor(blk).to(dev)
+        coff = torch.as_tensor(off).to(dev)
+        cioff = torch.as_tensor(ioff).to(dev)
+        keep += [cinv, cblk, coff, cioff]
+        h.coarse_inv = ptr(cinv)
+        h.coarse_blk, h.coarse_off, h.coarse_ioff = ptr(cblk), ptr(coff), ptr(cioff)
+        h.degree = params.smoother_degree
+        h.eig_ratio = params.eig_ratio
+        h.eig_safety = params.eig_safety
+        h.scale = 1.0
+        h.mask0 = ptr(mask0) if mask0 is not None else None
+        self.keep.append(keep)
+
+    def _pack_preconditioner(self):
+        dev = self.device
+        tdt = torch.float64 if self.precision == "fp64" else torch.float32
+        kry = [e.krylov for e in self.engines]
+        sys_ = [e.system for e in self.engines]
+        # displacement block: scalar P2 Laplacian hierarchies scaled by mu_k
+        mus = [s.mu for s in sys_]
+        ulev, ucoarse = pack_hierarchy([(k.levels, k.cinv) for k in kry], mus, skip_A0=True)
+        lap0 = torch.cat([s.prim_t["Lap"] * s.mu for s in sys_]).to(tdt)
+        level0 = (self.pats["22"][0], (torch.cat([s.pat_dev["22"][1].to(torch.int64)
+                                                  + int(self.p2_off[k])
+                                                  for k, s in enumerate(sys_)])).to(torch.int32),
+                  lap0)
+        self.mask0 = torch.cat([k.mask0 for k in kry]).contiguous()
+        # pressure blocks: field-major block-diagonal hierarchies, scale 1
+        plev, pcoarse = pack_hierarchy([(k.plevels, k.pcinv) for k in kry], [1.0] * self.K)
+        pmap = np.concatenate([self.pd_off[k] + (np.arange(s.F)[None, :] * s.nP1
+                                                 + np.arange(s.nP1)[:, None]).reshape(-1)
+                               for k, s in enumerate(sys_)])
+        self.pmap = torch.as_tensor(_i32(pmap)).to(dev)
+        d = PrecDesc()
+        cfg = kry[0].cfg
+        self._upload_hier(d.u, ulev, ucoarse, cfg.amg, self.mask0, 3, level0=level0)
+        self._upload_hier(d.p, plev, pcoarse, cfg.pamg, None, 1)
+        d.nP1 = int(self.pd_off[-1])
+        d.F = 1
+        d.pmap = ptr(self.pmap)
+        self.prec = d
+        self.hier_depth = (len(ulev), len(plev))
+
+    def _state_views(self):
+        """Instance-major right-hand side and state buffers; each engine's b
+        and each solver's up_ / up become views into them."""
+        dev = self.device
+        f64 = torch.float64
+        self.B_i = torch.zeros(self.n, dtype=f64, device=dev)
+        self.X_i = torch.zeros(self.n, dtype=f64, device=dev)
+        self.Xprev_i = torch.zeros(self.n, dtype=f64, device=dev)
+        self.Bp = torch.empty(self.n, dtype=f64, device=dev)
+        self.Xp = torch.empty(self.n, dtype=f64, device=dev)
+        for k, (s, e) in enumerate(zip(self.solvers, self.engines)):
+            a, b = int(self.n_off[k]), int(self.n_off[k + 1])
+            self.Xprev_i[a:b].copy_(s.up_.data)
+            self.X_i[a:b].copy_(s.up.data)
+            s.up_.data = self.Xprev_i[a:b]
+            s.up.data = self.X_i[a:b]
+            e.b = self.B_i[a:b]
+        self.t = [float(s.problem.time) for s in self.solvers]
+
+    # -- operators (tests and diagnostics) -------------------------------------
+    def pack(self, x_inst):
+        """Instance-major vector -> packed numbering."""
+        return x_inst[self.perm]
+
+    def unpack(self, x_packed):
+        out = torch.empty_like(x_packed)
+        out[self.perm] = x_packed
+        return out
+
+    def apply(self, xp, b=None, dtype=0):
+        y = torch.empty_like(xp)
+        check(_bind().mpet_batch_apply(self.handle, dtype, ptr(xp), ptr(b), ptr(y),
+                                       _lib.stream_ptr()))
+        return y
+
+    def precond(self, rp, dtype=None):
+        dtype = (0 if self.precision == "fp64" else 1) if dtype is None else dtype
+        z = torch.empty_like(rp)
+        check(_bind().mpet_batch_precond_apply(self.handle, dtype, ptr(rp), ptr(z),
+                                               _lib.stream_ptr()))
+        return z
+
+    # -- time stepping -----------------------------------------------------------
+    def step(self, active=None, raise_on_fail=True):
+        """One implicit step of every instance k with active[k] (default:
+        all): the reference's per-step sequence (:461-494) for each, one
+        batched solve for all.  Advances each instance's time; returns the
+        per-instance SolveStats (zeros for inactive instances)."""
+        act = np.ones(self.K, dtype=np.int32) if active is None else \
+            np.asarray(active, dtype=np.int32).copy()
+        t_next = list(self.t)
+        for k in np.nonzero(act)[0]:
+            e, s = self.engines[k], self.solvers[k]
+            t_next[k] = e.assemble_rhs(s.up_.data, self.t[k])
+            x = s.up.data
+            x.copy_(s.up_.data)
+            if e.bc_dofs.numel():
+                g = torch.as_tensor(e.bc_values(None)).to(self.device)
+                x.index_copy_(0, e.bc_dofs, g)
+        torch.index_select(self.B_i, 0, self.perm, out=self.Bp)
+        torch.index_select(self.X_i, 0, self.perm, out=self.Xp)
+        stats = (SolveStats * self.K)()
+        rc = _bind().mpet_batch_solve(self.handle, ptr(self.Bp), ptr(self.Xp),
+                                      act.ctypes.data, stats, _lib.stream_ptr())
+        self.stats = stats
+        if rc != 0 and (raise_on_fail or rc != 3):
+            check(rc)
+        self.X_i.index_copy_(0, self.perm, self.Xp)
+        for k in np.nonzero(act)[0]:
+            self.t[k] = t_next[k]
+        return stats
+
+    def commit(self, active=None):
+        """up_ <- up for the stepped instances (the generator's update after
+        its yield, :497-499) and their time Constants to t_{n+1}."""
+        if active is None:
+            self.Xprev_i.copy_(self.X_i)
+            ks = range(self.K)
+        else:
+            ks = np.nonzero(np.asarray(active))[0]
+            for k in ks:
+                self.solvers[k].up_.data.copy_(self.solvers[k].up.data)
+        for k in ks:
+            self.solvers[k].problem.time.assign(self.t[k])
+
+    def solve(self):
+        """Generator: steps every instance until its own T (each instance's
+        loop condition t < T - 1e-9 of :456).  Yields (stepped, stats) per
+        batched step, stepped = indices of the instances advanced; each
+        solver's up holds its new state during the yield and up_ is updated
+        after it, as in the single-instance generator."""
+        Ts = [float(s.params["T"]) for s in self.solvers]
+        for s in self.solvers:
+            s.solver_monitor.setdefault("niter", [])
+            s.solver_monitor.setdefault("outer", [])
+            s.solver_monitor.setdefault("residual", [])
+        while True:
+            act = np.array([1 if self.t[k] < Ts[k] - 1e-9 else 0 for k in range(self.K)],
+                           dtype=np.int32)
+            if not act.any():
+                return
+            stats = self.step(act)
+            for k in np.nonzero(act)[0]:
+                m = self.solvers[k].solver_monitor
+                m["niter"].append(int(stats[k].inner_iterations))
+                m["outer"].append(int(stats[k].outer_iterations))
+                m["residual"].append(float(stats[k].residual_norm))
+            yield np.nonzero(act)[0], stats
+            self.commit(act if not act.all() else None)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            torch.cuda.synchronize()
+            _bind().mpet_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -26,7 +26,7 @@
 //
 // Dirichlet invariant: Krylov vectors have zero Dirichlet components and
 // operators zero their Dirichlet rows (see spmv.hip).
-#include "mpet_ops.h"
+#include "amg_cycle.h"
 #include <vector>
 #include <math.h>
 #include <stdio.h>
@@ -34,48 +34,6 @@
 #include <string.h>
 
 namespace mpet {
-
-constexpr int NT = 256;            // threads per block everywhere
-constexpr int MAX_PARTIALS = 8192; // per-block partial sums of one reduction
-constexpr int MAXLEV = 16;
-constexpr int MAXDEG = 16;
-
-// ---------------------------------------------------------------------------
-// reductions
-// ---------------------------------------------------------------------------
-// All lanes of an aligned group of G lanes get the group sum.
-template <int G, typename T>
-__device__ __forceinline__ T group_allsum(T v) {
-#pragma unroll
-  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-// Block sum of v, valid in thread 0.
-__device__ __forceinline__ double block_sum(double v) {
-  __shared__ double sh[NT / 64];
-  v = wave_sum(v);
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  if (l == 0) sh[w] = v;
-  __syncthreads();
-  double r = 0.0;
-  if (threadIdx.x == 0)
-    for (int i = 0; i < NT / 64; ++i) r += sh[i];
-  return r;
-}
-
-// Fixed-order sum of n partials by one block; valid in thread 0.
-__device__ __forceinline__ double sum_partials(const double* __restrict__ p, int n) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += NT) s += p[i];
-  return block_sum(s);
-}
-
-static int grid_for(long long items, int per_block, int cap) {
-  long long b = (items + per_block - 1) / per_block;
-  if (b < 1) b = 1;
-  return (int)(b < cap ? b : cap);
-}
 
 // ---------------------------------------------------------------------------
 // MINRES state
@@ -87,121 +45,6 @@ struct MinresState {
   int inactive[2];
 };
 
-// ---------------------------------------------------------------------------
-// AMG level kernels on NC-column node-interleaved vectors (NC = 3 for the
-// displacement block: one scalar matrix, three components; NC = 1 for the
-// pressure blocks).  LPR lanes per row; after an xor all-reduce, lane c of
-// the row group finishes component c.
-// MODE 0: y = A x        MODE 1: y = b - A x       MODE 2: y += A x
-// MODE 3: Chebyshev step d = cd d + cr Dinv (b - A x); y = x + d
-// ---------------------------------------------------------------------------
-template <typename T, int NC, int LPR, int MODE>
-__global__ void __launch_bounds__(NT)
-spmm_kernel(int nrows, const int* __restrict__ rp, const int* __restrict__ ci,
-            const T* __restrict__ val, const T* __restrict__ x, const T* __restrict__ b,
-            T* __restrict__ y, const T* __restrict__ dinv, T* __restrict__ d, T cd, T cr,
-            const uint8_t* __restrict__ mask, const int* __restrict__ skip) {
-  static_assert(NC <= LPR, "one lane per component in the epilogue");
-  if (skip && *skip) return;
-  long long gid = (long long)blockIdx.x * NT + threadIdx.x;
-  int r = (int)(gid / LPR);
-  int lane = (int)(gid % LPR);
-  if (r >= nrows) return;
-  T acc[NC];
-#pragma unroll
-  for (int c = 0; c < NC; ++c) acc[c] = 0;
-  const int kb = rp[r], ke = rp[r + 1];
-  for (int k = kb + lane; k < ke; k += LPR) {
-    const T a = val[k];
-    const long long col = (long long)NC * ci[k];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) acc[c] += a * x[col + c];
-  }
-  T mine = 0;
-#pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    T v = group_allsum<LPR>(acc[c]);
-    if (lane == c) mine = v;
-  }
-  if (lane >= NC) return;
-  const long long o = (long long)NC * r + lane;
-  const bool m = mask && mask[r];
-  if (MODE == 0) {
-    y[o] = m ? (T)0 : mine;
-  } else if (MODE == 1) {
-    y[o] = m ? (T)0 : b[o] - mine;
-  } else if (MODE == 2) {
-    if (!m) y[o] += mine;
-  } else {
-    T dd = m ? (T)0 : cd * d[o] + cr * dinv[r] * (b[o] - mine);
-    d[o] = dd;
-    y[o] = m ? (T)0 : x[o] + dd;
-  }
-}
-
-// First Chebyshev step from x0 = 0: d = y = cr * Dinv * (scale * b); the
-// scaled right-hand side is kept in bcopy when given.
-template <typename T, int NC>
-__global__ void __launch_bounds__(NT)
-cheb0_kernel(int nrows, const T* __restrict__ b, T scale, const T* __restrict__ dinv, T cr,
-             T* __restrict__ d, T* __restrict__ y, T* __restrict__ bcopy,
-             const uint8_t* __restrict__ mask, const int* __restrict__ skip) {
-  if (skip && *skip) return;
-  long long i = (long long)blockIdx.x * NT + threadIdx.x;
-  if (i >= (long long)NC * nrows) return;
-  int r = (int)(i / NC);
-  T bv = scale * b[i];
-  if (bcopy) bcopy[i] = bv;
-  T v = (mask && mask[r]) ? (T)0 : cr * dinv[r] * bv;
-  d[i] = v;
-  y[i] = v;
-}
-
-// Dense coarsest solve x = Cinv b (NC columns), one wave per row.
-template <typename T, int NC>
-__global__ void __launch_bounds__(NT)
-coarse_kernel(int n, const T* __restrict__ cinv, const T* __restrict__ b, T* __restrict__ x,
-              const int* __restrict__ skip) {
-  if (skip && *skip) return;
-  const int r = blockIdx.x * (NT / 64) + threadIdx.x / 64;
-  const int lane = threadIdx.x % 64;
-  if (r >= n) return;
-  T acc[NC];
-#pragma unroll
-  for (int c = 0; c < NC; ++c) acc[c] = 0;
-  for (int k = lane; k < n; k += 64) {
-    const T a = cinv[(long long)r * n + k];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) acc[c] += a * b[NC * k + c];
-  }
-#pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    T v = group_allsum<64>(acc[c]);
-    if (lane == c) x[NC * r + c] = v;
-  }
-}
-
-// Pressure vectors: node-interleaved [c][F] <-> field-major [f][c].
-template <typename T>
-__global__ void __launch_bounds__(NT)
-p_gather_kernel(int nP1, int F, const T* __restrict__ src, T* __restrict__ dst,
-                const int* __restrict__ skip) {
-  if (skip && *skip) return;
-  long long i = (long long)blockIdx.x * NT + threadIdx.x;
-  if (i >= (long long)F * nP1) return;
-  int c = (int)(i / F), f = (int)(i % F);
-  dst[(long long)f * nP1 + c] = src[i];
-}
-template <typename T>
-__global__ void __launch_bounds__(NT)
-p_scatter_kernel(int nP1, int F, const T* __restrict__ src, T* __restrict__ dst,
-                 const int* __restrict__ skip) {
-  if (skip && *skip) return;
-  long long i = (long long)blockIdx.x * NT + threadIdx.x;
-  if (i >= (long long)F * nP1) return;
-  int c = (int)(i / F), f = (int)(i % F);
-  dst[i] = src[(long long)f * nP1 + c];
-}
 
 // ---------------------------------------------------------------------------
 // MINRES kernels
@@ -496,160 +339,6 @@ __global__ void cvt_up_kernel(long long n, const float* __restrict__ a, double* 
 // ===========================================================================
 using namespace mpet;
 
-#define HIPCHK(expr)                                                        \
-  do {                                                                      \
-    hipError_t _e = (expr);                                                 \
-    if (_e != hipSuccess) {                                                 \
-      char _m[400];                                                         \
-      snprintf(_m, sizeof(_m), "%s: %s", #expr, hipGetErrorString(_e));     \
-      mpet_set_error(_m);                                                   \
-      return ERR_HIP;                                                       \
-    }                                                                       \
-  } while (0)
-
-static void chebyshev(double lmin, double lmax, int deg, double* cd, double* cr) {
-  double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
-  double sigma = theta / delta, rho = 1.0 / sigma;
-  cd[0] = 0.0;
-  cr[0] = 1.0 / theta;
-  for (int i = 1; i < deg; ++i) {
-    double rn = 1.0 / (2.0 * sigma - rho);
-    cd[i] = rn * rho;
-    cr[i] = 2.0 * rn / delta;
-    rho = rn;
-  }
-}
-
-template <typename T>
-struct LevelDev {
-  int n = 0, nc = 0;
-  const int *rp = nullptr, *ci = nullptr;
-  const T *v = nullptr, *dinv = nullptr;
-  const int *Prp = nullptr, *Pci = nullptr, *Rrp = nullptr, *Rci = nullptr;
-  const T *Pv = nullptr, *Rv = nullptr;
-  const uint8_t* mask = nullptr;
-  double cd[MAXDEG], cr[MAXDEG];
-  int lpr = 8, lprP = 4, lprR = 8;
-  T *B = nullptr, *X = nullptr, *R = nullptr, *E = nullptr, *S1 = nullptr, *S2 = nullptr,
-    *D = nullptr;
-};
-
-// One AMG hierarchy applied to NC-column vectors.
-template <typename T>
-struct Hier {
-  int nlev = 0, deg = 2, NC = 1;
-  LevelDev<T> lev[MAXLEV];
-  const T* cinv = nullptr;
-  double scale = 1.0;      // the V-cycle approximates (scale * A0)^-1
-};
-
-template <typename T>
-struct Precond {
-  Hier<T> u, p;
-  int nP1 = 0, F = 0;
-  long long nu = 0;
-  T *pin = nullptr, *pout = nullptr;   // field-major pressure buffers
-};
-
-template <typename T, int NC, int MODE>
-static void spmm(const LevelDev<T>& L, int which, const T* x, const T* b, T* y, T* d, T cd,
-                 T cr, const int* skip, hipStream_t s) {
-  // which: 0 level operator, 1 prolongator (rows n), 2 restriction (rows nc)
-  int nrows = which == 2 ? L.nc : L.n;
-  const int* rp = which == 0 ? L.rp : (which == 1 ? L.Prp : L.Rrp);
-  const int* ci = which == 0 ? L.ci : (which == 1 ? L.Pci : L.Rci);
-  const T* v = which == 0 ? L.v : (which == 1 ? L.Pv : L.Rv);
-  int lpr = which == 0 ? L.lpr : (which == 1 ? L.lprP : L.lprR);
-  if (lpr < NC) lpr = NC == 3 ? 4 : NC;
-  const uint8_t* mask = which == 2 ? nullptr : L.mask;
-  long long threads = (long long)nrows * lpr;
-  int blocks = (int)((threads + NT - 1) / NT);
-  if (blocks == 0) return;
-#define SPMM_CASE(LP)                                                                          \
-  case LP:                                                                                     \
-    hipLaunchKernelGGL((spmm_kernel<T, NC, (LP < NC ? 4 : LP), MODE>), dim3(blocks), dim3(NT), \
-                       0, s, nrows, rp, ci, v, x, b, y, L.dinv, d, cd, cr, mask, skip);       \
-    break;
-  switch (lpr) {
-    SPMM_CASE(1) SPMM_CASE(2) SPMM_CASE(4) SPMM_CASE(8) SPMM_CASE(16) SPMM_CASE(32)
-    SPMM_CASE(64)
-    default: break;
-  }
-#undef SPMM_CASE
-}
-
-// k Chebyshev steps on level l; from zero (x0 == nullptr, b scaled by
-// bscale and copied to bcopy when given) or from x0; result in out.
-template <typename T, int NC>
-static void smooth(const Hier<T>& H, int l, const T* b, T bscale, T* bcopy, const T* x0,
-                   T* out, const int* skip, hipStream_t s) {
-  const LevelDev<T>& L = H.lev[l];
-  const int k = H.deg;
-  const T* cur = x0;
-  int i = 0;
-  T* bufs[2] = {L.S1, L.S2};
-  int flip = 0;
-  const T* bb = b;
-  if (!x0) {
-    T* dst = (k == 1) ? out : bufs[flip];
-    int blocks = (int)(((long long)NC * L.n + NT - 1) / NT);
-    hipLaunchKernelGGL((cheb0_kernel<T, NC>), dim3(blocks), dim3(NT), 0, s, L.n, b, bscale,
-                       L.dinv, (T)L.cr[0], L.D, dst, bcopy, L.mask, skip);
-    if (bcopy) bb = bcopy;
-    cur = dst;
-    i = 1;
-    flip ^= 1;
-  }
-  for (; i < k; ++i) {
-    T* dst = (i == k - 1) ? out : bufs[flip];
-    if (dst == cur) dst = bufs[flip ^ 1];
-    spmm<T, NC, 3>(L, 0, cur, bb, dst, L.D, (T)L.cd[i], (T)L.cr[i], skip, s);
-    cur = dst;
-    flip ^= 1;
-  }
-}
-
-template <typename T, int NC>
-static void vcycle(const Hier<T>& H, int l, const T* b, T bscale, T* out, const int* skip,
-                   hipStream_t s) {
-  const LevelDev<T>& L = H.lev[l];
-  if (l == H.nlev - 1) {
-    const T* bb = b;
-    if (bscale != (T)1) {   // single-level hierarchy: scale into L.B first
-      int blocks = (int)(((long long)NC * L.n + NT - 1) / NT);
-      hipLaunchKernelGGL((cheb0_kernel<T, NC>), dim3(blocks), dim3(NT), 0, s, L.n, b, bscale,
-                         L.dinv, (T)0, L.D, L.X, L.B, L.mask, skip);
-      bb = L.B;
-    }
-    int blocks = (L.n + NT / 64 - 1) / (NT / 64);
-    hipLaunchKernelGGL((coarse_kernel<T, NC>), dim3(blocks), dim3(NT), 0, s, L.n, H.cinv, bb,
-                       out, skip);
-    return;
-  }
-  T* bcopy = (bscale != (T)1) ? L.B : nullptr;
-  smooth<T, NC>(H, l, b, bscale, bcopy, nullptr, L.X, skip, s);
-  const T* bb = bcopy ? bcopy : b;
-  spmm<T, NC, 1>(L, 0, L.X, bb, L.R, nullptr, 0, 0, skip, s);
-  const LevelDev<T>& C = H.lev[l + 1];
-  spmm<T, NC, 0>(L, 2, L.R, nullptr, C.B, nullptr, 0, 0, skip, s);
-  vcycle<T, NC>(H, l + 1, C.B, (T)1, C.E, skip, s);
-  spmm<T, NC, 2>(L, 1, C.E, nullptr, L.X, nullptr, 0, 0, skip, s);
-  smooth<T, NC>(H, l, bb, (T)1, nullptr, L.X, out, skip, s);
-}
-
-template <typename T>
-static void precond_apply(const Precond<T>& P, const T* r, T* z, const int* skip,
-                          hipStream_t s) {
-  vcycle<T, 3>(P.u, 0, r, (T)(1.0 / P.u.scale), z, skip, s);
-  long long np = (long long)P.F * P.nP1;
-  int eb = (int)((np + NT - 1) / NT);
-  hipLaunchKernelGGL((p_gather_kernel<T>), dim3(eb), dim3(NT), 0, s, P.nP1, P.F, r + P.nu, P.pin,
-                     skip);
-  vcycle<T, 1>(P.p, 0, P.pin, (T)1, P.pout, skip, s);
-  hipLaunchKernelGGL((p_scatter_kernel<T>), dim3(eb), dim3(NT), 0, s, P.nP1, P.F, P.pout,
-                     z + P.nu, skip);
-}
-
 // --- MINRES workspace in precision T -------------------------------------
 template <typename T>
 struct Minres {
@@ -712,81 +401,18 @@ static void apply_M(MpetSolver* S, const Precond<T>& P, const T* r, T* z, const 
   precond_apply(P, r, z, skip, s);
 }
 
-template <typename T>
-static int alloc_dev(MpetSolver* S, T** p, size_t count) {
-  void* q = nullptr;
-  if (hipMalloc(&q, sizeof(T) * (count ? count : 1)) != hipSuccess) {
-    mpet_set_error("hipMalloc failed (out of device memory)");
-    return ERR_OOM;
-  }
-  S->allocs.push_back(q);
-  *p = (T*)q;
-  return OK;
-}
-
-template <typename T>
-static int build_hier(MpetSolver* S, const MpetHierDesc* d, int NC, Hier<T>& H) {
-  H.nlev = d->nlev;
-  H.deg = d->degree;
-  H.NC = NC;
-  H.scale = d->scale != 0.0 ? d->scale : 1.0;
-  if (H.nlev < 1 || H.nlev > MAXLEV || H.deg < 1 || H.deg > MAXDEG) {
-    mpet_set_error("invalid hierarchy description");
-    return ERR_INVALID;
-  }
-  for (int l = 0; l < H.nlev; ++l) {
-    const MpetAmgLevelDesc& ld = d->lev[l];
-    LevelDev<T>& L = H.lev[l];
-    L.n = ld.n;
-    L.nc = ld.nc;
-    L.rp = ld.A_rp; L.ci = ld.A_ci; L.v = (const T*)ld.A_v; L.dinv = (const T*)ld.dinv;
-    L.Prp = ld.P_rp; L.Pci = ld.P_ci; L.Pv = (const T*)ld.P_v;
-    L.Rrp = ld.R_rp; L.Rci = ld.R_ci; L.Rv = (const T*)ld.R_v;
-    L.mask = l == 0 ? d->mask0 : nullptr;
-    L.lpr = ld.lpr ? ld.lpr : 8;
-    L.lprP = ld.lprP ? ld.lprP : 4;
-    L.lprR = ld.lprR ? ld.lprR : 8;
-    double lmax = ld.rho * d->eig_safety;
-    chebyshev(lmax / d->eig_ratio, lmax, H.deg, L.cd, L.cr);
-    size_t m = (size_t)NC * L.n;
-    int rc;
-    if ((rc = alloc_dev(S, &L.B, m)) || (rc = alloc_dev(S, &L.X, m)) ||
-        (rc = alloc_dev(S, &L.R, m)) || (rc = alloc_dev(S, &L.E, m)) ||
-        (rc = alloc_dev(S, &L.S1, m)) || (rc = alloc_dev(S, &L.S2, m)) ||
-        (rc = alloc_dev(S, &L.D, m)))
-      return rc;
-  }
-  H.cinv = (const T*)d->coarse_inv;
-  return OK;
-}
-
-template <typename T>
-static int build_precond(MpetSolver* S, const MpetPrecDesc* d, Precond<T>& P) {
-  int rc;
-  if ((rc = build_hier(S, &d->u, 3, P.u)) || (rc = build_hier(S, &d->p, 1, P.p))) return rc;
-  P.nP1 = d->nP1;
-  P.F = d->F;
-  P.nu = S->nu;
-  if (P.u.lev[0].n * 3ll != S->nu || (long long)P.p.lev[0].n != (long long)P.F * P.nP1) {
-    mpet_set_error("preconditioner sizes do not match the operator");
-    return ERR_INVALID;
-  }
-  size_t np = (size_t)P.F * P.nP1;
-  if ((rc = alloc_dev(S, &P.pin, np)) || (rc = alloc_dev(S, &P.pout, np))) return rc;
-  return OK;
-}
 
 template <typename T>
 static int build_minres(MpetSolver* S, Minres<T>& M) {
   M.n = S->n;
   M.nu = S->nu;
   int rc;
-  if ((rc = alloc_dev(S, &M.R[0], M.n)) || (rc = alloc_dev(S, &M.R[1], M.n)) ||
-      (rc = alloc_dev(S, &M.W[0], M.n)) || (rc = alloc_dev(S, &M.W[1], M.n)) ||
-      (rc = alloc_dev(S, &M.z, M.n)) || (rc = alloc_dev(S, &M.y, M.n)) ||
-      (rc = alloc_dev(S, &M.v, M.n)) || (rc = alloc_dev(S, &M.x, M.n)) ||
-      (rc = alloc_dev(S, &M.b, M.n)) || (rc = alloc_dev(S, &M.st, 1)) ||
-      (rc = alloc_dev(S, &M.partial, MAX_PARTIALS)))
+  if ((rc = alloc_dev(S->allocs, &M.R[0], M.n)) || (rc = alloc_dev(S->allocs, &M.R[1], M.n)) ||
+      (rc = alloc_dev(S->allocs, &M.W[0], M.n)) || (rc = alloc_dev(S->allocs, &M.W[1], M.n)) ||
+      (rc = alloc_dev(S->allocs, &M.z, M.n)) || (rc = alloc_dev(S->allocs, &M.y, M.n)) ||
+      (rc = alloc_dev(S->allocs, &M.v, M.n)) || (rc = alloc_dev(S->allocs, &M.x, M.n)) ||
+      (rc = alloc_dev(S->allocs, &M.b, M.n)) || (rc = alloc_dev(S->allocs, &M.st, 1)) ||
+      (rc = alloc_dev(S->allocs, &M.partial, MAX_PARTIALS)))
     return rc;
   HIPCHK(hipHostMalloc((void**)&M.st_host, sizeof(MinresState), hipHostMallocDefault));
   S->host_allocs.push_back(M.st_host);
@@ -992,20 +618,20 @@ extern "C" int mpet_solver_create(const MpetOpDesc* op64, const MpetOpDesc* op32
     S->csr_lpr = csr64->lpr ? csr64->lpr : 16;
   }
   int rc = OK;
-  if (prec64 && (rc = build_precond(S, prec64, S->pc64))) { mpet_solver_destroy(S); return rc; }
-  if (prec32 && (rc = build_precond(S, prec32, S->pc32))) { mpet_solver_destroy(S); return rc; }
+  if (prec64 && (rc = build_precond(S->allocs, S->nu, prec64, S->pc64))) { mpet_solver_destroy(S); return rc; }
+  if (prec32 && (rc = build_precond(S->allocs, S->nu, prec32, S->pc32))) { mpet_solver_destroy(S); return rc; }
   S->has_pc64 = prec64 != nullptr;
   S->has_pc32 = prec32 != nullptr;
   const bool need64 = S->opt.mode != 1, need32 = S->opt.mode == 1;
   if (need64 && (rc = build_minres(S, S->m64))) { mpet_solver_destroy(S); return rc; }
   if (need32 && (rc = build_minres(S, S->m32))) { mpet_solver_destroy(S); return rc; }
   if (S->opt.mode == 2 &&
-      ((rc = alloc_dev(S, &S->r32, S->n)) || (rc = alloc_dev(S, &S->z32, S->n)))) {
+      ((rc = alloc_dev(S->allocs, &S->r32, S->n)) || (rc = alloc_dev(S->allocs, &S->z32, S->n)))) {
     mpet_solver_destroy(S);
     return rc;
   }
-  if ((rc = alloc_dev(S, &S->r64, S->n)) || (rc = alloc_dev(S, &S->g64, S->n)) ||
-      (rc = alloc_dev(S, &S->scal, 1)) || (rc = alloc_dev(S, &S->partial, MAX_PARTIALS))) {
+  if ((rc = alloc_dev(S->allocs, &S->r64, S->n)) || (rc = alloc_dev(S->allocs, &S->g64, S->n)) ||
+      (rc = alloc_dev(S->allocs, &S->scal, 1)) || (rc = alloc_dev(S->allocs, &S->partial, MAX_PARTIALS))) {
     mpet_solver_destroy(S);
     return rc;
   }

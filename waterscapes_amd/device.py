@@ -26,7 +26,7 @@ class DeviceSystem:
     """Assembled total-pressure system of one MPET instance on one GPU."""
 
     def __init__(self, space: MixedSpace, params, dt, theta, K_cells=None, device="cuda",
-                 build_csr=False, need_fp32=True):
+                 build_csr=False, need_fp32=True, structured=True):
         if not torch.cuda.is_available():
             raise RuntimeError("DeviceSystem needs a GPU (no CPU fallback)")
         self.device = torch.device(device)
@@ -80,8 +80,9 @@ class DeviceSystem:
         self.nnz = {k: int(v[1].size) for k, v in self.pat_host.items()}
         self.dmask = torch.zeros(self.n, dtype=torch.uint8, device=dev)
         self._assemble()
-        self.op64 = self._structured(0)
-        self.op32 = self._structured(1) if need_fp32 else None
+        # structured operator copies (a batch packs its own from the primitives)
+        self.op64 = self._structured(0) if structured else None
+        self.op32 = self._structured(1) if structured and need_fp32 else None
         self.csr = self.expand_csr(0) if build_csr else None
 
     # -- assembly --------------------------------------------------------------

@@ -33,11 +33,13 @@ class AmgLevelDesc(C.Structure):
 class HierDesc(C.Structure):
     _fields_ = [("nlev", C.c_int32), ("lev", AmgLevelDesc * MAXLEV), ("coarse_inv", C.c_void_p),
                 ("degree", C.c_int32), ("eig_ratio", C.c_double), ("eig_safety", C.c_double),
-                ("scale", C.c_double), ("mask0", C.c_void_p)]
+                ("scale", C.c_double), ("mask0", C.c_void_p), ("coarse_blk", C.c_void_p),
+                ("coarse_off", C.c_void_p), ("coarse_ioff", C.c_void_p)]
 
 
 class PrecDesc(C.Structure):
-    _fields_ = [("u", HierDesc), ("p", HierDesc), ("nP1", C.c_int32), ("F", C.c_int32)]
+    _fields_ = [("u", HierDesc), ("p", HierDesc), ("nP1", C.c_int32), ("F", C.c_int32),
+                ("pmap", C.c_void_p)]
 
 
 class CsrDesc(C.Structure):
@@ -115,7 +117,7 @@ def _pick_lpr(avg, nc=1):
 class KrylovSolver:
     """Owns the device preconditioner data and the native solver handle."""
 
-    def __init__(self, system, bc_dofs, config: SolverConfig = None):
+    def __init__(self, system, bc_dofs, config: SolverConfig = None, create_handle=True):
         self.sys = system
         self.cfg = cfg = config or SolverConfig()
         dev = system.device
@@ -149,6 +151,10 @@ class KrylovSolver:
             raise ValueError("p0_weight must be 'robust' or 'reference'")
         self._build_hierarchy()
         self._build_pressure()
+        self.handle = None
+        self.last_stats = None
+        if not create_handle:        # host hierarchies only (packed by a batch)
+            return
         L = _bind()
         self.prec64 = self._prec_desc(0) if not pc32 else None
         self.prec32 = self._prec_desc(1) if pc32 else None

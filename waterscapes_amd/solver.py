@@ -127,7 +127,7 @@ class StepEngine:
     """Device-resident state of one MPET instance and its time step."""
 
     def __init__(self, problem, space, dt, theta, config: SolverConfig, device="cuda",
-                 extrapolate=False):
+                 extrapolate=False, batched=False):
         self.problem = problem
         self.extrapolate = bool(extrapolate)
         self.prev = None
@@ -146,8 +146,12 @@ class StepEngine:
             raise NotImplementedError("traction s is not supported by the GPU backend yet")
         if problem.u_has_nullspace or any(problem.p_has_nullspace):
             raise NotImplementedError("nullspace (rigid-motion / constant) handling")
+        # batched: the batch packs the operator and hierarchies of all its
+        # instances, so no per-instance structured copies or solver handle
         self.system = DeviceSystem(space, params, dt, theta, K_cells=cellK, device=device,
-                                   build_csr=(config.mode == "fp64" and config.use_csr is not False))
+                                   build_csr=(not batched and config.mode == "fp64"
+                                              and config.use_csr is not False),
+                                   structured=not batched)
         # Dirichlet conditions (create_dirichlet_bcs, :123-144)
         self.bcs = [DirichletBC(space, 0, problem.u_bar, problem.momentum_boundary_markers, 0)]
         for i in range(A):
@@ -155,7 +159,8 @@ class StepEngine:
                                         problem.continuity_boundary_markers[i], 0))
         self.bc_dofs_host = np.concatenate([bc.dofs for bc in self.bcs]).astype(np.int64)
         self.bc_dofs = torch.as_tensor(self.bc_dofs_host).to(self.device)
-        self.krylov = KrylovSolver(self.system, [bc.dofs for bc in self.bcs], config)
+        self.krylov = KrylovSolver(self.system, [bc.dofs for bc in self.bcs], config,
+                                   create_handle=not batched)
         self.config = config
         # sources
         nu, F = space.nu, space.A + 1
