@@ -214,6 +214,33 @@ int mpet_cell_source(const MpetMeshDesc* mesh, const int* test_nodes, int ntest,
                      long long dof_base, int dof_stride, int dof_off, double* b,
                      hipStream_t stream);
 
+/* Exterior-facet terms, step 1 (per facet): for facet f, test node a,
+ * component c:
+ *   loc[f][a][c] = scale |F_f| sum_m sum_n T[a][m][n] u[f][m][c] w[f][n]
+ * with T (ntest x nm x nn) a reference-triangle table, u either an array
+ * (u + f*u_fstride + m*ncomp + c; u_fstride 0 broadcasts one block) or
+ * gathered from the vector xg at gdof[f*nm + m] (ncomp 1), and w optional
+ * (nn values per facet; NULL = 1).  Replaces the facet integrals of
+ * assemble(L0) ds_m(1) (:313, :482), assemble(L1_i) ds_i(1) (:322, :471),
+ * rhs(L2_i) and lhs(L2_i) ds_i(2) (:324, :439-441, :474-476). */
+int mpet_facet_local(int nF, int ntest, int nm, int nn, int ncomp, const double* T,
+                     const double* area, const double* u, long long u_fstride,
+                     const double* xg, const int* gdof, const double* w, double scale,
+                     double* loc, hipStream_t stream);
+
+/* Step 2 for vectors: b[row_dof[r] + c] += sum_{k in inc_rp[r]..} loc[inc[k]][c]
+ * (inc[k] = f * ntest + a: the incident facets of boundary node r, fixed
+ * order; one writer per entry, no atomics). */
+int mpet_facet_scatter_vec(int nrows, int ntest, int ncomp, const long long* row_dof,
+                           const int* inc_rp, const int* inc, const double* loc, double* b,
+                           hipStream_t stream);
+
+/* Step 2 for the Robin matrix: vals (P1 node pattern rp11/ci11) at
+ * (row_node[r], fnode[3f + b]) += loc[f][a][b] for (f, a) incident to r. */
+int mpet_facet_scatter_mat(int nrows, const int* row_node, const int* inc_rp, const int* inc,
+                           const int* fnode, const int* rp11, const int* ci11,
+                           const double* loc, double* vals, hipStream_t stream);
+
 /* y = A x (b == NULL) or y = b - A x, Dirichlet rows zeroed: the MatMult
  * of PETSc's KSP on the symmetrically eliminated Acopy (:457, :489, :493)
  * for iterates with zero Dirichlet components; with b it is the lifting

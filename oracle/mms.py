@@ -54,6 +54,26 @@ def exact_solutions(params, u_expr=None, p_exprs=None):
     return u, p, f, g
 
 
+def boundary_data(params, u, p, beta):
+    """Natural boundary data of the exact solution (sympy):
+      sigma = 2 mu eps(u) + p_0 I, traction s = sigma . n (the reference
+      test's sigma, test_convergence_totalpressuresolver.py:60-70, :133-138);
+      network flux I_i = -K_i grad p_i . n (the weak form's boundary term
+      +dt int I_i w_i, :322, against -dt int (K_i grad p_i . n) w_i from
+      integrating div(K_i grad p_i) by parts);
+      Robin data p_robin_i = p_i + (K_i grad p_i . n) / beta_i, so that
+      K_i grad p_i . n = -beta_i (p_i - p_robin_i) (:324).
+    Returns (sigma 3x3, grad_p list of 3-vectors per network)."""
+    E, nu = params["E"], params["nu"]
+    mu = E / (2.0 * (1.0 + nu))
+    xs = (X, Y, Z)
+    grad_u = [[S.diff(u[i], xs[j]) for j in range(3)] for i in range(3)]
+    sigma = [[mu * (grad_u[i][j] + grad_u[j][i]) + (p[0] if i == j else 0) for j in range(3)]
+             for i in range(3)]
+    grads = [[S.diff(p[i + 1], v) for v in xs] for i in range(int(params["J"]))]
+    return sigma, grads
+
+
 def lambdify(expr):
     """numpy callable f(X (n,3), t) -> (n,) for a scalar sympy expression."""
     fn = S.lambdify((X, Y, Z, T), expr, modules="numpy")

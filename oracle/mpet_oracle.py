@@ -531,7 +531,7 @@ class OracleProblem:
                  p_bar=None, f_degree=3, g_degree=3, K_cells=None,
                  momentum_markers=None, continuity_markers=None,
                  s=None, s_degree=3, I=None, I_degree=3, beta=None,
-                 p_robin=None, robin_degree=1):
+                 p_robin=None, robin_degree=1, p_robin_degree=None):
         self.m = mesh
         self.params = params
         self.A = int(params["J"])
@@ -544,6 +544,7 @@ class OracleProblem:
         self.s, self.s_degree = s, s_degree
         self.I, self.I_degree = I, I_degree
         self.beta, self.p_robin, self.robin_degree = beta, p_robin, robin_degree
+        self.p_robin_degree = robin_degree if p_robin_degree is None else p_robin_degree
         self.K_cells = K_cells
         self.lay = Layout(mesh["nP2"], mesh["nP1"], self.A)
         nF = mesh["facets"].shape[0]
@@ -699,12 +700,11 @@ class OracleProblem:
                 Mf = facet_source_matrix(m["coords"], tri_v, self.beta[i], self.robin_degree, t_theta)
                 loc = (1 - self.theta) * np.einsum("fab,fb->fa", Mf, pprev)
                 if self.p_robin is not None and self.p_robin[i] is not None:
-                    # int beta p_robin w with the product beta*p_robin interpolated
-                    pr = facet_source_vector(
-                        m["coords"], tri_v, 3,
-                        lambda X, t, bi=self.beta[i], pi_=self.p_robin[i]:
-                            (np.asarray(bi(X, t)) * np.asarray(pi_(X, t))).reshape(-1, 1),
-                        self.robin_degree, t_theta)[..., 0]
+                    # int beta_h p_robin_h w: each coefficient interpolated in
+                    # its own P_k (FFC coefficient semantics), product exact
+                    pr = facet_product_vector(m["coords"], tri_v, self.beta[i],
+                                              self.robin_degree, self.p_robin[i],
+                                              self.p_robin_degree, t_theta)
                     loc = loc - pr
                 b += self.dt * scatter_vector(lay.n, lay.pdofs(tri_n, i + 1), loc)
         # L0: f . v dx + s . v ds_m(1) at t_{n+1}
@@ -795,6 +795,27 @@ def facet_source_matrix(coords, tris, beta, degree, t):
     lam = np.stack([1 - x - y, x, y], axis=1)
     bq = np.einsum("qm,fm->fq", Lk, bv)
     return np.einsum("q,f,qa,qb,fq->fab", qw, a2, lam, lam, bq)
+
+
+def facet_product_vector(coords, tris, f1, k1, f2, k2, t):
+    """(nF,3) int_F f1_h f2_h psi_a, f1 and f2 interpolated into P_k1 / P_k2
+    on each facet, the product integrated exactly."""
+    v, e1, e2, a2 = facet_geometry(coords, tris)
+    vals = []
+    for f, k in ((f1, k1), (f2, k2)):
+        kn = lagrange_nodes(k, 2)
+        X = v[:, 0][:, None, :] + kn[None, :, 0, None] * e1[:, None, :] \
+            + kn[None, :, 1, None] * e2[:, None, :]
+        nF, nk = X.shape[:2]
+        vals.append(np.asarray(f(X.reshape(-1, 3), t), float).reshape(nF, nk))
+    qp, qw = tri_rule(k1 + k2 + 2)
+    L1 = lagrange_eval(k1, qp, 2) if k1 > 0 else np.ones((len(qp), 1))
+    L2 = lagrange_eval(k2, qp, 2) if k2 > 0 else np.ones((len(qp), 1))
+    x, y = qp[:, 0], qp[:, 1]
+    lam = np.stack([1 - x - y, x, y], axis=1)
+    q1 = np.einsum("qm,fm->fq", L1, vals[0])
+    q2 = np.einsum("qm,fm->fq", L2, vals[1])
+    return np.einsum("q,f,qa,fq,fq->fa", qw, a2, lam, q1, q2)
 
 
 def outward_normals(coords, cells, cl, verts):

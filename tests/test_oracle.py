@@ -236,3 +236,151 @@ def test_library_exports_every_header_symbol():
     assert len(names) >= 15
     for nm in names:
         assert hasattr(L, nm), nm
+
+
+# ---------------------------------------------------------------------------
+# exterior-facet terms (traction s, fluxes I_i, Robin beta_i / p_robin_i)
+# ---------------------------------------------------------------------------
+def _tri(seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((2, 3, 3)), np.array([[0, 1, 2], [3, 4, 5]])
+
+
+def test_facet_integrals_closed_forms():
+    """P1 facet mass |F|/12 (1 + delta); partition of unity of the P1 and
+    P2 facet test functions; products of interpolants are exact."""
+    X, tris = _tri(3)
+    coords = X.reshape(-1, 3)
+    v = coords[tris]
+    area = 0.5 * np.linalg.norm(np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]), axis=1)
+    M = O.facet_p1_mass(coords, tris)
+    for f in range(2):
+        assert np.allclose(M[f], area[f] / 12.0 * (np.ones((3, 3)) + np.eye(3)), rtol=1e-13)
+    # beta constant: facet_source_matrix == beta * P1 facet mass
+    Mb = O.facet_source_matrix(coords, tris, lambda P, t: np.full(P.shape[0], 2.5), 0, 0.0)
+    assert np.allclose(Mb, 2.5 * M, rtol=1e-13)
+    # linear data with P1 test: int f_h psi_a = M f_nodes (f linear is its own interpolant)
+    lin = lambda P, t: (1.0 + P[:, 0] - 2.0 * P[:, 1] + 0.5 * P[:, 2]).reshape(-1, 1)  # noqa
+    loc = O.facet_source_vector(coords, tris, 3, lin, 1, 0.0)[..., 0]
+    fn = lin(coords, 0.0)[:, 0][tris]
+    assert np.allclose(loc, np.einsum("fab,fb->fa", M, fn), rtol=1e-12)
+    # partition of unity (P2 test functions): sum_a int f v_a = int f
+    loc2 = O.facet_source_vector(coords, tris, 6, lin, 1, 0.0)[..., 0]
+    assert np.allclose(loc2.sum(1), loc.sum(1), rtol=1e-12)
+    # product of a constant and a linear interpolant
+    pv = O.facet_product_vector(coords, tris, lambda P, t: np.full(P.shape[0], 3.0), 0,
+                                lambda P, t: lin(P, t)[:, 0], 1, 0.0)
+    assert np.allclose(pv, 3.0 * np.einsum("fab,fb->fa", M, fn), rtol=1e-12)
+
+
+UNIT = dict(J=2, c=(1.0, 1.0), alpha=(1.0, 1.0), K=(1.0, 1.0), S=((1.0, 1.0), (1.0, 1.0)),
+            E=1.0, nu=0.35)       # the reference test's parameters (:117-125)
+
+
+def _right_face_markers(mesh, A):
+    mid = mesh.facet_midpoints()
+    right = mesh.exterior & (np.abs(mid[:, 0] - 1.0) < 1e-12)
+    mk_u = _all_dirichlet_markers(mesh)
+    mk_u[right] = 1
+    mk_p = [_all_dirichlet_markers(mesh) for _ in range(A)]
+    mk_p[0][right] = 1
+    mk_p[1][right] = 2
+    return mk_u, mk_p
+
+
+def _facet_mms_problem(n, params, theta=1.0, dt=1.0, T=1.0):
+    """Dirichlet everywhere except the face x = 1: traction s there for the
+    momentum equation, flux I_1 for network 1 and a Robin condition for
+    network 2 (beta = 2), all from the exact solution."""
+    mesh, space, m = _small_problem(n, int(params["J"]))
+    A = int(params["J"])
+    # pressures with a non-zero normal derivative on x = 1
+    pb = mms.S.cos(mms.S.pi * mms.X / 3) * mms.S.cos(mms.S.pi * mms.Y) * mms.S.cos(mms.S.pi * mms.Z)
+    u, p, f, g = mms.exact_solutions(params, p_exprs=[(j + 1) * mms.T * pb for j in range(A)])
+    sigma, grads = mms.boundary_data(params, u, p, 2.0)
+    K = params["K"]
+    mk_u, mk_p = _right_face_markers(mesh, A)
+    sig = [[mms.lambdify(sigma[i][j]) for j in range(3)] for i in range(3)]
+
+    def traction(P, t, nrm):
+        S_ = np.stack([np.stack([sig[i][j](P, t) for j in range(3)], -1) for i in range(3)], -2)
+        return np.einsum("nij,nj->ni", S_, nrm)
+    # outward normal of the face x = 1 is e_x
+    I1 = mms.lambdify(-K[0] * grads[0][0])
+    pr2 = mms.lambdify(p[2] + K[1] * grads[1][0] / 2.0)
+    prob = O.OracleProblem(
+        m, params, dict(dt=dt, theta=theta, T=T), f=mms.lambdify_vec(f),
+        g=[mms.lambdify(gi) for gi in g], u_bar=mms.lambdify_vec(u),
+        p_bar=[mms.lambdify(p[j + 1]) for j in range(A)], f_degree=3, g_degree=3,
+        momentum_markers=mk_u, continuity_markers=mk_p, s=traction, s_degree=3,
+        I=[I1, None], I_degree=3, beta=[None, lambda P, t: np.full(P.shape[0], 2.0)],
+        p_robin=[None, pr2], robin_degree=0, p_robin_degree=3)
+    return mesh, space, m, prob, (u, p, f, g)
+
+
+def _initial(prob, m, u, p):
+    lay = prob.lay
+    up0 = np.zeros(lay.n)
+    up0[: lay.nu] = mms.lambdify_vec(u)(m["x2"], 0.0).reshape(-1)
+    for j in range(len(p)):
+        up0[lay.pdofs(np.arange(m["nP1"]), j)] = mms.lambdify(p[j])(m["coords"], 0.0)
+    return up0
+
+
+@pytest.mark.parametrize("theta", [1.0, 0.5])
+def test_facet_terms_known_answer(theta):
+    """Steady linear pressures with u = 0 (S = 0, so g = 0 and f = -grad p_0
+    is constant) are represented exactly by P1/P2: with the exact traction
+    (p_0 n), flux (-K grad p_1 . n) and Robin data on x = 1 and Dirichlet
+    data elsewhere, one step must reproduce them to round-off.  A wrong sign,
+    scale or time level of any facet term leaves an O(1) error."""
+    params = dict(UNIT)
+    params["S"] = ((0.0, 0.0), (0.0, 0.0))
+    mesh, space, m = _small_problem(2, 2)
+    Xs, Ys, Zs = mms.X, mms.Y, mms.Z
+    pj = [1.0 + 0.5 * Xs + 0.2 * Ys, 2.0 - 0.3 * Xs + 0.1 * Zs]
+    p0 = -(pj[0] + pj[1])
+    u = [0 * Xs, 0 * Xs, 0 * Xs]
+    f = [-mms.S.diff(p0, v) for v in (Xs, Ys, Zs)]
+    mk_u, mk_p = _right_face_markers(mesh, 2)
+    P0 = mms.lambdify(p0)
+    prob = O.OracleProblem(
+        m, params, dict(dt=0.7, theta=theta, T=0.7), f=mms.lambdify_vec(f),
+        g=None, u_bar=mms.lambdify_vec(u), p_bar=[mms.lambdify(q) for q in pj], f_degree=1,
+        momentum_markers=mk_u, continuity_markers=mk_p,
+        s=lambda P, t, nrm: P0(P, t)[:, None] * nrm, s_degree=1,
+        I=[mms.lambdify(-1.0 * mms.S.diff(pj[0], Xs)), None], I_degree=0,
+        beta=[None, lambda P, t: np.full(P.shape[0], 2.0)],
+        p_robin=[None, mms.lambdify(pj[1] + mms.S.diff(pj[1], Xs) / 2.0)], robin_degree=0,
+        p_robin_degree=1)
+    exact = _initial(prob, m, u, [p0] + pj)
+    (up, t), = list(prob.solve_direct(exact.copy()))
+    assert np.abs(up - exact).max() <= 1e-10 * np.abs(exact).max(), np.abs(up - exact).max()
+    # a flipped flux sign is detected
+    b_ok, _, _ = prob.rhs(exact, 0.0)
+    prob.I = [mms.lambdify(1.0 * mms.S.diff(pj[0], Xs)), None]
+    b_bad, _, _ = prob.rhs(exact, 0.0)
+    assert np.abs(b_bad - b_ok).max() > 1e-3
+
+
+def test_mms_convergence_rates_with_facet_terms():
+    """Traction, flux and Robin data derived from an exact solution with
+    non-zero normal derivatives on x = 1 keep second-order L2 rates."""
+    rate = lambda a, b: np.log(a / b) / np.log(2.0)  # noqa: E731
+    errs = []
+    for n in (3, 6):
+        mesh, space, m, prob, (u, p, f, g) = _facet_mms_problem(n, UNIT)
+        for up, t in prob.solve_direct(_initial(prob, m, u, p)):
+            pass
+        lay = prob.lay
+        e = {}
+        uh = up[: lay.nu].reshape(-1, 3)
+        e["u"] = np.sqrt(sum(mms.error_norms(m["coords"], m["cells"], m["cell_p2"], m["x2"],
+                                             uh[:, k], u[k], t, 2)[0] ** 2 for k in range(3)))
+        for j in (1, 2):
+            e[f"p{j}"] = mms.error_norms(m["coords"], m["cells"], m["cell_p1"], m["coords"],
+                                         up[lay.pdofs(np.arange(m["nP1"]), j)], p[j], t, 1)[0]
+        errs.append(e)
+    ru = rate(errs[0]["u"], errs[1]["u"])
+    rp1, rp2 = rate(errs[0]["p1"], errs[1]["p1"]), rate(errs[0]["p2"], errs[1]["p2"])
+    assert ru > 1.70 and rp1 > 1.70 and rp2 > 1.70, (ru, rp1, rp2, errs)

@@ -343,7 +343,7 @@ class MPETBatch:
             keep += [rp, ci, v, dinv]
             L.A_rp, L.A_ci, L.A_v, L.dinv = ptr(rp), ptr(ci), ptr(v), ptr(dinv)
             L.rho = lev["rho"]
-            L.lpr = _pick_lpr(ci.numel() / max(lev["n"], 1), nc)
+            L.lpr = _pick_lpr(ci.numel() / max(lev["n"], 1), nc, lev["n"])
             if lev["nc"]:
                 arrs = []
                 for k in ("P", "R"):
@@ -355,8 +355,8 @@ class MPETBatch:
                 (L.P_rp, L.P_ci, L.P_v), (L.R_rp, L.R_ci, L.R_v) = \
                     [tuple(ptr(t) for t in a) for a in arrs]
                 nnzp = arrs[0][1].numel()
-                L.lprP = _pick_lpr(nnzp / max(lev["n"], 1), nc)
-                L.lprR = _pick_lpr(nnzp / max(lev["nc"], 1), nc)
+                L.lprP = _pick_lpr(nnzp / max(lev["n"], 1), nc, lev["n"])
+                L.lprR = _pick_lpr(nnzp / max(lev["nc"], 1), nc, lev["nc"])
         inv, blk, off, ioff = coarse
         cinv = torch.as_tensor(inv).to(dev, tdt)
         cblk = torch.as_tensor(blk).to(dev)

@@ -26,7 +26,7 @@ class DeviceSystem:
     """Assembled total-pressure system of one MPET instance on one GPU."""
 
     def __init__(self, space: MixedSpace, params, dt, theta, K_cells=None, device="cuda",
-                 build_csr=False, need_fp32=True, structured=True):
+                 build_csr=False, need_fp32=True, structured=True, robin=None):
         if not torch.cuda.is_available():
             raise RuntimeError("DeviceSystem needs a GPU (no CPU fallback)")
         self.device = torch.device(device)
@@ -80,6 +80,7 @@ class DeviceSystem:
         self.nnz = {k: int(v[1].size) for k, v in self.pat_host.items()}
         self.dmask = torch.zeros(self.n, dtype=torch.uint8, device=dev)
         self._assemble()
+        self._add_robin(robin or [])
         # structured operator copies (a batch packs its own from the primitives)
         self.op64 = self._structured(0) if structured else None
         self.op32 = self._structured(1) if structured and need_fp32 else None
@@ -101,9 +102,27 @@ class DeviceSystem:
         t = self.prim_t
         self.prim = _lib.Primitives(ptr(t["Kuu"]), ptr(t["Lap"]), ptr(t["Dv"]), ptr(t["Dt"]),
                                     ptr(t["M1"]), ptr(t["St"]))
+        self.primK = self.prim
         check(_lib.lib().mpet_assemble_primitives(
             C.byref(self.mesh_desc), C.byref(self.pat), ptr(self.Kcell), self.A, self.mu,
             C.byref(self.prim), _lib.stream_ptr()))
+
+    def _add_robin(self, robin):
+        """Robin terms of lhs(L2_i) (:324, added to A at :439-441): A gets
+        -dt theta int_{ds_i(2)} beta_i p_i w_i, i.e. St_i += int beta_i psi psi
+        (St_i is scaled by -dt theta in A).  The preconditioner prec (:279-283)
+        and the up_-linear right-hand side keep the Robin-free St ("StK")."""
+        t = self.prim_t
+        t["StK"] = t["St"]
+        if not robin:
+            return
+        from .facets import robin_matrix
+        t["StK"] = t["St"].clone()
+        n11 = self.nnz["11"]
+        for i, fs, beta, time in robin:
+            robin_matrix(fs, beta, time, self.pat_dev["11"], t["St"][i * n11:(i + 1) * n11])
+        self.primK = _lib.Primitives(ptr(t["Kuu"]), ptr(t["Lap"]), ptr(t["Dv"]), ptr(t["Dt"]),
+                                     ptr(t["M1"]), ptr(t["StK"]))
 
     def _structured(self, dtype):
         """SoA copy of the primitives in float64 (dtype 0) or float32 (1)."""
@@ -159,7 +178,8 @@ class DeviceSystem:
             coef = _lib.make_coef(self.Cm, self.Cs, self.mu)
         else:
             coef = _lib.make_coef(np.diag(self.Pd), self.Ps, self.mu)
-        check(_lib.lib().mpet_expand_csr(C.byref(self.pat), C.byref(self.prim), self.nP2,
+        prim = self.prim if which == 0 else self.primK
+        check(_lib.lib().mpet_expand_csr(C.byref(self.pat), C.byref(prim), self.nP2,
                                          self.nP1, self.A, C.byref(coef), which, ptr(rp),
                                          ptr(cols), ptr(vals), _lib.stream_ptr()))
         return rp, cols, vals
@@ -199,7 +219,7 @@ class DeviceSystem:
 
     def rhs_pressure(self, up_prev, b):
         coef = _lib.make_coef(self.Bm, self.Bs, self.mu)
-        check(_lib.lib().mpet_rhs_pressure(C.byref(self.pat), C.byref(self.prim), self.nP2,
+        check(_lib.lib().mpet_rhs_pressure(C.byref(self.pat), C.byref(self.primK), self.nP2,
                                            self.nP1, self.A, C.byref(coef), ptr(up_prev),
                                            ptr(b), _lib.stream_ptr()))
         return b

@@ -31,6 +31,7 @@ import torch
 
 from . import forms, refel
 from .device import DeviceSystem
+from .facets import FacetSet, FacetTerm
 from .fem import Constant, Function, MixedSpace
 from .krylov import KrylovSolver, SolverConfig
 from .problem import resolved_params
@@ -137,21 +138,26 @@ class StepEngine:
         params, cellK = resolved_params(problem)
         self.params = params
         A = space.A
-        for j in range(A):
-            for name, lst in (("I", problem.I), ("beta", problem.beta)):
-                if not _is_zero(lst[j]):
-                    raise NotImplementedError(
-                        f"facet terms ({name}) are not supported by the GPU backend yet")
-        if not _is_zero(problem.s):
-            raise NotImplementedError("traction s is not supported by the GPU backend yet")
         if problem.u_has_nullspace or any(problem.p_has_nullspace):
             raise NotImplementedError("nullspace (rigid-motion / constant) handling")
+        # exterior-facet terms (markers 1 = Neumann, 2 = Robin, :309-324)
+        self.t0 = float(problem.time)
+        self.robin = []
+        robin_lhs = []
+        for i in range(A):
+            if _is_zero(problem.beta[i]):
+                continue
+            fs = FacetSet(space, problem.continuity_boundary_markers[i], ROBIN_MARKER,
+                          self.device)
+            if fs.n:
+                self.robin.append((i, fs))
+                robin_lhs.append((i, fs, problem.beta[i], None))
         # batched: the batch packs the operator and hierarchies of all its
         # instances, so no per-instance structured copies or solver handle
         self.system = DeviceSystem(space, params, dt, theta, K_cells=cellK, device=device,
                                    build_csr=(not batched and config.mode == "fp64"
                                               and config.use_csr is not False),
-                                   structured=not batched)
+                                   structured=not batched, robin=robin_lhs)
         # Dirichlet conditions (create_dirichlet_bcs, :123-144)
         self.bcs = [DirichletBC(space, 0, problem.u_bar, problem.momentum_boundary_markers, 0)]
         for i in range(A):
@@ -172,6 +178,25 @@ class StepEngine:
             gi = problem.g[i]
             if not gi.is_zero():
                 self.g_src.append(_Source(self, gi, 1, 1, self.dt, nu, F, i + 1))
+        # facet sources: traction s on ds_m(1) (P2 test, 3 components), fluxes
+        # I_i on ds_i(1) and Robin data on ds_i(2) (P1 test, network i rows)
+        self.s_term = None
+        if not _is_zero(problem.s):
+            fs = FacetSet(space, problem.momentum_boundary_markers, NEUMANN_MARKER, self.device)
+            if fs.n:
+                self.s_term = FacetTerm(fs, 2, 0, 3, 0, 3)
+        self.I_terms = []
+        for i in range(A):
+            if _is_zero(problem.I[i]):
+                continue
+            fs = FacetSet(space, problem.continuity_boundary_markers[i], NEUMANN_MARKER,
+                          self.device)
+            if fs.n:
+                self.I_terms.append((i, FacetTerm(fs, 1, nu, F, i + 1, 1)))
+        self.robin_terms = []
+        for i, fs in self.robin:
+            gdof = torch.as_tensor((nu + F * fs.p1 + (i + 1)).astype(np.int32)).to(self.device)
+            self.robin_terms.append((i, FacetTerm(fs, 1, nu, F, i + 1, 1), gdof))
         self.b = torch.empty(space.ndofs, dtype=torch.float64, device=self.device)
         self.stats = []
 
@@ -184,15 +209,26 @@ class StepEngine:
         Expressions bound to it see the same values.  Returns t_{n+1}."""
         b = self.b
         time = self.problem.time
+        prob = self.problem
+        dt, theta = float(self.dt), self.theta
         self.system.rhs_pressure(up_prev, b)
-        t_theta = float(t_n) + self.theta * float(self.dt)
+        t_theta = float(t_n) + theta * dt
         time.assign(t_theta)
-        for src in self.g_src:
+        for src in self.g_src:                      # dt g_i w_i dx          (:322)
             src.add(b, None)
-        t_np1 = float(time) + (1.0 - self.theta) * float(self.dt)
+        for i, term in self.I_terms:                # dt I_i w_i ds_i(1)     (:322)
+            term.add_source(b, prob.I[i], dt)
+        for i, term, gdof in self.robin_terms:      # rhs(L2_i) on ds_i(2)   (:324)
+            if theta != 1.0:
+                term.add_robin_prev(b, prob.beta[i], up_prev, gdof, dt * (1.0 - theta))
+            if not _is_zero(prob.p_robin[i]):
+                term.add_product(b, prob.p_robin[i], prob.beta[i], -dt)
+        t_np1 = float(time) + (1.0 - theta) * dt
         time.assign(t_np1)
-        if self.f_src is not None:
+        if self.f_src is not None:                  # f . v dx               (:313)
             self.f_src.add(b, None)
+        if self.s_term is not None:                 # s . v ds_m(1)          (:313)
+            self.s_term.add_source(b, prob.s, 1.0)
         return t_np1
 
     def step(self, up_prev, x, t_n, testing_guess=False):
