@@ -161,6 +161,170 @@ def cpu_baseline(n_sample, A, steps, budget_s=30.0):
     return dict(ndofs=space.ndofs, per_step=per_step, setup=setup, steps=k)
 
 
+def uneven_instance(rng, big=False):
+    """One instance of the uneven-batch workload family: N ~ U{8..32} (48 for
+    a large request), A ~ U{1..4} (4), E log-uniform [1e3, 1e5], nu ~
+    U[0.30, 0.49], K_j log-uniform [1e-6, 1e-2], S_ij log-uniform
+    [1e-4, 1e-2] (symmetric), alpha_j = 0.25, c_j = 1e-5; u = 0 on z = 0,
+    p_j = sin(2 pi t) on z = 1; theta = 1, dt = 0.0125."""
+    N = 48 if big else int(rng.integers(8, 33))
+    A = 4 if big else int(rng.integers(1, 5))
+    E = float(10 ** rng.uniform(3, 5))
+    nu = float(rng.uniform(0.30, 0.49))
+    K = tuple(float(k) for k in 10 ** rng.uniform(-6, -2, A))
+    S = np.zeros((A, A))
+    for i in range(A):
+        for j in range(i + 1, A):
+            S[i, j] = S[j, i] = 10 ** rng.uniform(-4, -2)
+    params = dict(J=A, E=E, nu=nu, alpha=(0.25,) * A, c=(1e-5,) * A, K=K,
+                  S=[list(r) for r in S])
+    return N, params
+
+
+def make_uneven_solver(N, params, precision="mixed", **extra):
+    from waterscapes_amd import (MPETProblem, MPETTotalPressureSolver, Constant, Expression,
+                                 UnitCubeMesh, CompiledSubDomain)
+    A = params["J"]
+    mesh = UnitCubeMesh(N, N, N)
+    time_c = Constant(0.0)
+    problem = MPETProblem(mesh, time_c, params=params)
+    problem.p_bar = [Expression("sin(2.0*M_PI*t)", t=time_c, degree=1) for _ in range(A)]
+    CompiledSubDomain("on_boundary && near(x[2], 0.0)").mark(problem.momentum_boundary_markers, 0)
+    for j in range(A):
+        CompiledSubDomain("on_boundary && near(x[2], 1.0)").mark(
+            problem.continuity_boundary_markers[j], 0)
+    sp = dict(dt=0.0125, theta=1.0, T=1e9, direct_solver=False, precision=precision,
+              krylov_rtol=1e-10)
+    sp.update(extra)
+    return MPETTotalPressureSolver(problem, sp)
+
+
+def main_batch(args):
+    """Uneven batch: `--instances` instances stepped together (mixed), vs the
+    same instances one at a time with the plain FP64 GPU path (timed on a
+    random sample of `--sample` instances, scaled to the batch)."""
+    import torch
+    from waterscapes_amd.batch import MPETBatch
+    rng = np.random.default_rng(args.seed)
+    specs = [uneven_instance(rng) for _ in range(args.instances)]
+    t0 = time.perf_counter()
+    solvers = [make_uneven_solver(N, p) for N, p in specs]
+    batch = MPETBatch(solvers, precision="mixed", outer_tol=args.outer_tol,
+                      inner_rtol=args.inner_rtol)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    ndofs = sum(s.up.data.numel() for s in solvers)
+    print(f"[bench] batch of {len(solvers)} ({ndofs} DOFs) set up in {setup:.1f}s",
+          file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        batch.step()
+        batch.commit()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    iters = []
+    for _ in range(args.steps):
+        st = batch.step()
+        batch.commit()
+        iters.append(max(int(st[k].inner_iterations) for k in range(batch.K)))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t1
+    inst_steps = batch.K * args.steps
+    mem = torch.cuda.max_memory_allocated()
+    batch.close()
+    del batch, solvers
+    import gc
+    gc.collect()
+    torch.cuda.empty_cache()
+    # plain FP64 per-instance path on a sample
+    idx = rng.choice(len(specs), size=min(args.sample, len(specs)), replace=False)
+    seq_time, seq_steps = 0.0, 0
+    for i in idx:
+        N, p = specs[i]
+        s = make_uneven_solver(N, p, precision="fp64")
+        s.engine()
+        tot = run_steps(s, args.warmup, args.steps)[0]
+        seq_time += tot
+        seq_steps += args.steps
+        del s
+        gc.collect()
+    seq_rate = seq_steps / seq_time
+    rate = inst_steps / wall
+    out = {
+        "metric": "uneven batch: instance-steps/s (mixed, batched) vs plain FP64 one at a time",
+        "value": round(rate, 4), "unit": "instance-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * wall / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32-in-fp64", "data": "synthetic",
+        "config": {"workload": f"{len(specs)} uneven MPET instances (N 8..32, A 1..4), "
+                               f"theta=1, dt=0.0125", "dofs": int(ndofs),
+                   "seed": args.seed, "outer_tol": args.outer_tol,
+                   "inner_rtol": args.inner_rtol},
+        "dof_steps_per_s": round(rate * ndofs / len(specs), 1),
+        "max_inner_iterations_per_step": iters,
+        "setup_s": round(setup, 2), "max_hbm_bytes": int(mem),
+        "fp64_one_at_a_time": {"instance_steps_per_s": round(seq_rate, 4),
+                               "sample": [int(i) for i in idx]},
+        "speedup_vs_fp64_one_at_a_time": round(rate / seq_rate, 3),
+    }
+    print(json.dumps(out))
+
+
+def main_serve(args):
+    """Serving: Poisson arrivals at `--rate` req/s for `--duration` s from the
+    uneven family (1 in 20 at N = 48, A = 4), 10..100 steps each (scaled by
+    `--steps-scale`), HBM budget `--budget` GB; p50/p99 latency and
+    aggregate instance-steps/s."""
+    import torch
+    from waterscapes_amd.serve import MPETServer, Request
+    rng = np.random.default_rng(args.seed)
+    t = 0.0
+    reqs = []
+    k = 0
+    while True:
+        t += rng.exponential(1.0 / args.rate)
+        if t > args.duration:
+            break
+        big = (k % 20) == 19
+        N, p = uneven_instance(rng, big)
+        steps = max(1, int(round(rng.integers(10, 101) * args.steps_scale)))
+        reqs.append(Request(make_solver=lambda N=N, p=p: make_uneven_solver(N, p),
+                            steps=steps, arrival=t, rid=k))
+        k += 1
+    srv = MPETServer(hbm_budget=args.budget * 1e9, precision="mixed", outer_tol=args.outer_tol,
+                     inner_rtol=args.inner_rtol)
+    for r in reqs:
+        srv.submit(r)
+    t0 = time.perf_counter()
+    last = t0
+    while srv.waiting or srv.active:
+        if not srv.tick():
+            time.sleep(1e-3)
+        if time.perf_counter() - last > 10:
+            last = time.perf_counter()
+            print(f"[serve] t={last - t0:.0f}s active {len(srv.active)} waiting "
+                  f"{len(srv.waiting)} finished {len(srv.finished)}", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    lat = srv.latencies()
+    done = [r for r in srv.finished if not r.refused]
+    steps = sum(r.steps for r in done)
+    out = {
+        "metric": "served MPET requests: p50/p99 latency, aggregate instance-steps/s",
+        "value": round(steps / wall, 4), "unit": "instance-steps/s", "n_gpus": 1,
+        "steps": int(steps), "warmup": 0, "ms_per_step": round(1e3 * wall / max(srv.ticks, 1), 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32-in-fp64", "data": "synthetic",
+        "config": {"workload": f"Poisson arrivals {args.rate} req/s for {args.duration}s, "
+                               f"uneven family + 1 in 20 at 48^3 A=4, budget {args.budget} GB",
+                   "requests": len(reqs), "seed": args.seed},
+        "latency_s": {"p50": round(float(np.percentile(lat, 50)), 3),
+                      "p99": round(float(np.percentile(lat, 99)), 3)} if lat.size else None,
+        "refused": sum(r.refused for r in srv.finished),
+        "ticks": srv.ticks, "wall_s": round(wall, 2),
+        "max_hbm_bytes": int(torch.cuda.max_memory_allocated()),
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,12 +338,24 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=8)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-reference", action="store_true")
+    ap.add_argument("--workload", choices=("single", "batch", "serve"), default="single")
+    ap.add_argument("--instances", type=int, default=96)
+    ap.add_argument("--sample", type=int, default=8)
+    ap.add_argument("--seed", type=int, default=11)
+    ap.add_argument("--rate", type=float, default=1.0)
+    ap.add_argument("--duration", type=float, default=20.0)
+    ap.add_argument("--steps-scale", type=float, default=1.0)
+    ap.add_argument("--budget", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: runs on one GPU only (batches would shard by instance across GPUs "
               "with no communication; one instance is not partitioned); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        return main_batch(args)
+    if args.workload == "serve":
+        return main_serve(args)
     import torch
     from waterscapes_amd import _lib
     K, W = args.steps, args.warmup

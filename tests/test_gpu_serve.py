@@ -1,0 +1,108 @@
+"""Serving loop on the GPU: admission by HBM footprint, continuous batching
+(requests joining while others are mid-flight), isolation of each result,
+and refusal of requests larger than the budget.
+
+The clock is virtual (one unit per scheduling tick) so arrivals interleave
+with running requests deterministically."""
+import numpy as np
+import pytest
+
+from conftest import CFG1
+from test_gpu_api import build_problem, field_errors
+from test_gpu_batch import P_A1, P_A3
+
+pytestmark = pytest.mark.gpu
+
+ISOLATION_RTOL = 1e-9     # per-field relative L2, served alone vs in the mix
+
+# (mesh n, params, arrival tick, steps)
+REQS = [(3, CFG1, 0, 3), (2, P_A3, 0, 2), (4, P_A1, 1, 4), (2, CFG1, 2, 2), (3, P_A3, 3, 1)]
+
+
+class VClock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        return self.t
+
+
+def _requests(which):
+    from waterscapes_amd.serve import Request
+    out = []
+    for i in which:
+        n, params, arr, steps = REQS[i]
+        out.append(Request(make_solver=lambda n=n, params=params:
+                           build_problem(n, params, "fp64", False, T=10.0, dt=0.05)[1],
+                           steps=steps, arrival=float(arr), rid=i))
+    return out
+
+
+def _serve(reqs, budget=200e9):
+    from waterscapes_amd.serve import MPETServer
+    clock = VClock()
+    srv = MPETServer(hbm_budget=budget, precision="mixed", outer_tol=1e-11, clock=clock)
+    for r in reqs:
+        srv.submit(r)
+    concurrency = []
+    while srv.waiting or srv.active:
+        concurrency.append(srv.tick())          # instances stepped together
+        clock.t += 1.0
+    return srv, concurrency
+
+
+def test_continuous_batching_and_isolation():
+    srv, conc = _serve(_requests(range(len(REQS))))
+    done = {r.rid: r for r in srv.finished}
+    assert len(done) == len(REQS) and not any(r.refused for r in done.values())
+    # requests arriving later joined while earlier ones were still running
+    assert done[2].admitted > 0 and done[2].admitted < done[0].done
+    assert max(conc) >= 3
+    for r in done.values():
+        assert r.steps_done == r.steps and len(r.niter) == r.steps and min(r.niter) > 0
+    # each result is the same served alone
+    for i in range(len(REQS)):
+        alone, _ = _serve(_requests([i]))
+        a = alone.finished[0]
+        from waterscapes_amd.fem import MixedSpace
+        from waterscapes_amd.mesh import UnitCubeMesh
+        n, params = REQS[i][0], REQS[i][1]
+        space = MixedSpace(UnitCubeMesh(n, n, n), params["J"])
+        errs = field_errors(space, done[i].result, a.result)
+        assert max(errs) <= ISOLATION_RTOL, (i, errs)
+
+
+def test_admission_by_footprint():
+    from waterscapes_amd.serve import footprint_bytes
+    reqs = _requests([0, 2, 3])
+    # budget for one instance at a time: the others wait until memory frees
+    s = reqs[1].make_solver()
+    one = footprint_bytes(s.up_.function_space(), "mixed")
+    del s
+    srv, conc = _serve(reqs, budget=1.05 * one)
+    assert max(conc) == 1
+    assert all(not r.refused and r.steps_done == r.steps for r in srv.finished)
+    assert srv.used == 0
+    # a request larger than the whole budget is refused, the rest still run
+    srv, conc = _serve(_requests([2, 1]), budget=0.5 * one)
+    by = {r.rid: r for r in srv.finished}
+    assert by[2].refused and by[2].result is None
+    assert not by[1].refused and by[1].steps_done == REQS[1][3]
+
+
+def test_footprint_estimate_bounds_allocation():
+    import torch
+    from waterscapes_amd.batch import MPETBatch
+    from waterscapes_amd.serve import footprint_bytes
+    torch.cuda.synchronize()
+    base = torch.cuda.memory_allocated()
+    s = build_problem(4, CFG1, "fp64", False, T=10.0, dt=0.05)[1]
+    b = MPETBatch([s], precision="mixed")
+    b.step()
+    torch.cuda.synchronize()
+    used = torch.cuda.memory_allocated() - base
+    est = footprint_bytes(s.up_.function_space(), "mixed")
+    b.close()
+    # torch's allocations only (native workspaces are extra): the estimate
+    # must cover them and not overshoot grossly
+    assert 0.5 * used <= est <= 6.0 * used, (est, used)

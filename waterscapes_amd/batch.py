@@ -158,6 +158,21 @@ def pack_hierarchy(hiers, scales, skip_A0=False):
     return out, (inv, blk, _i32(off), ioff)
 
 
+def batch_engine(solver, precision="mixed"):
+    """The batch-ready device state of one solver (assembled system, Dirichlet
+    data, RHS kernels, host AMG hierarchies), created once and kept on the
+    solver, so that re-packing a batch never repeats assembly or AMG setup."""
+    eng = getattr(solver, "_engine", None)
+    if eng is not None and getattr(eng, "batched", False) and eng.config.mode == precision:
+        return eng
+    p = solver.params
+    cfg = SolverConfig(mode=precision, p0_weight=p["p0_weight"])
+    eng = StepEngine(solver.problem, solver.up_.function_space(), p["dt"], p["theta"], cfg,
+                     p["device"], batched=True)
+    solver._engine = eng
+    return eng
+
+
 class MPETBatch:
     """Advance many MPETTotalPressureSolver instances together.
 
@@ -181,19 +196,11 @@ class MPETBatch:
         self.K = K = len(self.solvers)
         dev = torch.device(self.solvers[0].params["device"])
         self.device = dev
-        cfg = SolverConfig(mode=precision, rtol=rtol, outer_tol=outer_tol, inner_rtol=inner_rtol,
-                           maxit=maxit, max_outer=max_outer)
         self.engines = []
         for s in self.solvers:
             if str(torch.device(s.params["device"])) != str(dev):
                 raise ValueError("all instances of a batch must be on one device")
-            cfg_k = SolverConfig(mode=precision, rtol=rtol, outer_tol=outer_tol,
-                                 inner_rtol=inner_rtol, maxit=maxit, max_outer=max_outer,
-                                 p0_weight=s.params["p0_weight"])
-            eng = StepEngine(s.problem, s.up_.function_space(), s.params["dt"],
-                             s.params["theta"], cfg_k, s.params["device"], batched=True)
-            s._engine = eng
-            self.engines.append(eng)
+            self.engines.append(batch_engine(s, precision))
         self.keep = []
         self._layout()
         self._pack_operator()

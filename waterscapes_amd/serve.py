@@ -1,0 +1,211 @@
+"""Serving loop: MPET solve requests admitted by HBM footprint and stepped
+by continuous batching on one GPU.
+
+A request is one MPETTotalPressureSolver problem and a number of implicit
+time steps.  The server keeps every admitted instance's device state
+(assembled operator, Dirichlet data, AMG hierarchies, time-stepping state)
+resident until the request completes.  At every scheduling tick it
+
+  1. admits waiting requests in arrival order while their estimated HBM
+     footprint fits the budget (a request larger than the whole budget is
+     refused with MPET_ERR_BUDGET);
+  2. re-packs the batch if the set of instances changed (no assembly or AMG
+     setup is repeated: each instance's device state is reused);
+  3. runs one batched step of every admitted instance (requests that
+     arrived while others were running join mid-flight: their steps overlap
+     on the device with the others');
+  4. retires finished requests (final state copied to the host) and frees
+     their memory.
+
+The reference has no serving layer (it steps one problem per
+MPETTotalPressureSolver, mpet/mpettotalpressuresolver.py:456-502); each
+request's steps are that solver's steps.
+"""
+from __future__ import annotations
+
+import gc
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .batch import MPETBatch, batch_engine
+
+ERR_BUDGET = 6
+
+
+class BudgetError(RuntimeError):
+    """A request whose footprint exceeds the whole HBM budget."""
+    code = ERR_BUDGET
+
+
+@dataclass
+class Request:
+    """make_solver() -> MPETTotalPressureSolver with its initial state set
+    (called at admission, so waiting requests hold no device memory)."""
+    make_solver: object
+    steps: int
+    arrival: float = 0.0
+    rid: object = None
+    footprint: int = 0
+    # filled by the server
+    solver: object = None
+    admitted: float = None
+    done: float = None
+    steps_done: int = 0
+    result: np.ndarray = None
+    niter: list = field(default_factory=list)
+    refused: bool = False
+
+
+def footprint_bytes(space, precision="mixed", robin=False):
+    """Estimated HBM bytes of one resident instance: FP64 primitives, the
+    packed FP64 / FP32 operator copies, the AMG hierarchies with their work
+    vectors, and the step / MINRES vectors.  From the node patterns only (no
+    device allocation)."""
+    dm = space.dofmap
+    nz = {k: int(dm.pattern(k)[1].size) for k in ("22", "21", "12", "11")}
+    A, n, nP2, nP1 = space.A, space.ndofs, space.nP2, space.nP1
+    F = A + 1
+    op_vals = 9 * nz["22"] + 3 * nz["21"] + 3 * nz["12"] + (1 + A) * nz["11"]
+    idx = 4 * (nz["22"] + nz["21"] + nz["12"] + nz["11"] + nP2 + nP1 + 2)
+    prim = 8 * (op_vals + nz["22"] + 3 * nz["12"] + (A * nz["11"] if robin else 0))
+    s = 4 if precision == "mixed" else 8
+    ops = 8 * op_vals + (s * op_vals if precision == "mixed" else 0) + 2 * idx
+    # hierarchy: level 0 (Laplacian values, its pattern is shared) plus the
+    # coarse levels (bounded by ~half of level 0) and 7 work vectors per row
+    amg = s * nz["22"] + (s + 4) * (nz["22"] + F * nz["11"]) // 2 \
+        + 7 * s * (3 * nP2 + F * nP1) * 2
+    vecs = 9 * s * n + 12 * 8 * n
+    return int(prim + ops + amg + vecs)
+
+
+class MPETServer:
+    """Continuous-batching server over one GPU (see module docstring)."""
+
+    def __init__(self, hbm_budget=200e9, precision="mixed", outer_tol=1e-10, inner_rtol=3e-4,
+                 rtol=1e-10, max_instances=512, clock=time.perf_counter):
+        if not torch.cuda.is_available():
+            raise RuntimeError("MPETServer needs a GPU (no CPU fallback)")
+        self.budget = float(hbm_budget)
+        self.precision = precision
+        self.tol = dict(outer_tol=outer_tol, inner_rtol=inner_rtol, rtol=rtol)
+        self.max_instances = max_instances
+        self.clock = clock
+        self.waiting = []          # arrival order
+        self.active = []
+        self.finished = []
+        self.used = 0
+        self.batch = None
+        self._batch_ids = None
+        self.t0 = clock()
+        self.steps_run = 0
+        self.ticks = 0
+
+    # -- requests ---------------------------------------------------------------
+    def submit(self, req: Request):
+        self.waiting.append(req)
+
+    def _admit(self, now):
+        changed = False
+        while self.waiting and len(self.active) < self.max_instances:
+            req = self.waiting[0]
+            if req.arrival > now:
+                break
+            if req.footprint == 0:
+                s = req.make_solver()
+                req.solver = s
+                prob = s.problem
+                robin = any(not getattr(b, "is_zero", lambda: False)() for b in prob.beta)
+                req.footprint = footprint_bytes(s.up_.function_space(), self.precision, robin)
+            if req.footprint > self.budget:
+                self.waiting.pop(0)
+                req.refused = True
+                req.solver = None
+                req.done = now
+                self.finished.append(req)
+                continue
+            if self.used + req.footprint > self.budget:
+                break                      # wait until memory frees up
+            self.waiting.pop(0)
+            if req.solver is None:
+                req.solver = req.make_solver()
+            batch_engine(req.solver, self.precision)     # resident device state
+            req.admitted = now
+            self.used += req.footprint
+            self.active.append(req)
+            changed = True
+        return changed
+
+    def _repack(self):
+        if self.batch is not None:
+            self.batch.close()
+            self.batch = None
+        if self.active:
+            self.batch = MPETBatch([r.solver for r in self.active], precision=self.precision,
+                                   **self.tol)
+        self._batch_ids = [id(r) for r in self.active]
+
+    def _retire(self, now):
+        keep = []
+        for r in self.active:
+            if r.steps_done >= r.steps:
+                r.result = r.solver.up.vector().get_local().copy()
+                r.done = now
+                self.used -= r.footprint
+                r.solver._engine = None
+                r.solver = None
+                self.finished.append(r)
+            else:
+                keep.append(r)
+        changed = len(keep) != len(self.active)
+        self.active = keep
+        if changed:
+            if self.batch is not None:
+                self.batch.close()
+                self.batch = None
+            gc.collect()
+        return changed
+
+    # -- scheduling ---------------------------------------------------------------
+    def tick(self):
+        """Admit, (re)pack, one batched step, retire.  Returns the number of
+        instances stepped."""
+        now = self.clock() - self.t0
+        changed = self._admit(now)
+        if not self.active:
+            return 0
+        if changed or self.batch is None or self._batch_ids != [id(r) for r in self.active]:
+            self._repack()
+        b = self.batch
+        stats = b.step()
+        b.commit()
+        for k, r in enumerate(self.active):
+            r.steps_done += 1
+            r.niter.append(int(stats[k].inner_iterations))
+        self.steps_run += len(self.active)
+        self.ticks += 1
+        torch.cuda.synchronize()
+        self._retire(self.clock() - self.t0)
+        return len(stats)
+
+    def run(self, until_idle=True, idle_sleep=1e-3):
+        """Tick until every submitted request has finished."""
+        while self.waiting or self.active:
+            if not self.tick():
+                if self.waiting and self.waiting[0].arrival > self.clock() - self.t0:
+                    time.sleep(min(idle_sleep, self.waiting[0].arrival -
+                                   (self.clock() - self.t0)))
+                elif self.waiting and not self.active:
+                    # the head request does not fit although nothing runs:
+                    # only possible when it exceeds the budget (refused above)
+                    raise BudgetError("request cannot be admitted")
+        if self.batch is not None:
+            self.batch.close()
+            self.batch = None
+        return self.finished
+
+    def latencies(self):
+        lat = [r.done - r.arrival for r in self.finished if not r.refused]
+        return np.array(lat)

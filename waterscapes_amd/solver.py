@@ -130,6 +130,7 @@ class StepEngine:
     def __init__(self, problem, space, dt, theta, config: SolverConfig, device="cuda",
                  extrapolate=False, batched=False):
         self.problem = problem
+        self.batched = bool(batched)
         self.extrapolate = bool(extrapolate)
         self.prev = None
         self.space = space
