@@ -183,8 +183,9 @@ class MPETBatch:
     and results are read from them as usual.
     """
 
-    def __init__(self, solvers, precision="mixed", rtol=1e-10, outer_tol=1e-10,
-                 inner_rtol=3e-4, maxit=5000, max_outer=30, check_every=10, use_graphs=True):
+    def __init__(self, solvers, precision="mixed", rtol=1e-10, outer_tol=3e-11,
+                 inner_rtol=3e-4, maxit=5000, max_outer=30, check_every=10, use_graphs=True,
+                 extrapolate=None):
         if not torch.cuda.is_available():
             raise RuntimeError("MPETBatch needs a GPU (no CPU fallback)")
         if precision not in ("mixed", "fp64"):
@@ -193,6 +194,9 @@ class MPETBatch:
         if not self.solvers:
             raise ValueError("empty batch")
         self.precision = precision
+        # initial guess 2 u_n - u_{n-1} (the mixed path's outer tolerance
+        # bounds the error independently of the guess)
+        self.extrapolate = (precision == "mixed") if extrapolate is None else bool(extrapolate)
         self.K = K = len(self.solvers)
         dev = torch.device(self.solvers[0].params["device"])
         self.device = dev
@@ -465,6 +469,8 @@ class MPETBatch:
             t_next[k] = e.assemble_rhs(s.up_.data, self.t[k])
             x = s.up.data
             x.copy_(s.up_.data)
+            if self.extrapolate and e.prev is not None:
+                x.add_(s.up_.data).sub_(e.prev)
             if e.bc_dofs.numel():
                 g = torch.as_tensor(e.bc_values(None)).to(self.device)
                 x.index_copy_(0, e.bc_dofs, g)
@@ -484,11 +490,16 @@ class MPETBatch:
     def commit(self, active=None):
         """up_ <- up for the stepped instances (the generator's update after
         its yield, :497-499) and their time Constants to t_{n+1}."""
+        ks = range(self.K) if active is None else np.nonzero(np.asarray(active))[0]
+        if self.extrapolate:
+            for k in ks:
+                e = self.engines[k]
+                if e.prev is None:
+                    e.prev = torch.empty_like(self.solvers[k].up_.data)
+                e.prev.copy_(self.solvers[k].up_.data)
         if active is None:
             self.Xprev_i.copy_(self.X_i)
-            ks = range(self.K)
         else:
-            ks = np.nonzero(np.asarray(active))[0]
             for k in ks:
                 self.solvers[k].up_.data.copy_(self.solvers[k].up.data)
         for k in ks:

@@ -78,7 +78,25 @@ spmm_kernel(int nrows, const int* __restrict__ rp, const int* __restrict__ ci,
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = 0;
   const int kb = rp[r], ke = rp[r + 1];
-  for (int k = kb + lane; k < ke; k += LPR) {
+  // two nonzeros per lane per trip: both index / value / gather loads are in
+  // flight together (same summation order as one at a time)
+  int k = kb + lane;
+  for (; k + LPR < ke; k += 2 * LPR) {
+    const T a0 = val[k], a1 = val[k + LPR];
+    const long long c0 = (long long)NC * ci[k], c1 = (long long)NC * ci[k + LPR];
+    T x0[NC], x1[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      x0[c] = x[c0 + c];
+      x1[c] = x[c1 + c];
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      acc[c] += a0 * x0[c];
+      acc[c] += a1 * x1[c];
+    }
+  }
+  if (k < ke) {
     const T a = val[k];
     const long long col = (long long)NC * ci[k];
 #pragma unroll
@@ -282,7 +300,32 @@ struct Precond {
   long long nu = 0;
   const int* pmap = nullptr;           // explicit pressure order (batches)
   T *pin = nullptr, *pout = nullptr;   // field-major pressure buffers
+  // optional second stream: the pressure V-cycle runs beside the
+  // displacement one (fork / join through the two events; inside a captured
+  // graph these become parallel branches)
+  hipStream_t ps = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
+
+// Create the fork stream and events of P (default priority, no timing).
+template <typename T>
+static int make_fork(Precond<T>& P) {
+  if (hipStreamCreateWithFlags(&P.ps, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&P.ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&P.ev_join, hipEventDisableTiming) != hipSuccess) {
+    mpet_set_error("preconditioner stream / event creation failed");
+    return ERR_HIP;
+  }
+  return OK;
+}
+template <typename T>
+static void free_fork(Precond<T>& P) {
+  if (P.ev_fork) (void)hipEventDestroy(P.ev_fork);
+  if (P.ev_join) (void)hipEventDestroy(P.ev_join);
+  if (P.ps) (void)hipStreamDestroy(P.ps);
+  P.ev_fork = P.ev_join = nullptr;
+  P.ps = nullptr;
+}
 
 template <typename T, int NC, int MODE>
 static void spmm(const LevelDev<T>& L, int which, const T* x, const T* b, T* y, T* d, T cd,
@@ -377,22 +420,32 @@ static void vcycle(const Hier<T>& H, int l, const T* b, T bscale, T* out, const 
 template <typename T>
 static void precond_apply(const Precond<T>& P, const T* r, T* z, const int* skip,
                           hipStream_t s) {
+  hipStream_t sp = s;
+  if (P.ps) {                      // pressure blocks on the second stream
+    (void)hipEventRecord(P.ev_fork, s);
+    (void)hipStreamWaitEvent(P.ps, P.ev_fork, 0);
+    sp = P.ps;
+  }
   vcycle<T, 3>(P.u, 0, r, (T)(1.0 / P.u.scale), z, skip, s);
   long long np = (long long)P.F * P.nP1;
   int eb = (int)((np + NT - 1) / NT);
   if (P.pmap)
-    hipLaunchKernelGGL((p_gather_map_kernel<T>), dim3(eb), dim3(NT), 0, s, np, P.pmap, r + P.nu,
+    hipLaunchKernelGGL((p_gather_map_kernel<T>), dim3(eb), dim3(NT), 0, sp, np, P.pmap, r + P.nu,
                        P.pin, skip);
   else
-    hipLaunchKernelGGL((p_gather_kernel<T>), dim3(eb), dim3(NT), 0, s, P.nP1, P.F, r + P.nu,
+    hipLaunchKernelGGL((p_gather_kernel<T>), dim3(eb), dim3(NT), 0, sp, P.nP1, P.F, r + P.nu,
                        P.pin, skip);
-  vcycle<T, 1>(P.p, 0, P.pin, (T)1, P.pout, skip, s);
+  vcycle<T, 1>(P.p, 0, P.pin, (T)1, P.pout, skip, sp);
   if (P.pmap)
-    hipLaunchKernelGGL((p_scatter_map_kernel<T>), dim3(eb), dim3(NT), 0, s, np, P.pmap, P.pout,
+    hipLaunchKernelGGL((p_scatter_map_kernel<T>), dim3(eb), dim3(NT), 0, sp, np, P.pmap, P.pout,
                        z + P.nu, skip);
   else
-    hipLaunchKernelGGL((p_scatter_kernel<T>), dim3(eb), dim3(NT), 0, s, P.nP1, P.F, P.pout,
+    hipLaunchKernelGGL((p_scatter_kernel<T>), dim3(eb), dim3(NT), 0, sp, P.nP1, P.F, P.pout,
                        z + P.nu, skip);
+  if (P.ps) {
+    (void)hipEventRecord(P.ev_join, P.ps);
+    (void)hipStreamWaitEvent(s, P.ev_join, 0);
+  }
 }
 
 template <typename T>

@@ -22,6 +22,7 @@
 // Instance k in iteration i is skipped when inactive[k][i & 1] is set; the
 // shared preconditioner launches stop once every instance is inactive.
 #include "amg_cycle.h"
+#include "op_rows.h"
 #include <algorithm>
 #include <vector>
 #include <math.h>
@@ -35,6 +36,7 @@ namespace batch {
 constexpr int G = 16;            // lanes per node row of the operator kernels
 constexpr int ROWS = NT / G;     // node rows per operator tile
 constexpr int SEG = 2048;        // DOFs per elementwise segment
+constexpr double INNER_MARGIN = 0.3;   // as in the single-instance solver
 
 struct State {                   // MINRES state of one instance
   double beta1, beta, oldb, dbar, epsln, phibar, cs, sn, phi, gamma, delta, oldeps, alfa;
@@ -87,7 +89,8 @@ template <typename T>
 __global__ void __launch_bounds__(NT)
 bop_u_kernel(BOp<T> op, const Tile* __restrict__ tiles, const T* __restrict__ x,
              const T* __restrict__ b, T* __restrict__ y, K1Args<T> k1) {
-  const Tile tl = tiles[blockIdx.x];
+  const int t = blockIdx.x;
+  const Tile tl = tiles[t];
   T s, c1;
   const bool active = k1_setup(k1, tl.inst, s, c1);
   double acc = 0.0;
@@ -95,20 +98,8 @@ bop_u_kernel(BOp<T> op, const Tile* __restrict__ tiles, const T* __restrict__ x,
   const int a = tl.begin + threadIdx.x / G;
   if (active && a < tl.end) {
     T y0 = 0, y1 = 0, y2 = 0;
-    const long long n22 = op.nnz22, n21 = op.nnz21;
-    for (int k = op.rp22[a] + lane; k < op.rp22[a + 1]; k += G) {
-      const long long cb = 3ll * op.ci22[k];
-      const T x0 = x[cb], x1 = x[cb + 1], x2 = x[cb + 2];
-      y0 += op.Kuu[k] * x0 + op.Kuu[n22 + k] * x1 + op.Kuu[2 * n22 + k] * x2;
-      y1 += op.Kuu[3 * n22 + k] * x0 + op.Kuu[4 * n22 + k] * x1 + op.Kuu[5 * n22 + k] * x2;
-      y2 += op.Kuu[6 * n22 + k] * x0 + op.Kuu[7 * n22 + k] * x1 + op.Kuu[8 * n22 + k] * x2;
-    }
-    for (int k = op.rp21[a] + lane; k < op.rp21[a + 1]; k += G) {
-      const T p0 = x[op.ci21[k]];
-      y0 += op.Dv[k] * p0;
-      y1 += op.Dv[n21 + k] * p0;
-      y2 += op.Dv[2 * n21 + k] * p0;
-    }
+    urow_apply<T, G>(op.rp22, op.ci22, op.Kuu, op.nnz22, op.rp21, op.ci21, op.Dv, op.nnz21, x,
+                     0, 1, a, lane, y0, y1, y2);
     y0 = group_allsum<G>(y0);
     y1 = group_allsum<G>(y1);
     y2 = group_allsum<G>(y2);
@@ -130,7 +121,7 @@ bop_u_kernel(BOp<T> op, const Tile* __restrict__ tiles, const T* __restrict__ x,
   }
   if (k1.st) {
     const double bs = block_sum(acc);
-    if (threadIdx.x == 0) k1.partial[blockIdx.x] = bs;
+    if (threadIdx.x == 0) k1.partial[t] = bs;
   }
 }
 
@@ -140,7 +131,8 @@ template <typename T>
 __global__ void __launch_bounds__(NT)
 bop_p_kernel(BOp<T> op, const Tile* __restrict__ tiles, int part_off, const T* __restrict__ x,
              const T* __restrict__ b, T* __restrict__ y, K1Args<T> k1) {
-  const Tile tl = tiles[blockIdx.x];
+  const int t = blockIdx.x;
+  const Tile tl = tiles[t];
   T s, c1;
   const bool active = k1_setup(k1, tl.inst, s, c1);
   double acc = 0.0;
@@ -149,30 +141,12 @@ bop_p_kernel(BOp<T> op, const Tile* __restrict__ tiles, int part_off, const T* _
   if (active && c < tl.end) {
     const MpetBatchCoef& ic = op.coef[tl.inst];
     const int F = ic.F;
-    T yd = 0;
-    const long long n12 = op.nnz12;
-    for (int k = op.rp12[c] + lane; k < op.rp12[c + 1]; k += G) {
-      const long long cb = 3ll * op.ci12[k];
-      yd += op.Dt[k] * x[cb] + op.Dt[n12 + k] * x[cb + 1] + op.Dt[2 * n12 + k] * x[cb + 2];
-    }
+    T yd = prow_div<T, G>(op.rp12, op.ci12, op.Dt, op.nnz12, x, c, lane);
     T m[MPET_MAXF], sv[MPET_MAXF];
 #pragma unroll
     for (int g = 0; g < MPET_MAXF; ++g) { m[g] = 0; sv[g] = 0; }
-    const T* St = op.St + ic.st_base;
-    const long long kbase = ic.k11_base, n11k = ic.nnz11;
-    for (int k = op.rp11[c] + lane; k < op.rp11[c + 1]; k += G) {
-      const T* xp = x + op.ci11[k];
-      const T mv = op.M1[k];
-      const long long kl = k - kbase;
-#pragma unroll
-      for (int g = 0; g < MPET_MAXF; ++g) {
-        if (g < F) {
-          const T xv = xp[g];
-          m[g] += mv * xv;
-          if (g >= 1) sv[g] += St[(g - 1) * n11k + kl] * xv;
-        }
-      }
-    }
+    prow_mass<T, G, MPET_MAXF>(op.rp11, op.ci11, op.M1, op.St + ic.st_base, ic.nnz11,
+                               ic.k11_base, x, 0, 1, F, c, lane, m, sv);
     yd = group_allsum<G>(yd);
     T mine = 0;
 #pragma unroll
@@ -203,7 +177,7 @@ bop_p_kernel(BOp<T> op, const Tile* __restrict__ tiles, int part_off, const T* _
   }
   if (k1.st) {
     const double bs = block_sum(acc);
-    if (threadIdx.x == 0) k1.partial[part_off + blockIdx.x] = bs;
+    if (threadIdx.x == 0) k1.partial[part_off + t] = bs;
   }
 }
 
@@ -666,6 +640,8 @@ extern "C" int mpet_batch_destroy(void* h) {
   for (auto& g : B->graph)
     if (g) (void)hipGraphExecDestroy(g);
   if (B->graph_stream) (void)hipStreamDestroy(B->graph_stream);
+  free_fork(B->pc64);
+  free_fork(B->pc32);
   for (void* p : B->allocs)
     if (p) (void)hipFree(p);
   for (void* p : B->host_allocs)
@@ -718,6 +694,8 @@ extern "C" int mpet_batch_create(const MpetBatchOpDesc* op64, const MpetBatchOpD
   if (!rc && prec32) rc = build_precond(B->allocs, B->nu, prec32, B->pc32);
   B->has_pc64 = prec64 != nullptr;
   B->has_pc32 = prec32 != nullptr;
+  if (!rc && prec64) rc = make_fork(B->pc64);
+  if (!rc && prec32) rc = make_fork(B->pc32);
   if (!rc && opt->mode == 0) rc = build_minres(B, B->m64);
   if (!rc && opt->mode == 1) rc = build_minres(B, B->m32);
   if (!rc && ((rc = alloc_n(B, &B->r64, (size_t)B->n)) ||
@@ -843,6 +821,24 @@ extern "C" int mpet_batch_solve(void* h, const double* b, double* x, const int32
         // restart without the instances that stopped refining
         if ((rc = set_keep(B, s))) return rc;
         if ((rc = minres_start<float>(B, B->opt.inner_rtol, B->opt.maxit, s))) return rc;
+      }
+      // per-instance inner target: inner_rtol relative, but no deeper than
+      // needed for the outer target (INNER_MARGIN of it)
+      {
+        bool upd = false;
+        for (int k = 0; k < K; ++k) {
+          State& st = B->st_host[k];
+          if (!B->keep_host[k] || st.done) continue;
+          const double tol = fmax(B->opt.inner_rtol * st.beta1,
+                                  INNER_MARGIN * B->opt.outer_tol * B->ref_host[k]);
+          if (tol != st.tol) {
+            st.tol = tol;
+            upd = true;
+          }
+        }
+        if (upd)
+          HIPCHK(hipMemcpyAsync(B->m32.st, B->st_host, sizeof(State) * K,
+                                hipMemcpyHostToDevice, s));
       }
       if ((rc = minres_continue<float>(B, B->opt.maxit, s))) return rc;
       int broke = 0;

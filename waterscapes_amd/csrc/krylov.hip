@@ -27,6 +27,7 @@
 // Dirichlet invariant: Krylov vectors have zero Dirichlet components and
 // operators zero their Dirichlet rows (see spmv.hip).
 #include "amg_cycle.h"
+#include "op_rows.h"
 #include <vector>
 #include <math.h>
 #include <stdio.h>
@@ -90,19 +91,8 @@ k1_op_kernel(MpetOpT<T> op, const T* __restrict__ z, EpiK1<T> epi, const MinresS
       const int a = tile * K1_ROWS + threadIdx.x / K1_G;
       if (a < op.nP2) {
         T y0 = 0, y1 = 0, y2 = 0;
-        for (int k = op.rp22[a] + lane; k < op.rp22[a + 1]; k += K1_G) {
-          const int cb = 3 * op.ci22[k];
-          const T x0 = z[cb], x1 = z[cb + 1], x2 = z[cb + 2];
-          y0 += op.Kuu[k] * x0 + op.Kuu[n22 + k] * x1 + op.Kuu[2 * n22 + k] * x2;
-          y1 += op.Kuu[3 * n22 + k] * x0 + op.Kuu[4 * n22 + k] * x1 + op.Kuu[5 * n22 + k] * x2;
-          y2 += op.Kuu[6 * n22 + k] * x0 + op.Kuu[7 * n22 + k] * x1 + op.Kuu[8 * n22 + k] * x2;
-        }
-        for (int k = op.rp21[a] + lane; k < op.rp21[a + 1]; k += K1_G) {
-          const T p0 = z[nu + (long long)F * op.ci21[k]];
-          y0 += op.Dv[k] * p0;
-          y1 += op.Dv[n21 + k] * p0;
-          y2 += op.Dv[2 * n21 + k] * p0;
-        }
+        urow_apply<T, K1_G>(op.rp22, op.ci22, op.Kuu, n22, op.rp21, op.ci21, op.Dv, n21, z, nu,
+                            F, a, lane, y0, y1, y2);
         y0 = group_allsum<K1_G>(y0);
         y1 = group_allsum<K1_G>(y1);
         y2 = group_allsum<K1_G>(y2);
@@ -114,23 +104,12 @@ k1_op_kernel(MpetOpT<T> op, const T* __restrict__ z, EpiK1<T> epi, const MinresS
     } else {
       const int c = (tile - ntu) * K1_ROWS + threadIdx.x / K1_G;
       if (c < op.nP1) {
-        T yd = 0;
-        for (int k = op.rp12[c] + lane; k < op.rp12[c + 1]; k += K1_G) {
-          const int cb = 3 * op.ci12[k];
-          yd += op.Dt[k] * z[cb] + op.Dt[n12 + k] * z[cb + 1] + op.Dt[2 * n12 + k] * z[cb + 2];
-        }
+        T yd = prow_div<T, K1_G>(op.rp12, op.ci12, op.Dt, n12, z, c, lane);
         T m[F], sv[F];
 #pragma unroll
         for (int g = 0; g < F; ++g) { m[g] = 0; sv[g] = 0; }
-        for (int k = op.rp11[c] + lane; k < op.rp11[c + 1]; k += K1_G) {
-          const T* xp = z + nu + (long long)F * op.ci11[k];
-          const T mv = op.M1[k];
-#pragma unroll
-          for (int g = 0; g < F; ++g) {
-            m[g] += mv * xp[g];
-            if (g >= 1) sv[g] += op.St[(g - 1) * n11 + k] * xp[g];
-          }
-        }
+        prow_mass<T, K1_G, F>(op.rp11, op.ci11, op.M1, op.St, n11, 0, z, nu, F, F, c, lane, m,
+                              sv);
         yd = group_allsum<K1_G>(yd);
         T mine = 0;
 #pragma unroll
@@ -167,8 +146,7 @@ k1_csr_kernel(long long nrows, const long long* __restrict__ rp, const int* __re
   for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     long long r = tile * ROWS + threadIdx.x / LPR;
     if (r < nrows) {
-      T sum = 0;
-      for (long long k = rp[r] + lane; k < rp[r + 1]; k += LPR) sum += val[k] * z[ci[k]];
+      T sum = csr_row<T, LPR>(rp, ci, val, z, r, lane);
       sum = group_allsum<LPR>(sum);
       if (lane == 0) acc += epi(r, sum, s, c1);
     }
@@ -338,6 +316,11 @@ __global__ void cvt_up_kernel(long long n, const float* __restrict__ a, double* 
 // host side
 // ===========================================================================
 using namespace mpet;
+
+// Fraction of the outer target that the last inner solve of a refinement
+// aims for (FP32 rounding makes the FP64 residual differ slightly from the
+// inner recurrence's estimate).
+constexpr double INNER_MARGIN = 0.3;
 
 // --- MINRES workspace in precision T -------------------------------------
 template <typename T>
@@ -622,6 +605,10 @@ extern "C" int mpet_solver_create(const MpetOpDesc* op64, const MpetOpDesc* op32
   if (prec32 && (rc = build_precond(S->allocs, S->nu, prec32, S->pc32))) { mpet_solver_destroy(S); return rc; }
   S->has_pc64 = prec64 != nullptr;
   S->has_pc32 = prec32 != nullptr;
+  if ((prec64 && (rc = make_fork(S->pc64))) || (prec32 && (rc = make_fork(S->pc32)))) {
+    mpet_solver_destroy(S);
+    return rc;
+  }
   const bool need64 = S->opt.mode != 1, need32 = S->opt.mode == 1;
   if (need64 && (rc = build_minres(S, S->m64))) { mpet_solver_destroy(S); return rc; }
   if (need32 && (rc = build_minres(S, S->m32))) { mpet_solver_destroy(S); return rc; }
@@ -657,6 +644,8 @@ extern "C" int mpet_solver_destroy(void* h) {
   if (S->graph32) (void)hipGraphExecDestroy(S->graph32);
   if (S->graph64) (void)hipGraphExecDestroy(S->graph64);
   if (S->graph_stream) (void)hipStreamDestroy(S->graph_stream);
+  free_fork(S->pc64);
+  free_fork(S->pc32);
   for (void* p : S->allocs)
     if (p) (void)hipFree(p);
   for (void* p : S->host_allocs)
@@ -719,6 +708,18 @@ extern "C" int mpet_solver_solve(void* h, const double* b, double* x, MpetSolveS
       if (!(beta == beta)) { rc = ERR_NAN; break; }
       if (beta <= S->opt.outer_tol * bn || beta == 0.0) { conv_outer = true; break; }
       if (outer >= S->opt.max_outer) break;
+      // Inner target: inner_rtol relative, but no deeper than needed to reach
+      // the outer target (with margin): ||r||_{M^-1} <= INNER_MARGIN * outer_tol
+      // * ||b_I||_{M^-1}.  Avoids overshooting the outer tolerance by up to
+      // inner_rtol in the last refinement step.
+      {
+        const double tol = fmax(S->opt.inner_rtol * beta, INNER_MARGIN * S->opt.outer_tol * bn);
+        if (tol != S->m32.st_host->tol) {
+          S->m32.st_host->tol = tol;
+          HIPCHK(hipMemcpyAsync(S->m32.st, S->m32.st_host, sizeof(MinresState),
+                                hipMemcpyHostToDevice, s));
+        }
+      }
       int it = 0, conv = 0;
       rc = minres_continue(S, S->m32, S->pc32, false, S->opt.inner_rtol, S->opt.maxit,
                            S->opt.check_every, S->opt.use_graphs ? &S->graph32 : nullptr, s,

@@ -17,6 +17,7 @@
 // the FP64 outer residual of iterative refinement.
 #include "mpet_common.h"
 #include "mpet_ops.h"
+#include "op_rows.h"
 
 namespace mpet {
 
@@ -29,9 +30,7 @@ csr_spmv_kernel(long long nrows, const long long* __restrict__ rp, const int* __
   long long r = gid / LPR;
   int lane = (int)(gid % LPR);
   if (r >= nrows) return;   // whole groups exit together (LPR divides 256)
-  long long kb = rp[r], ke = rp[r + 1];
-  T s = 0;
-  for (long long k = kb + lane; k < ke; k += LPR) s += val[k] * x[ci[k]];
+  T s = csr_row<T, LPR>(rp, ci, val, x, r, lane);
   s = group_sum<LPR>(s);
   if (lane == 0) {
     T out = b ? b[r] - s : s;
@@ -56,26 +55,14 @@ __global__ void __launch_bounds__(256)
 op_apply_kernel(MpetOpT<T> op, const T* __restrict__ x, const T* __restrict__ b,
                 T* __restrict__ y, int nbu) {
   const long long nu = 3ll * op.nP2;
-  if ((int)blockIdx.x < nbu) {
+  const int bid = blockIdx.x;
+  if (bid < nbu) {
     const int lane = threadIdx.x % GU;
-    const int a = blockIdx.x * (256 / GU) + threadIdx.x / GU;
+    const int a = bid * (256 / GU) + threadIdx.x / GU;
     if (a >= op.nP2) return;
     T y0 = 0, y1 = 0, y2 = 0;
-    const long long n22 = op.nnz22;
-    for (int k = op.rp22[a] + lane; k < op.rp22[a + 1]; k += GU) {
-      const int cb = 3 * op.ci22[k];
-      const T x0 = x[cb], x1 = x[cb + 1], x2 = x[cb + 2];
-      y0 += op.Kuu[k] * x0 + op.Kuu[n22 + k] * x1 + op.Kuu[2 * n22 + k] * x2;
-      y1 += op.Kuu[3 * n22 + k] * x0 + op.Kuu[4 * n22 + k] * x1 + op.Kuu[5 * n22 + k] * x2;
-      y2 += op.Kuu[6 * n22 + k] * x0 + op.Kuu[7 * n22 + k] * x1 + op.Kuu[8 * n22 + k] * x2;
-    }
-    const long long n21 = op.nnz21;
-    for (int k = op.rp21[a] + lane; k < op.rp21[a + 1]; k += GU) {
-      const T p0 = x[nu + (long long)F * op.ci21[k]];
-      y0 += op.Dv[k] * p0;
-      y1 += op.Dv[n21 + k] * p0;
-      y2 += op.Dv[2 * n21 + k] * p0;
-    }
+    urow_apply<T, GU>(op.rp22, op.ci22, op.Kuu, op.nnz22, op.rp21, op.ci21, op.Dv, op.nnz21, x,
+                      nu, F, a, lane, y0, y1, y2);
     y0 = op_allsum<GU>(y0);
     y1 = op_allsum<GU>(y1);
     y2 = op_allsum<GU>(y2);
@@ -88,28 +75,13 @@ op_apply_kernel(MpetOpT<T> op, const T* __restrict__ x, const T* __restrict__ b,
     }
   } else {
     const int lane = threadIdx.x % GP;
-    const int c = (blockIdx.x - nbu) * (256 / GP) + threadIdx.x / GP;
+    const int c = (bid - nbu) * (256 / GP) + threadIdx.x / GP;
     if (c >= op.nP1) return;
-    T yd = 0;
-    const long long n12 = op.nnz12;
-    for (int k = op.rp12[c] + lane; k < op.rp12[c + 1]; k += GP) {
-      const int cb = 3 * op.ci12[k];
-      yd += op.Dt[k] * x[cb] + op.Dt[n12 + k] * x[cb + 1] + op.Dt[2 * n12 + k] * x[cb + 2];
-    }
+    T yd = prow_div<T, GP>(op.rp12, op.ci12, op.Dt, op.nnz12, x, c, lane);
     T m[F], s[F];
 #pragma unroll
     for (int g = 0; g < F; ++g) { m[g] = 0; s[g] = 0; }
-    const long long n11 = op.nnz11;
-    for (int k = op.rp11[c] + lane; k < op.rp11[c + 1]; k += GP) {
-      const T* xp = x + nu + (long long)F * op.ci11[k];
-      const T mv = op.M1[k];
-#pragma unroll
-      for (int g = 0; g < F; ++g) {
-        const T xv = xp[g];
-        m[g] += mv * xv;
-        if (g >= 1) s[g] += op.St[(g - 1) * n11 + k] * xv;
-      }
-    }
+    prow_mass<T, GP, F>(op.rp11, op.ci11, op.M1, op.St, op.nnz11, 0, x, nu, F, F, c, lane, m, s);
     yd = op_allsum<GP>(yd);
     T mine = 0;
 #pragma unroll

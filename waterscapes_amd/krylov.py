@@ -82,7 +82,7 @@ class SolverConfig:
     mode: str = "fp64"
     rtol: float = 1e-10
     maxit: int = 5000
-    outer_tol: float = 1e-10
+    outer_tol: float = 3e-11
     inner_rtol: float = 3e-4
     max_outer: int = 30
     check_every: int = 0              # 0: 1 for fp64, 10 for mixed

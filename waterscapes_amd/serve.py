@@ -77,14 +77,14 @@ def footprint_bytes(space, precision="mixed", robin=False):
     # coarse levels (bounded by ~half of level 0) and 7 work vectors per row
     amg = s * nz["22"] + (s + 4) * (nz["22"] + F * nz["11"]) // 2 \
         + 7 * s * (3 * nP2 + F * nP1) * 2
-    vecs = 9 * s * n + 12 * 8 * n
+    vecs = 9 * s * n + 13 * 8 * n
     return int(prim + ops + amg + vecs)
 
 
 class MPETServer:
     """Continuous-batching server over one GPU (see module docstring)."""
 
-    def __init__(self, hbm_budget=200e9, precision="mixed", outer_tol=1e-10, inner_rtol=3e-4,
+    def __init__(self, hbm_budget=200e9, precision="mixed", outer_tol=3e-11, inner_rtol=3e-4,
                  rtol=1e-10, max_instances=512, clock=time.perf_counter):
         if not torch.cuda.is_available():
             raise RuntimeError("MPETServer needs a GPU (no CPU fallback)")

@@ -33,6 +33,7 @@ from . import forms, refel
 from .device import DeviceSystem
 from .facets import FacetSet, FacetTerm
 from .fem import Constant, Function, MixedSpace
+from .amg import AmgParams
 from .krylov import KrylovSolver, SolverConfig
 from .problem import resolved_params
 
@@ -261,8 +262,10 @@ class MPETTotalPressureSolver(object):
     "mixed" = FP32 MINRES/AMG in FP64 refinement, or "fp64-pc32" = FP64
     MINRES with the FP32 preconditioner), "krylov_rtol", "outer_tol",
     "inner_rtol", "maxit", "p0_weight" ("robust" / "reference"),
-    "extrapolate" (initial guess 2 u_n - u_{n-1} instead of up_; fewer
-    iterations at the same tolerance but a larger error), "device"."""
+    "extrapolate" (initial guess 2 u_n - u_{n-1} instead of up_; None =
+    on for "mixed", whose outer tolerance bounds the error independently of
+    the guess, off otherwise), "u_smoother_degree" / "p_smoother_degree"
+    (Chebyshev degrees of the two AMG hierarchies), "device"."""
 
     def __init__(self, problem, params=None):
         self.problem = problem
@@ -295,11 +298,13 @@ class MPETTotalPressureSolver(object):
         params.add("testing", False)
         params.add("precision", "fp64")
         params.add("krylov_rtol", 1e-10)
-        params.add("outer_tol", 1e-10)
+        params.add("outer_tol", 3e-11)
         params.add("inner_rtol", 3e-4)
         params.add("maxit", 5000)
         params.add("p0_weight", "robust")
-        params.add("extrapolate", False)
+        params.add("extrapolate", None)
+        params.add("u_smoother_degree", 2)
+        params.add("p_smoother_degree", 2)
         params.add("device", "cuda")
         return params
 
@@ -336,13 +341,16 @@ class MPETTotalPressureSolver(object):
                                 p0_weight=p["p0_weight"])
         return SolverConfig(mode=p["precision"], rtol=float(p["krylov_rtol"]),
                             outer_tol=float(p["outer_tol"]), inner_rtol=float(p["inner_rtol"]),
-                            maxit=int(p["maxit"]), p0_weight=p["p0_weight"])
+                            maxit=int(p["maxit"]), p0_weight=p["p0_weight"],
+                            amg=AmgParams(smoother_degree=int(p["u_smoother_degree"])),
+                            pamg=AmgParams(smoother_degree=int(p["p_smoother_degree"])))
 
     def engine(self, direct=None):
         direct = self.params["direct_solver"] if direct is None else direct
         if self._engine is None:
             cfg = self._config(direct)
-            ex = bool(self.params["extrapolate"])
+            ex = self.params["extrapolate"]
+            ex = (cfg.mode == "mixed") if ex is None else bool(ex)
             self._engine = StepEngine(self.problem, self.up_.function_space(),
                                       self.params["dt"], self.params["theta"], cfg,
                                       self.params["device"], extrapolate=ex)
