@@ -486,9 +486,15 @@ def main():
                                                        _lib.ptr(v), _lib.ptr(x64), None, None,
                                                        _lib.ptr(y64), lpr, sp))
             out["fp64"]["csr_spmv"] = {"ms": round(ms64, 4), "GB_s": round(csr_bytes(fsys) / (ms64 * 1e-3) / 1e9, 1)}
-    out["roofline"] = {"bound": "hbm", "kernel": "op_apply_kernel<float> (structured A, FP32)",
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "op_apply_48cube_A4_traffic.json")
+    if args.n == 48 and args.A == 4 and os.path.exists(tfile):
+        # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter passes
+        # (calibrated on a copy kernel of known size), recorded there
+        traffic = int(json.load(open(tfile))["traffic_bytes_per_launch"])
+    out["roofline"] = {"bound": "hbm", "kernel": "op_apply_kernel<float,5,16,16> (structured A, FP32)",
                        "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-                       "frac": round(achieved / 8000.0, 4), "traffic": None,
+                       "frac": round(achieved / 8000.0, 4), "traffic": traffic,
                        "bytes_per_launch": int(abytes), "ms_per_launch": round(ms, 4)}
     if not args.no_cpu_baseline:
         cb = cpu_baseline(args.cpu_n, args.A, steps=20)
