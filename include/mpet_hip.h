@@ -112,6 +112,12 @@ typedef struct {
   const int32_t *R_rp, *R_ci;    /* restriction nc x n (= P^T) */
   const void* R_v;
   int32_t lpr, lprP, lprR;       /* lanes per row of the three SpMMs */
+  /* Optional compact copy of the level operator for FP32 hierarchies (NULL:
+   * none): values as IEEE half of A/A_vscale, columns as signed 16-bit
+   * offsets from the row index (same row pointers A_rp). */
+  const void* A_v16;
+  const int16_t* A_dci;
+  double A_vscale;
 } MpetAmgLevelDesc;
 
 #define MPET_MAXLEV 16

@@ -7,6 +7,7 @@
 // preconditioner of the MINRES solve (mpet/mpettotalpressuresolver.py:448).
 #pragma once
 #include "mpet_ops.h"
+#include <hip/hip_fp16.h>
 #include <vector>
 #include <stdio.h>
 
@@ -62,12 +63,16 @@ static int grid_for(long long items, int per_block, int cap) {
 // MODE 0: y = A x        MODE 1: y = b - A x       MODE 2: y += A x
 // MODE 3: Chebyshev step d = cd d + cr Dinv (b - A x); y = x + d
 // ---------------------------------------------------------------------------
-template <typename T, int NC, int LPR, int MODE>
+template <typename T, int NC, int LPR, int MODE, bool COMPACT = false>
 __global__ void __launch_bounds__(NT)
 spmm_kernel(int nrows, const int* __restrict__ rp, const int* __restrict__ ci,
             const T* __restrict__ val, const T* __restrict__ x, const T* __restrict__ b,
             T* __restrict__ y, const T* __restrict__ dinv, T* __restrict__ d, T cd, T cr,
-            const uint8_t* __restrict__ mask, const int* __restrict__ skip) {
+            const uint8_t* __restrict__ mask, const int* __restrict__ skip,
+            const __half* __restrict__ val16 = nullptr, const int16_t* __restrict__ dci = nullptr,
+            T vscale = (T)1) {
+  // COMPACT: values stored as half precision times 1/vscale, columns as
+  // signed 16-bit offsets from the row (half the bytes per nonzero)
   static_assert(NC <= LPR, "one lane per component in the epilogue");
   if (skip && *skip) return;
   long long gid = (long long)blockIdx.x * NT + threadIdx.x;
@@ -78,12 +83,20 @@ spmm_kernel(int nrows, const int* __restrict__ rp, const int* __restrict__ ci,
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = 0;
   const int kb = rp[r], ke = rp[r + 1];
+  auto value = [&](int k) -> T {
+    if constexpr (COMPACT) return (T)__half2float(val16[k]);
+    else return val[k];
+  };
+  auto column = [&](int k) -> long long {
+    if constexpr (COMPACT) return (long long)NC * (r + (int)dci[k]);
+    else return (long long)NC * ci[k];
+  };
   // two nonzeros per lane per trip: both index / value / gather loads are in
   // flight together (same summation order as one at a time)
   int k = kb + lane;
   for (; k + LPR < ke; k += 2 * LPR) {
-    const T a0 = val[k], a1 = val[k + LPR];
-    const long long c0 = (long long)NC * ci[k], c1 = (long long)NC * ci[k + LPR];
+    const T a0 = value(k), a1 = value(k + LPR);
+    const long long c0 = column(k), c1 = column(k + LPR);
     T x0[NC], x1[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
@@ -97,10 +110,14 @@ spmm_kernel(int nrows, const int* __restrict__ rp, const int* __restrict__ ci,
     }
   }
   if (k < ke) {
-    const T a = val[k];
-    const long long col = (long long)NC * ci[k];
+    const T a = value(k);
+    const long long col = column(k);
 #pragma unroll
     for (int c = 0; c < NC; ++c) acc[c] += a * x[col + c];
+  }
+  if constexpr (COMPACT) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] *= vscale;
   }
   T mine = 0;
 #pragma unroll
@@ -275,6 +292,11 @@ struct LevelDev {
   const int *Prp = nullptr, *Pci = nullptr, *Rrp = nullptr, *Rci = nullptr;
   const T *Pv = nullptr, *Rv = nullptr;
   const uint8_t* mask = nullptr;
+  // optional compact copy of the level operator (half values times 1/vscale,
+  // 16-bit column offsets from the row), used by the level SpMMs when set
+  const __half* v16 = nullptr;
+  const int16_t* dci = nullptr;
+  double vscale = 1.0;
   double cd[MAXDEG], cr[MAXDEG];
   int lpr = 8, lprP = 4, lprR = 8;
   T *B = nullptr, *X = nullptr, *R = nullptr, *E = nullptr, *S1 = nullptr, *S2 = nullptr,
@@ -341,6 +363,25 @@ static void spmm(const LevelDev<T>& L, int which, const T* x, const T* b, T* y, 
   long long threads = (long long)nrows * lpr;
   int blocks = (int)((threads + NT - 1) / NT);
   if (blocks == 0) return;
+  if (which == 0 && L.v16) {
+    if constexpr (sizeof(T) == 4) {
+#define SPMM_CASE(LP)                                                                          \
+  case LP:                                                                                     \
+    hipLaunchKernelGGL((spmm_kernel<T, NC, (LP < NC ? 4 : LP), MODE, true>), dim3(blocks),     \
+                       dim3(NT), 0, s, nrows, rp, ci, v, x, b, y, L.dinv, d, cd, cr, mask, skip, \
+                       L.v16, L.dci, (T)L.vscale);                                             \
+    break;
+      switch (lpr) {
+        SPMM_CASE(4) SPMM_CASE(8) SPMM_CASE(16) SPMM_CASE(32) SPMM_CASE(64)
+        default: hipLaunchKernelGGL((spmm_kernel<T, NC, (NC > 4 ? 8 : 4), MODE, true>),
+                                    dim3((unsigned)(((long long)nrows * (NC > 4 ? 8 : 4) + NT - 1) / NT)),
+                                    dim3(NT), 0, s, nrows, rp, ci, v, x, b, y, L.dinv, d, cd, cr,
+                                    mask, skip, L.v16, L.dci, (T)L.vscale);
+      }
+#undef SPMM_CASE
+      return;
+    }
+  }
 #define SPMM_CASE(LP)                                                                          \
   case LP:                                                                                     \
     hipLaunchKernelGGL((spmm_kernel<T, NC, (LP < NC ? 4 : LP), MODE>), dim3(blocks), dim3(NT), \
@@ -480,6 +521,9 @@ static int build_hier(std::vector<void*>& allocs, const MpetHierDesc* d, int NC,
     L.Prp = ld.P_rp; L.Pci = ld.P_ci; L.Pv = (const T*)ld.P_v;
     L.Rrp = ld.R_rp; L.Rci = ld.R_ci; L.Rv = (const T*)ld.R_v;
     L.mask = l == 0 ? d->mask0 : nullptr;
+    L.v16 = (const __half*)ld.A_v16;
+    L.dci = ld.A_dci;
+    L.vscale = ld.A_vscale != 0.0 ? ld.A_vscale : 1.0;
     L.lpr = ld.lpr ? ld.lpr : 8;
     L.lprP = ld.lprP ? ld.lprP : 4;
     L.lprR = ld.lprR ? ld.lprR : 8;

@@ -27,7 +27,8 @@ class AmgLevelDesc(C.Structure):
                 ("A_v", C.c_void_p), ("dinv", C.c_void_p), ("rho", C.c_double),
                 ("P_rp", C.c_void_p), ("P_ci", C.c_void_p), ("P_v", C.c_void_p),
                 ("R_rp", C.c_void_p), ("R_ci", C.c_void_p), ("R_v", C.c_void_p),
-                ("lpr", C.c_int32), ("lprP", C.c_int32), ("lprR", C.c_int32)]
+                ("lpr", C.c_int32), ("lprP", C.c_int32), ("lprR", C.c_int32),
+                ("A_v16", C.c_void_p), ("A_dci", C.c_void_p), ("A_vscale", C.c_double)]
 
 
 class HierDesc(C.Structure):
@@ -96,6 +97,12 @@ class SolverConfig:
     # a factor ~mu and costs ~4x MINRES iterations at E = 1500; "robust"
     # uses 1/(2 mu) + 1/lambda, "reference" the bare mass.
     p0_weight: str = "robust"
+    # FP32 hierarchies: keep the finest displacement-level operator as half
+    # precision values (scaled into half's range) with 16-bit column offsets,
+    # half the bytes of its smoothing / residual SpMMs; the V-cycle stays a
+    # fixed symmetric operator, so MINRES is unaffected beyond the slightly
+    # perturbed preconditioner
+    compact_level0: bool = True
 
 
 def _csr_i32(rp):
@@ -117,6 +124,25 @@ def _pick_lpr(avg, nc=1, rows=None):
         while rows * lanes < (1 << 20) and lanes < 64 and lanes < avg:
             lanes *= 2
     return max(lanes, 4 if nc == 3 else 1)
+
+
+def compact_level(pattern_host, values):
+    """Half-precision copy of a level operator on a host CSR pattern
+    (rp, ci) with device FP64 values: values * 2^e rounded to half (the
+    largest magnitude mapped near 2^10), columns as int16 offsets from the
+    row.  None if some offset does not fit in 16 bits."""
+    rp, ci = pattern_host
+    rows = np.repeat(np.arange(rp.size - 1, dtype=np.int64), np.diff(rp))
+    off = ci.astype(np.int64) - rows
+    if off.size == 0 or np.abs(off).max() > 32767:
+        return None
+    vmax = float(values.abs().max().item())
+    if not vmax > 0:
+        return None
+    e = 10 - int(np.ceil(np.log2(vmax)))
+    v16 = (values * (2.0 ** e)).to(torch.float16)
+    dci = torch.as_tensor(off.astype(np.int16)).to(values.device)
+    return {"A_v16": v16, "A_dci": dci, "vscale": 2.0 ** (-e), "exp": e}
 
 
 class KrylovSolver:
@@ -236,6 +262,11 @@ class KrylovSolver:
         lap = s.prim_t["Lap"] if dtype == 0 else s.prim_t["Lap"].to(torch.float32)
         u = self._upload_levels(self.levels, self.cinv, dtype,
                                 level0=(s.pat_dev["22"][0], s.pat_dev["22"][1], lap))
+        if dtype == 1 and self.cfg.compact_level0:
+            c = compact_level(s.pat_host["22"], s.prim_t["Lap"])
+            if c is not None:
+                u[0][0].update(c)
+                self.compact = c
         p = self._upload_levels(self.plevels, self.pcinv, dtype)
         self.level_dev[dtype] = (u, p)
         return u, p
@@ -277,6 +308,8 @@ class KrylovSolver:
             L.dinv = ptr(ld["dinv"])
             L.rho = lev["rho"]
             L.lpr = _pick_lpr(ld["A_ci"].numel() / max(lev["n"], 1), nc, lev["n"])
+            if "A_v16" in ld:
+                L.A_v16, L.A_dci, L.A_vscale = ptr(ld["A_v16"]), ptr(ld["A_dci"]), ld["vscale"]
             if lev["nc"]:
                 L.P_rp, L.P_ci, L.P_v = ptr(ld["P_rp"]), ptr(ld["P_ci"]), ptr(ld["P_v"])
                 L.R_rp, L.R_ci, L.R_v = ptr(ld["R_rp"]), ptr(ld["R_ci"]), ptr(ld["R_v"])
