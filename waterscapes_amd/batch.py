@@ -32,7 +32,7 @@ import torch
 from . import _lib
 from ._lib import check, ptr, MAXF
 from .krylov import (MAXLEV, HierDesc, PrecDesc, SolveStats, SolverConfig, SolverOptions,
-                     _pick_lpr)
+                     _pick_lpr, compact_level)
 from .solver import StepEngine
 
 
@@ -353,6 +353,9 @@ class MPETBatch:
             dinv = torch.as_tensor(lev["dinv"]).to(dev, tdt)
             keep += [rp, ci, v, dinv]
             L.A_rp, L.A_ci, L.A_v, L.dinv = ptr(rp), ptr(ci), ptr(v), ptr(dinv)
+            if "A_v16" in lev:
+                keep += [lev["A_v16"], lev["A_dci"]]
+                L.A_v16, L.A_dci, L.A_vscale = ptr(lev["A_v16"]), ptr(lev["A_dci"]), lev["vscale"]
             L.rho = lev["rho"]
             L.lpr = _pick_lpr(ci.numel() / max(lev["n"], 1), nc, lev["n"])
             if lev["nc"]:
@@ -397,6 +400,17 @@ class MPETBatch:
                                                   for k, s in enumerate(sys_)])).to(torch.int32),
                   lap0)
         self.mask0 = torch.cat([k.mask0 for k in kry]).contiguous()
+        if self.precision == "mixed" and kry[0].cfg.compact_level0:
+            # half-precision finest level, as for one instance
+            rp_h = np.concatenate([s.pat_host["22"][0][:-1].astype(np.int64)
+                                   + int(self.nnz_off["22"][k]) for k, s in enumerate(sys_)]
+                                  + [self.nnz_off["22"][-1:]])
+            ci_h = np.concatenate([s.pat_host["22"][1].astype(np.int64) + int(self.p2_off[k])
+                                   for k, s in enumerate(sys_)])
+            c = compact_level((rp_h, ci_h),
+                              torch.cat([s.prim_t["Lap"] * s.mu for s in sys_]))
+            if c is not None:
+                ulev[0].update(c)
         # pressure blocks: field-major block-diagonal hierarchies, scale 1
         plev, pcoarse = pack_hierarchy([(k.plevels, k.pcinv) for k in kry], [1.0] * self.K)
         pmap = np.concatenate([self.pd_off[k] + (np.arange(s.F)[None, :] * s.nP1
