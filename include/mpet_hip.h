@@ -118,6 +118,9 @@ typedef struct {
   const void* A_v16;
   const int16_t* A_dci;
   double A_vscale;
+  /* Batches (NULL otherwise): instance of each row of this level, so that
+   * the V-cycle skips the rows of instances whose solve has stopped. */
+  const int32_t* grp;
 } MpetAmgLevelDesc;
 
 #define MPET_MAXLEV 16

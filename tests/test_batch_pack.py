@@ -49,6 +49,31 @@ def test_packed_vcycle_equals_blockwise():
         assert err < 1e-12, (k, err)
         r0 += n
     assert r0 == A0.shape[0]
+    # grp: the instance of every row of every level, and no nonzero of A, P
+    # or R couples two instances (what lets the device V-cycle skip the rows
+    # of instances that stopped)
+    for l, lev in enumerate(levels):
+        rows = [lv[0][min(l, len(lv[0]) - 1)]["n"] for lv in hiers]
+        assert np.array_equal(lev["grp"], np.repeat(np.arange(3), rows)), l
+        g = lev["grp"]
+        A = sp.csr_matrix((lev["A_v"], lev["A_ci"], lev["A_rp"]), shape=(lev["n"],) * 2).tocoo()
+        assert np.array_equal(g[A.row], g[A.col]), l
+        if lev["nc"]:
+            gc = levels[l + 1]["grp"]
+            P = sp.csr_matrix((lev["P_v"], lev["P_ci"], lev["P_rp"]),
+                              shape=(lev["n"], lev["nc"])).tocoo()
+            assert np.array_equal(g[P.row], gc[P.col]), l
+            R = sp.csr_matrix((lev["R_v"], lev["R_ci"], lev["R_rp"]),
+                              shape=(lev["nc"], lev["n"])).tocoo()
+            assert np.array_equal(gc[R.row], g[R.col]), l
+    # so changing one instance's right-hand side (here to huge values) leaves
+    # the others' results bit-identical
+    bn = b.copy()
+    sel = levels[0]["grp"] == 1
+    bn[sel] = 1e30 * rng.standard_normal((int(sel.sum()), 1))
+    zn = _ref_vcycle(levels, Cinv, A0, None, prm, bn)
+    other = levels[0]["grp"] != 1
+    assert np.array_equal(zn[other], z[other])
 
 
 def test_csr_blockdiag_matches_scipy():

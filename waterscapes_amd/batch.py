@@ -128,7 +128,8 @@ def pack_hierarchy(hiers, scales, skip_A0=False):
     Each level's diagonal inverse is also divided by its spectral-radius
     estimate so that one Chebyshev interval (rho = 1) fits every instance.
     skip_A0: leave out the level-0 matrix (the caller supplies it on the
-    device).  Returns (levels, coarse) with coarse = (inv, blk, off, ioff)."""
+    device).  Each level also carries grp, the instance of each of its rows.
+    Returns (levels, coarse) with coarse = (inv, blk, off, ioff)."""
     depth = max(len(h[0]) for h in hiers)
     padded = [pad_levels(lv, ci, depth) for lv, ci in hiers]
     s = np.asarray(scales, dtype=np.float64)
@@ -138,7 +139,8 @@ def pack_hierarchy(hiers, scales, skip_A0=False):
         rows = _prefix([lv["n"] for lv in lev_k])
         d = dict(n=int(rows[-1]), rho=1.0, nc=0,
                  dinv=np.concatenate([lv["dinv"] / (s[k] * lv["rho"])
-                                      for k, lv in enumerate(lev_k)]))
+                                      for k, lv in enumerate(lev_k)]),
+                 grp=np.repeat(np.arange(len(lev_k), dtype=np.int32), np.diff(rows)))
         if not (l == 0 and skip_A0):
             d["A_rp"], d["A_ci"], d["A_v"], _, _ = csr_blockdiag(
                 [(lv["A_rp"], lv["A_ci"], lv["A_v"] * s[k], lv["n"], lv["n"])
@@ -351,8 +353,10 @@ class MPETBatch:
                 ci = torch.as_tensor(lev["A_ci"]).to(dev)
                 v = torch.as_tensor(lev["A_v"]).to(dev, tdt)
             dinv = torch.as_tensor(lev["dinv"]).to(dev, tdt)
-            keep += [rp, ci, v, dinv]
+            grp = torch.as_tensor(_i32(lev["grp"])).to(dev)
+            keep += [rp, ci, v, dinv, grp]
             L.A_rp, L.A_ci, L.A_v, L.dinv = ptr(rp), ptr(ci), ptr(v), ptr(dinv)
+            L.grp = ptr(grp)
             if "A_v16" in lev:
                 keep += [lev["A_v16"], lev["A_dci"]]
                 L.A_v16, L.A_dci, L.A_vscale = ptr(lev["A_v16"]), ptr(lev["A_dci"]), lev["vscale"]

@@ -18,6 +18,16 @@ constexpr int MAX_PARTIALS = 8192; // per-block partial sums of one reduction
 constexpr int MAXLEV = 16;
 constexpr int MAXDEG = 16;
 
+// Work skipped by the level kernels: everything when *all is set; the rows
+// of instance k when inst[k] is set (grp: row -> instance of the level).
+struct Skip {
+  const int* all = nullptr;
+  const int* inst = nullptr;
+};
+__device__ __forceinline__ bool row_skipped(const Skip& sk, const int* __restrict__ grp, int r) {
+  return sk.inst && grp && sk.inst[grp[r]];
+}
+
 // ---------------------------------------------------------------------------
 // reductions
 // ---------------------------------------------------------------------------
@@ -68,17 +78,17 @@ __global__ void __launch_bounds__(NT)
 spmm_kernel(int nrows, const int* __restrict__ rp, const int* __restrict__ ci,
             const T* __restrict__ val, const T* __restrict__ x, const T* __restrict__ b,
             T* __restrict__ y, const T* __restrict__ dinv, T* __restrict__ d, T cd, T cr,
-            const uint8_t* __restrict__ mask, const int* __restrict__ skip,
+            const uint8_t* __restrict__ mask, Skip sk, const int* __restrict__ grp,
             const __half* __restrict__ val16 = nullptr, const int16_t* __restrict__ dci = nullptr,
             T vscale = (T)1) {
   // COMPACT: values stored as half precision times 1/vscale, columns as
   // signed 16-bit offsets from the row (half the bytes per nonzero)
   static_assert(NC <= LPR, "one lane per component in the epilogue");
-  if (skip && *skip) return;
+  if (sk.all && *sk.all) return;
   long long gid = (long long)blockIdx.x * NT + threadIdx.x;
   int r = (int)(gid / LPR);
   int lane = (int)(gid % LPR);
-  if (r >= nrows) return;
+  if (r >= nrows || row_skipped(sk, grp, r)) return;   // whole row groups leave
   T acc[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = 0;
@@ -91,29 +101,44 @@ spmm_kernel(int nrows, const int* __restrict__ rp, const int* __restrict__ ci,
     if constexpr (COMPACT) return (long long)NC * (r + (int)dci[k]);
     else return (long long)NC * ci[k];
   };
-  // two nonzeros per lane per trip: both index / value / gather loads are in
-  // flight together (same summation order as one at a time)
-  int k = kb + lane;
-  for (; k + LPR < ke; k += 2 * LPR) {
-    const T a0 = value(k), a1 = value(k + LPR);
-    const long long c0 = column(k), c1 = column(k + LPR);
-    T x0[NC], x1[NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      x0[c] = x[c0 + c];
-      x1[c] = x[c1 + c];
-    }
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      acc[c] += a0 * x0[c];
-      acc[c] += a1 * x1[c];
-    }
+  // the epilogue operands are loaded first (by every lane, at clamped
+  // indices, so that no branch splits the loads) and their latency overlaps
+  // the row loop
+  const long long o = (long long)NC * r + (lane < NC ? lane : NC - 1);
+  T pb = 0, px = 0, pd = 0, pdinv = 0;
+  if constexpr (MODE == 1 || MODE == 3) pb = b[o];
+  if constexpr (MODE == 2) px = y[o];
+  if constexpr (MODE == 3) {
+    px = x[o];
+    pd = d[o];
+    pdinv = dinv[r];
   }
-  if (k < ke) {
-    const T a = value(k);
-    const long long col = column(k);
+  const int mrow = mask ? (int)mask[r] : 0;
+  // four nonzeros per lane per trip: every index, value and gather load of
+  // a trip is issued before the first use (a missing nonzero reloads the
+  // row's last one and contributes 0).  A lane's nonzeros are summed in the
+  // order k, k + LPR, k + 2 LPR, ... (the same order for any unrolling).
+  constexpr int U = 4;
+  for (int k = kb + lane; k < ke; k += U * LPR) {
+    T a[U];
+    long long cc[U];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) acc[c] += a * x[col + c];
+    for (int j = 0; j < U; ++j) {
+      const int kk = k + j * LPR;
+      const int kc = kk < ke ? kk : ke - 1;
+      const T v = value(kc);
+      cc[j] = column(kc);
+      a[j] = kk < ke ? v : (T)0;
+    }
+    T xv[U][NC];
+#pragma unroll
+    for (int j = 0; j < U; ++j)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) xv[j][c] = x[cc[j] + c];
+#pragma unroll
+    for (int j = 0; j < U; ++j)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[c] += a[j] * xv[j][c];
   }
   if constexpr (COMPACT) {
 #pragma unroll
@@ -126,18 +151,17 @@ spmm_kernel(int nrows, const int* __restrict__ rp, const int* __restrict__ ci,
     if (lane == c) mine = v;
   }
   if (lane >= NC) return;
-  const long long o = (long long)NC * r + lane;
-  const bool m = mask && mask[r];
+  const bool m = mrow != 0;
   if (MODE == 0) {
     y[o] = m ? (T)0 : mine;
   } else if (MODE == 1) {
-    y[o] = m ? (T)0 : b[o] - mine;
+    y[o] = m ? (T)0 : pb - mine;
   } else if (MODE == 2) {
-    if (!m) y[o] += mine;
+    if (!m) y[o] = px + mine;
   } else {
-    T dd = m ? (T)0 : cd * d[o] + cr * dinv[r] * (b[o] - mine);
+    T dd = m ? (T)0 : cd * pd + cr * pdinv * (pb - mine);
     d[o] = dd;
-    y[o] = m ? (T)0 : x[o] + dd;
+    y[o] = m ? (T)0 : px + dd;
   }
 }
 
@@ -147,11 +171,12 @@ template <typename T, int NC>
 __global__ void __launch_bounds__(NT)
 cheb0_kernel(int nrows, const T* __restrict__ b, T scale, const T* __restrict__ dinv, T cr,
              T* __restrict__ d, T* __restrict__ y, T* __restrict__ bcopy,
-             const uint8_t* __restrict__ mask, const int* __restrict__ skip) {
-  if (skip && *skip) return;
+             const uint8_t* __restrict__ mask, Skip sk, const int* __restrict__ grp) {
+  if (sk.all && *sk.all) return;
   long long i = (long long)blockIdx.x * NT + threadIdx.x;
   if (i >= (long long)NC * nrows) return;
   int r = (int)(i / NC);
+  if (row_skipped(sk, grp, r)) return;
   T bv = scale * b[i];
   if (bcopy) bcopy[i] = bv;
   T v = (mask && mask[r]) ? (T)0 : cr * dinv[r] * bv;
@@ -212,11 +237,12 @@ template <typename T, int NC>
 __global__ void __launch_bounds__(NT)
 coarse_blockdiag_kernel(int n, const int* __restrict__ blk, const int* __restrict__ off,
                         const int64_t* __restrict__ ioff, const T* __restrict__ inv,
-                        const T* __restrict__ b, T* __restrict__ x, const int* __restrict__ skip) {
-  if (skip && *skip) return;
+                        const T* __restrict__ b, T* __restrict__ x, Skip sk,
+                        const int* __restrict__ grp) {
+  if (sk.all && *sk.all) return;
   const int r = blockIdx.x * (NT / 64) + threadIdx.x / 64;
   const int lane = threadIdx.x % 64;
-  if (r >= n) return;
+  if (r >= n || row_skipped(sk, grp, r)) return;
   const int bk = blk[r];
   const int o = off[bk], sz = off[bk + 1] - o;
   const T* row = inv + ioff[bk] + (long long)(r - o) * sz;
@@ -292,6 +318,8 @@ struct LevelDev {
   const int *Prp = nullptr, *Pci = nullptr, *Rrp = nullptr, *Rci = nullptr;
   const T *Pv = nullptr, *Rv = nullptr;
   const uint8_t* mask = nullptr;
+  // row -> instance of this level and of the next coarser one (batches)
+  const int *grp = nullptr, *grpc = nullptr;
   // optional compact copy of the level operator (half values times 1/vscale,
   // 16-bit column offsets from the row), used by the level SpMMs when set
   const __half* v16 = nullptr;
@@ -351,7 +379,7 @@ static void free_fork(Precond<T>& P) {
 
 template <typename T, int NC, int MODE>
 static void spmm(const LevelDev<T>& L, int which, const T* x, const T* b, T* y, T* d, T cd,
-                 T cr, const int* skip, hipStream_t s) {
+                 T cr, Skip sk, hipStream_t s) {
   // which: 0 level operator, 1 prolongator (rows n), 2 restriction (rows nc)
   int nrows = which == 2 ? L.nc : L.n;
   const int* rp = which == 0 ? L.rp : (which == 1 ? L.Prp : L.Rrp);
@@ -360,6 +388,7 @@ static void spmm(const LevelDev<T>& L, int which, const T* x, const T* b, T* y, 
   int lpr = which == 0 ? L.lpr : (which == 1 ? L.lprP : L.lprR);
   if (lpr < NC) lpr = NC == 3 ? 4 : NC;
   const uint8_t* mask = which == 2 ? nullptr : L.mask;
+  const int* grp = which == 2 ? L.grpc : L.grp;
   long long threads = (long long)nrows * lpr;
   int blocks = (int)((threads + NT - 1) / NT);
   if (blocks == 0) return;
@@ -368,7 +397,7 @@ static void spmm(const LevelDev<T>& L, int which, const T* x, const T* b, T* y, 
 #define SPMM_CASE(LP)                                                                          \
   case LP:                                                                                     \
     hipLaunchKernelGGL((spmm_kernel<T, NC, (LP < NC ? 4 : LP), MODE, true>), dim3(blocks),     \
-                       dim3(NT), 0, s, nrows, rp, ci, v, x, b, y, L.dinv, d, cd, cr, mask, skip, \
+                       dim3(NT), 0, s, nrows, rp, ci, v, x, b, y, L.dinv, d, cd, cr, mask, sk, grp, \
                        L.v16, L.dci, (T)L.vscale);                                             \
     break;
       switch (lpr) {
@@ -376,7 +405,7 @@ static void spmm(const LevelDev<T>& L, int which, const T* x, const T* b, T* y, 
         default: hipLaunchKernelGGL((spmm_kernel<T, NC, (NC > 4 ? 8 : 4), MODE, true>),
                                     dim3((unsigned)(((long long)nrows * (NC > 4 ? 8 : 4) + NT - 1) / NT)),
                                     dim3(NT), 0, s, nrows, rp, ci, v, x, b, y, L.dinv, d, cd, cr,
-                                    mask, skip, L.v16, L.dci, (T)L.vscale);
+                                    mask, sk, grp, L.v16, L.dci, (T)L.vscale);
       }
 #undef SPMM_CASE
       return;
@@ -385,7 +414,7 @@ static void spmm(const LevelDev<T>& L, int which, const T* x, const T* b, T* y, 
 #define SPMM_CASE(LP)                                                                          \
   case LP:                                                                                     \
     hipLaunchKernelGGL((spmm_kernel<T, NC, (LP < NC ? 4 : LP), MODE>), dim3(blocks), dim3(NT), \
-                       0, s, nrows, rp, ci, v, x, b, y, L.dinv, d, cd, cr, mask, skip);       \
+                       0, s, nrows, rp, ci, v, x, b, y, L.dinv, d, cd, cr, mask, sk, grp);    \
     break;
   switch (lpr) {
     SPMM_CASE(1) SPMM_CASE(2) SPMM_CASE(4) SPMM_CASE(8) SPMM_CASE(16) SPMM_CASE(32)
@@ -399,7 +428,7 @@ static void spmm(const LevelDev<T>& L, int which, const T* x, const T* b, T* y, 
 // bscale and copied to bcopy when given) or from x0; result in out.
 template <typename T, int NC>
 static void smooth(const Hier<T>& H, int l, const T* b, T bscale, T* bcopy, const T* x0,
-                   T* out, const int* skip, hipStream_t s) {
+                   T* out, Skip sk, hipStream_t s) {
   const LevelDev<T>& L = H.lev[l];
   const int k = H.deg;
   const T* cur = x0;
@@ -411,7 +440,7 @@ static void smooth(const Hier<T>& H, int l, const T* b, T bscale, T* bcopy, cons
     T* dst = (k == 1) ? out : bufs[flip];
     int blocks = (int)(((long long)NC * L.n + NT - 1) / NT);
     hipLaunchKernelGGL((cheb0_kernel<T, NC>), dim3(blocks), dim3(NT), 0, s, L.n, b, bscale,
-                       L.dinv, (T)L.cr[0], L.D, dst, bcopy, L.mask, skip);
+                       L.dinv, (T)L.cr[0], L.D, dst, bcopy, L.mask, sk, L.grp);
     if (bcopy) bb = bcopy;
     cur = dst;
     i = 1;
@@ -420,14 +449,14 @@ static void smooth(const Hier<T>& H, int l, const T* b, T bscale, T* bcopy, cons
   for (; i < k; ++i) {
     T* dst = (i == k - 1) ? out : bufs[flip];
     if (dst == cur) dst = bufs[flip ^ 1];
-    spmm<T, NC, 3>(L, 0, cur, bb, dst, L.D, (T)L.cd[i], (T)L.cr[i], skip, s);
+    spmm<T, NC, 3>(L, 0, cur, bb, dst, L.D, (T)L.cd[i], (T)L.cr[i], sk, s);
     cur = dst;
     flip ^= 1;
   }
 }
 
 template <typename T, int NC>
-static void vcycle(const Hier<T>& H, int l, const T* b, T bscale, T* out, const int* skip,
+static void vcycle(const Hier<T>& H, int l, const T* b, T bscale, T* out, Skip sk,
                    hipStream_t s) {
   const LevelDev<T>& L = H.lev[l];
   if (l == H.nlev - 1) {
@@ -435,39 +464,39 @@ static void vcycle(const Hier<T>& H, int l, const T* b, T bscale, T* out, const 
     if (bscale != (T)1) {   // single-level hierarchy: scale into L.B first
       int blocks = (int)(((long long)NC * L.n + NT - 1) / NT);
       hipLaunchKernelGGL((cheb0_kernel<T, NC>), dim3(blocks), dim3(NT), 0, s, L.n, b, bscale,
-                         L.dinv, (T)0, L.D, L.X, L.B, L.mask, skip);
+                         L.dinv, (T)0, L.D, L.X, L.B, L.mask, sk, L.grp);
       bb = L.B;
     }
     int blocks = (L.n + NT / 64 - 1) / (NT / 64);
     if (H.cblk)
       hipLaunchKernelGGL((coarse_blockdiag_kernel<T, NC>), dim3(blocks), dim3(NT), 0, s, L.n,
-                         H.cblk, H.coff, H.cioff, H.cinv, bb, out, skip);
+                         H.cblk, H.coff, H.cioff, H.cinv, bb, out, sk, L.grp);
     else
       hipLaunchKernelGGL((coarse_kernel<T, NC>), dim3(blocks), dim3(NT), 0, s, L.n, H.cinv, bb,
-                         out, skip);
+                         out, sk.all);
     return;
   }
   T* bcopy = (bscale != (T)1) ? L.B : nullptr;
-  smooth<T, NC>(H, l, b, bscale, bcopy, nullptr, L.X, skip, s);
+  smooth<T, NC>(H, l, b, bscale, bcopy, nullptr, L.X, sk, s);
   const T* bb = bcopy ? bcopy : b;
-  spmm<T, NC, 1>(L, 0, L.X, bb, L.R, nullptr, 0, 0, skip, s);
+  spmm<T, NC, 1>(L, 0, L.X, bb, L.R, nullptr, 0, 0, sk, s);
   const LevelDev<T>& C = H.lev[l + 1];
-  spmm<T, NC, 0>(L, 2, L.R, nullptr, C.B, nullptr, 0, 0, skip, s);
-  vcycle<T, NC>(H, l + 1, C.B, (T)1, C.E, skip, s);
-  spmm<T, NC, 2>(L, 1, C.E, nullptr, L.X, nullptr, 0, 0, skip, s);
-  smooth<T, NC>(H, l, bb, (T)1, nullptr, L.X, out, skip, s);
+  spmm<T, NC, 0>(L, 2, L.R, nullptr, C.B, nullptr, 0, 0, sk, s);
+  vcycle<T, NC>(H, l + 1, C.B, (T)1, C.E, sk, s);
+  spmm<T, NC, 2>(L, 1, C.E, nullptr, L.X, nullptr, 0, 0, sk, s);
+  smooth<T, NC>(H, l, bb, (T)1, nullptr, L.X, out, sk, s);
 }
 
 template <typename T>
-static void precond_apply(const Precond<T>& P, const T* r, T* z, const int* skip,
-                          hipStream_t s) {
+static void precond_apply(const Precond<T>& P, const T* r, T* z, Skip sk, hipStream_t s) {
+  const int* skip = sk.all;
   hipStream_t sp = s;
   if (P.ps) {                      // pressure blocks on the second stream
     (void)hipEventRecord(P.ev_fork, s);
     (void)hipStreamWaitEvent(P.ps, P.ev_fork, 0);
     sp = P.ps;
   }
-  vcycle<T, 3>(P.u, 0, r, (T)(1.0 / P.u.scale), z, skip, s);
+  vcycle<T, 3>(P.u, 0, r, (T)(1.0 / P.u.scale), z, sk, s);
   long long np = (long long)P.F * P.nP1;
   int eb = (int)((np + NT - 1) / NT);
   if (P.pmap)
@@ -476,7 +505,7 @@ static void precond_apply(const Precond<T>& P, const T* r, T* z, const int* skip
   else
     hipLaunchKernelGGL((p_gather_kernel<T>), dim3(eb), dim3(NT), 0, sp, P.nP1, P.F, r + P.nu,
                        P.pin, skip);
-  vcycle<T, 1>(P.p, 0, P.pin, (T)1, P.pout, skip, sp);
+  vcycle<T, 1>(P.p, 0, P.pin, (T)1, P.pout, sk, sp);
   if (P.pmap)
     hipLaunchKernelGGL((p_scatter_map_kernel<T>), dim3(eb), dim3(NT), 0, sp, np, P.pmap, P.pout,
                        z + P.nu, skip);
@@ -487,6 +516,13 @@ static void precond_apply(const Precond<T>& P, const T* r, T* z, const int* skip
     (void)hipEventRecord(P.ev_join, P.ps);
     (void)hipStreamWaitEvent(s, P.ev_join, 0);
   }
+}
+template <typename T>
+static void precond_apply(const Precond<T>& P, const T* r, T* z, const int* skip,
+                          hipStream_t s) {
+  Skip sk;
+  sk.all = skip;
+  precond_apply(P, r, z, sk, s);
 }
 
 template <typename T>
@@ -521,6 +557,7 @@ static int build_hier(std::vector<void*>& allocs, const MpetHierDesc* d, int NC,
     L.Prp = ld.P_rp; L.Pci = ld.P_ci; L.Pv = (const T*)ld.P_v;
     L.Rrp = ld.R_rp; L.Rci = ld.R_ci; L.Rv = (const T*)ld.R_v;
     L.mask = l == 0 ? d->mask0 : nullptr;
+    L.grp = ld.grp;
     L.v16 = (const __half*)ld.A_v16;
     L.dci = ld.A_dci;
     L.vscale = ld.A_vscale != 0.0 ? ld.A_vscale : 1.0;
@@ -537,6 +574,7 @@ static int build_hier(std::vector<void*>& allocs, const MpetHierDesc* d, int NC,
         (rc = alloc_dev(allocs, &L.D, m)))
       return rc;
   }
+  for (int l = 0; l + 1 < H.nlev; ++l) H.lev[l].grpc = H.lev[l + 1].grp;
   H.cinv = (const T*)d->coarse_inv;
   H.cblk = d->coarse_blk;
   H.coff = d->coarse_off;

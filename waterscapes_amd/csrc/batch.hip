@@ -19,8 +19,10 @@
 // MINRES keeps one state per instance.  Row tiles and vector segments never
 // straddle instances, so every partial sum belongs to one instance and one
 // block per instance finalises its scalars in a fixed order (deterministic).
-// Instance k in iteration i is skipped when inactive[k][i & 1] is set; the
-// shared preconditioner launches stop once every instance is inactive.
+// Instance k in iteration i is skipped when inactive[k][i & 1] is set: its
+// operator rows, vector segments and preconditioner rows (through the
+// per-level row -> instance maps) are left alone, and the preconditioner
+// launches stop once every instance is inactive.
 #include "amg_cycle.h"
 #include "op_rows.h"
 #include <algorithm>
@@ -262,14 +264,18 @@ bk4f_kernel(const double* __restrict__ partial, const int* __restrict__ r0,
   }
 }
 
-// all[p] = 1 when every instance is inactive for parity p.
+// all[p] = 1 when every instance is inactive for parity p; iflag[p][k] =
+// instance k inactive for parity p (read by the preconditioner kernels).
 __global__ void __launch_bounds__(NT)
-ball_kernel(const State* st, int ninst, int p, int* all) {
+ball_kernel(const State* st, int ninst, int p, int* all, int* iflag) {
   __shared__ int any;
   if (threadIdx.x == 0) any = 0;
   __syncthreads();
-  for (int k = threadIdx.x; k < ninst; k += NT)
-    if (!st[k].inactive[p]) any = 1;
+  for (int k = threadIdx.x; k < ninst; k += NT) {
+    const int ia = st[k].inactive[p];
+    iflag[(long long)p * ninst + k] = ia;
+    if (!ia) any = 1;
+  }
   __syncthreads();
   if (threadIdx.x == 0) all[p] = any ? 0 : 1;
 }
@@ -367,6 +373,7 @@ struct BMinres {
   T *z = nullptr, *y = nullptr, *v = nullptr, *x = nullptr, *b = nullptr;
   State* st = nullptr;
   int* all = nullptr;               // all[2]
+  int* iflag = nullptr;             // iflag[2][ninst]
 };
 
 struct Batch {
@@ -452,12 +459,16 @@ void minres_iter(Batch* B, int i, hipStream_t s) {
                      M.st, cur);
   hipLaunchKernelGGL((bk2_kernel<T>), dim3(B->nseg), dim3(NT), 0, s, B->segs, M.y, Rb, Ra, M.st,
                      cur);
-  precond_apply(P, Ra, M.z, &M.all[cur], s);
+  Skip sk;
+  sk.all = &M.all[cur];
+  sk.inst = M.iflag + (long long)cur * B->ninst;
+  precond_apply(P, Ra, M.z, sk, s);
   hipLaunchKernelGGL((bdot_kernel<T>), dim3(B->nseg), dim3(NT), 0, s, B->segs, Ra, M.z,
                      (const State*)M.st, cur, B->partial);
   hipLaunchKernelGGL(bk4f_kernel, dim3(B->ninst), dim3(NT), 0, s, B->partial, B->sr0, B->sr1,
                      M.st, cur, next);
-  hipLaunchKernelGGL(ball_kernel, dim3(1), dim3(NT), 0, s, M.st, B->ninst, next, M.all);
+  hipLaunchKernelGGL(ball_kernel, dim3(1), dim3(NT), 0, s, M.st, B->ninst, next, M.all,
+                     M.iflag);
   hipLaunchKernelGGL((bk5_kernel<T>), dim3(B->nseg), dim3(NT), 0, s, B->segs, M.v, Wa, Wb, M.x,
                      M.st, cur);
 }
@@ -475,8 +486,8 @@ int minres_start(Batch* B, double rtol, int maxit, hipStream_t s) {
                      (const State*)nullptr, 0, B->partial);
   hipLaunchKernelGGL(binit_state_kernel, dim3(B->ninst), dim3(NT), 0, s, B->partial, B->sr0,
                      B->sr1, M.st, rtol, B->ref, maxit, B->keep);
-  hipLaunchKernelGGL(ball_kernel, dim3(1), dim3(NT), 0, s, M.st, B->ninst, 0, M.all);
-  hipLaunchKernelGGL(ball_kernel, dim3(1), dim3(NT), 0, s, M.st, B->ninst, 1, M.all);
+  hipLaunchKernelGGL(ball_kernel, dim3(1), dim3(NT), 0, s, M.st, B->ninst, 0, M.all, M.iflag);
+  hipLaunchKernelGGL(ball_kernel, dim3(1), dim3(NT), 0, s, M.st, B->ninst, 1, M.all, M.iflag);
   HIPCHK(hipMemcpyAsync(B->st_host, M.st, sizeof(State) * B->ninst, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   return OK;
@@ -532,7 +543,8 @@ int build_minres(Batch* B, BMinres<T>& M) {
       (rc = alloc_n(B, &M.W[0], n)) || (rc = alloc_n(B, &M.W[1], n)) ||
       (rc = alloc_n(B, &M.z, n)) || (rc = alloc_n(B, &M.y, n)) || (rc = alloc_n(B, &M.v, n)) ||
       (rc = alloc_n(B, &M.x, n)) || (rc = alloc_n(B, &M.b, n)) ||
-      (rc = alloc_n(B, &M.st, (size_t)B->ninst)) || (rc = alloc_n(B, &M.all, 2)))
+      (rc = alloc_n(B, &M.st, (size_t)B->ninst)) || (rc = alloc_n(B, &M.all, 2)) ||
+      (rc = alloc_n(B, &M.iflag, 2 * (size_t)B->ninst)))
     return rc;
   return OK;
 }

@@ -28,7 +28,8 @@ class AmgLevelDesc(C.Structure):
                 ("P_rp", C.c_void_p), ("P_ci", C.c_void_p), ("P_v", C.c_void_p),
                 ("R_rp", C.c_void_p), ("R_ci", C.c_void_p), ("R_v", C.c_void_p),
                 ("lpr", C.c_int32), ("lprP", C.c_int32), ("lprR", C.c_int32),
-                ("A_v16", C.c_void_p), ("A_dci", C.c_void_p), ("A_vscale", C.c_double)]
+                ("A_v16", C.c_void_p), ("A_dci", C.c_void_p), ("A_vscale", C.c_double),
+                ("grp", C.c_void_p)]
 
 
 class HierDesc(C.Structure):
