@@ -57,7 +57,9 @@ typedef struct {
 } MpetMeshDesc;
 
 /* Node-pair sparsity patterns (CSR, sorted columns): rows/cols are P2 or P1
- * nodes: 22 = P2xP2, 21 = P2xP1, 12 = P1xP2, 11 = P1xP1. */
+ * nodes: 22 = P2xP2, 21 = P2xP1, 12 = P1xP2, 11 = P1xP1.  Operator entry
+ * points reject a pattern with no nonzeros (MPET_ERR_INVALID); rows may be
+ * empty. */
 typedef struct {
   const int32_t *rp22, *ci22;
   const int32_t *rp21, *ci21;

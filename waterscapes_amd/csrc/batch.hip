@@ -143,12 +143,12 @@ bop_p_kernel(BOp<T> op, const Tile* __restrict__ tiles, int part_off, const T* _
   if (active && c < tl.end) {
     const MpetBatchCoef& ic = op.coef[tl.inst];
     const int F = ic.F;
-    T yd = prow_div<T, G>(op.rp12, op.ci12, op.Dt, op.nnz12, x, c, lane);
-    T m[MPET_MAXF], sv[MPET_MAXF];
+    T yd = 0, m[MPET_MAXF], sv[MPET_MAXF];
 #pragma unroll
     for (int g = 0; g < MPET_MAXF; ++g) { m[g] = 0; sv[g] = 0; }
-    prow_mass<T, G, MPET_MAXF>(op.rp11, op.ci11, op.M1, op.St + ic.st_base, ic.nnz11,
-                               ic.k11_base, x, 0, 1, F, c, lane, m, sv);
+    prow_apply<T, G, MPET_MAXF>(op.rp12, op.ci12, op.Dt, op.nnz12, op.rp11, op.ci11, op.M1,
+                                op.St + ic.st_base, ic.nnz11, ic.k11_base, x, 0, 1, F, c, lane,
+                                yd, m, sv);
     yd = group_allsum<G>(yd);
     T mine = 0;
 #pragma unroll
@@ -667,7 +667,8 @@ extern "C" int mpet_batch_create(const MpetBatchOpDesc* op64, const MpetBatchOpD
                                  const MpetPrecDesc* prec32, const MpetSolverOptions* opt,
                                  void** handle) {
   if (!op64 || !lay || !opt || !handle || lay->ninst < 1 || op64->ninst != lay->ninst ||
-      !lay->p2_off || !lay->p1_off || !lay->pd_off || !op64->dmask) {
+      !lay->p2_off || !lay->p1_off || !lay->pd_off || !op64->dmask || op64->nnz22 < 1 ||
+      op64->nnz21 < 1 || op64->nnz12 < 1 || op64->nnz11 < 1) {
     mpet_set_error("mpet_batch_create: invalid argument");
     return ERR_INVALID;
   }

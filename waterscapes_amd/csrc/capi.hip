@@ -31,7 +31,8 @@ template MpetOpT<float> mpet_make_op<float>(const MpetOpDesc*);
 
 extern "C" int mpet_op_apply(const MpetOpDesc* d, const void* x, const void* b, void* y,
                              hipStream_t stream) {
-  if (!d || !x || !y || d->A < 1 || d->A > MPET_MAXF - 1) {
+  if (!d || !x || !y || d->A < 1 || d->A > MPET_MAXF - 1 || d->pat.nnz22 < 1 ||
+      d->pat.nnz21 < 1 || d->pat.nnz12 < 1 || d->pat.nnz11 < 1) {
     mpet_set_error("mpet_op_apply: invalid argument");
     return ERR_INVALID;
   }

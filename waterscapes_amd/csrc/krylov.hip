@@ -104,12 +104,11 @@ k1_op_kernel(MpetOpT<T> op, const T* __restrict__ z, EpiK1<T> epi, const MinresS
     } else {
       const int c = (tile - ntu) * K1_ROWS + threadIdx.x / K1_G;
       if (c < op.nP1) {
-        T yd = prow_div<T, K1_G>(op.rp12, op.ci12, op.Dt, n12, z, c, lane);
-        T m[F], sv[F];
+        T yd = 0, m[F], sv[F];
 #pragma unroll
         for (int g = 0; g < F; ++g) { m[g] = 0; sv[g] = 0; }
-        prow_mass<T, K1_G, F>(op.rp11, op.ci11, op.M1, op.St, n11, 0, z, nu, F, F, c, lane, m,
-                              sv);
+        prow_apply<T, K1_G, F>(op.rp12, op.ci12, op.Dt, n12, op.rp11, op.ci11, op.M1, op.St, n11,
+                               0, z, nu, F, F, c, lane, yd, m, sv);
         yd = group_allsum<K1_G>(yd);
         T mine = 0;
 #pragma unroll
@@ -566,7 +565,8 @@ extern "C" int mpet_solver_create(const MpetOpDesc* op64, const MpetOpDesc* op32
                                   const MpetPrecDesc* prec32, const MpetSolverOptions* opt,
                                   void** handle) {
   if (!op64 || !opt || !handle || (!prec64 && !prec32) || op64->A < 1 ||
-      op64->A > MPET_MAXF - 1) {
+      op64->A > MPET_MAXF - 1 || op64->pat.nnz22 < 1 || op64->pat.nnz21 < 1 ||
+      op64->pat.nnz12 < 1 || op64->pat.nnz11 < 1) {
     mpet_set_error("mpet_solver_create: invalid argument");
     return ERR_INVALID;
   }

@@ -77,11 +77,11 @@ op_apply_kernel(MpetOpT<T> op, const T* __restrict__ x, const T* __restrict__ b,
     const int lane = threadIdx.x % GP;
     const int c = (bid - nbu) * (256 / GP) + threadIdx.x / GP;
     if (c >= op.nP1) return;
-    T yd = prow_div<T, GP>(op.rp12, op.ci12, op.Dt, op.nnz12, x, c, lane);
-    T m[F], s[F];
+    T yd = 0, m[F], s[F];
 #pragma unroll
     for (int g = 0; g < F; ++g) { m[g] = 0; s[g] = 0; }
-    prow_mass<T, GP, F>(op.rp11, op.ci11, op.M1, op.St, op.nnz11, 0, x, nu, F, F, c, lane, m, s);
+    prow_apply<T, GP, F>(op.rp12, op.ci12, op.Dt, op.nnz12, op.rp11, op.ci11, op.M1, op.St,
+                         op.nnz11, 0, x, nu, F, F, c, lane, yd, m, s);
     yd = op_allsum<GP>(yd);
     T mine = 0;
 #pragma unroll
