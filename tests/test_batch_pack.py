@@ -4,6 +4,7 @@ instance's block like that instance's own V-cycle on (s_k A_k): padding
 levels (identity prolongators ending in the instance's own coarsest solve)
 and the rho / scale folding into the diagonal may change rounding only."""
 import numpy as np
+import pytest
 import scipy.sparse as sp
 
 from test_gpu_solver import _ref_vcycle
@@ -19,11 +20,12 @@ def _laplacian(n, shift=0.0):
     return L
 
 
-def test_packed_vcycle_equals_blockwise():
+@pytest.mark.parametrize("deg,deg0", [(2, 0), (3, 1)])
+def test_packed_vcycle_equals_blockwise(deg, deg0):
     from waterscapes_amd import amg
     from waterscapes_amd.amg import AmgParams
     from waterscapes_amd.batch import pack_hierarchy
-    prm = AmgParams(max_coarse=40)
+    prm = AmgParams(max_coarse=40, smoother_degree=deg, level0_degree=deg0)
     mats = [_laplacian(4, 0.5), _laplacian(9), _laplacian(6, 0.1)]
     hiers = [amg.setup_host(A.indptr, A.indices, A.data, prm) for A in mats]
     depths = [len(h[0]) for h in hiers]

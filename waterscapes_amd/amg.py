@@ -27,6 +27,7 @@ class AmgParams:
     power_iters: int = 30
     omega_scale: float = 4.0 / 3.0
     smoother_degree: int = 2          # Chebyshev degree (pre and post)
+    level0_degree: int = 0            # degree on the finest level (0: smoother_degree)
     eig_ratio: float = 8.0            # Chebyshev interval [rho/ratio, 1.1 rho]
     eig_safety: float = 1.1
 

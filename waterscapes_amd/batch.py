@@ -384,6 +384,7 @@ class MPETBatch:
         h.coarse_inv = ptr(cinv)
         h.coarse_blk, h.coarse_off, h.coarse_ioff = ptr(cblk), ptr(coff), ptr(cioff)
         h.degree = params.smoother_degree
+        h.degree0 = params.level0_degree
         h.eig_ratio = params.eig_ratio
         h.eig_safety = params.eig_safety
         h.scale = 1.0

@@ -114,11 +114,11 @@ spmm_kernel(int nrows, const int* __restrict__ rp, const int* __restrict__ ci,
     pdinv = dinv[r];
   }
   const int mrow = mask ? (int)mask[r] : 0;
-  // four nonzeros per lane per trip: every index, value and gather load of
-  // a trip is issued before the first use (a missing nonzero reloads the
+  // U nonzeros per lane per trip: every index, value and gather load of a
+  // trip is issued before the first use (a missing nonzero reloads the
   // row's last one and contributes 0).  A lane's nonzeros are summed in the
   // order k, k + LPR, k + 2 LPR, ... (the same order for any unrolling).
-  constexpr int U = 4;
+  constexpr int U = (NC == 3 && LPR <= 4) ? 8 : 4;
   for (int k = kb + lane; k < ke; k += U * LPR) {
     T a[U];
     long long cc[U];
@@ -325,6 +325,7 @@ struct LevelDev {
   const __half* v16 = nullptr;
   const int16_t* dci = nullptr;
   double vscale = 1.0;
+  int deg = 2;                     // Chebyshev degree of this level's smoother
   double cd[MAXDEG], cr[MAXDEG];
   int lpr = 8, lprP = 4, lprR = 8;
   T *B = nullptr, *X = nullptr, *R = nullptr, *E = nullptr, *S1 = nullptr, *S2 = nullptr,
@@ -430,7 +431,7 @@ template <typename T, int NC>
 static void smooth(const Hier<T>& H, int l, const T* b, T bscale, T* bcopy, const T* x0,
                    T* out, Skip sk, hipStream_t s) {
   const LevelDev<T>& L = H.lev[l];
-  const int k = H.deg;
+  const int k = L.deg;
   const T* cur = x0;
   int i = 0;
   T* bufs[2] = {L.S1, L.S2};
@@ -564,8 +565,13 @@ static int build_hier(std::vector<void*>& allocs, const MpetHierDesc* d, int NC,
     L.lpr = ld.lpr ? ld.lpr : 8;
     L.lprP = ld.lprP ? ld.lprP : 4;
     L.lprR = ld.lprR ? ld.lprR : 8;
+    L.deg = (l == 0 && d->degree0 > 0) ? d->degree0 : H.deg;
+    if (L.deg > MAXDEG) {
+      mpet_set_error("invalid hierarchy description");
+      return ERR_INVALID;
+    }
     double lmax = ld.rho * d->eig_safety;
-    chebyshev(lmax / d->eig_ratio, lmax, H.deg, L.cd, L.cr);
+    chebyshev(lmax / d->eig_ratio, lmax, L.deg, L.cd, L.cr);
     size_t m = (size_t)NC * L.n;
     int rc;
     if ((rc = alloc_dev(allocs, &L.B, m)) || (rc = alloc_dev(allocs, &L.X, m)) ||

@@ -36,7 +36,7 @@ class HierDesc(C.Structure):
     _fields_ = [("nlev", C.c_int32), ("lev", AmgLevelDesc * MAXLEV), ("coarse_inv", C.c_void_p),
                 ("degree", C.c_int32), ("eig_ratio", C.c_double), ("eig_safety", C.c_double),
                 ("scale", C.c_double), ("mask0", C.c_void_p), ("coarse_blk", C.c_void_p),
-                ("coarse_off", C.c_void_p), ("coarse_ioff", C.c_void_p)]
+                ("coarse_off", C.c_void_p), ("coarse_ioff", C.c_void_p), ("degree0", C.c_int32)]
 
 
 class PrecDesc(C.Structure):
@@ -90,7 +90,9 @@ class SolverConfig:
     check_every: int = 0              # 0: 1 for fp64, 10 for mixed
     use_graphs: bool = None           # default: mixed only
     use_csr: bool = None              # default: fp64 only
-    amg: AmgParams = field(default_factory=AmgParams)
+    # displacement hierarchy: one Chebyshev step on the P2 level (as many
+    # MINRES iterations as two at 48^3, measured), three on the coarser ones
+    amg: AmgParams = field(default_factory=lambda: AmgParams(smoother_degree=3, level0_degree=1))
     pamg: AmgParams = field(default_factory=AmgParams)
     # weight of the total-pressure mass in the preconditioner.  The reference
     # form prec uses the bare mass p0 w0 dx (mpettotalpressuresolver.py:282),
@@ -319,6 +321,7 @@ class KrylovSolver:
                 L.lprR = _pick_lpr(nnzp / max(lev["nc"], 1), nc, lev["nc"])
         h.coarse_inv = ptr(cinv)
         h.degree = params.smoother_degree
+        h.degree0 = params.level0_degree
         h.eig_ratio = params.eig_ratio
         h.eig_safety = params.eig_safety
         h.scale = scale

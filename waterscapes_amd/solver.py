@@ -265,7 +265,9 @@ class MPETTotalPressureSolver(object):
     "extrapolate" (initial guess 2 u_n - u_{n-1} instead of up_; None =
     on for "mixed", whose outer tolerance bounds the error independently of
     the guess, off otherwise), "u_smoother_degree" / "p_smoother_degree"
-    (Chebyshev degrees of the two AMG hierarchies), "device"."""
+    (Chebyshev degrees of the two AMG hierarchies), "u_level0_degree" /
+    "p_level0_degree" (their degrees on the finest level, 0: the same),
+    "device"."""
 
     def __init__(self, problem, params=None):
         self.problem = problem
@@ -303,8 +305,10 @@ class MPETTotalPressureSolver(object):
         params.add("maxit", 5000)
         params.add("p0_weight", "robust")
         params.add("extrapolate", None)
-        params.add("u_smoother_degree", 2)
+        params.add("u_smoother_degree", 3)
         params.add("p_smoother_degree", 2)
+        params.add("u_level0_degree", 1)
+        params.add("p_level0_degree", 0)
         params.add("device", "cuda")
         return params
 
@@ -342,8 +346,10 @@ class MPETTotalPressureSolver(object):
         return SolverConfig(mode=p["precision"], rtol=float(p["krylov_rtol"]),
                             outer_tol=float(p["outer_tol"]), inner_rtol=float(p["inner_rtol"]),
                             maxit=int(p["maxit"]), p0_weight=p["p0_weight"],
-                            amg=AmgParams(smoother_degree=int(p["u_smoother_degree"])),
-                            pamg=AmgParams(smoother_degree=int(p["p_smoother_degree"])))
+                            amg=AmgParams(smoother_degree=int(p["u_smoother_degree"]),
+                                          level0_degree=int(p["u_level0_degree"])),
+                            pamg=AmgParams(smoother_degree=int(p["p_smoother_degree"]),
+                                           level0_degree=int(p["p_level0_degree"])))
 
     def engine(self, direct=None):
         direct = self.params["direct_solver"] if direct is None else direct
