@@ -115,16 +115,18 @@ def _csr_i32(rp):
 
 
 def _pick_lpr(avg, nc=1, rows=None):
-    """Lanes per row: about four nonzeros per lane on large levels (fewer
-    cross-lane reductions; measured fastest on the 48^3 finest levels), more
-    lanes while a small level would leave the chip under-filled (fewer than
-    ~1M lanes in flight); at least one lane per vector component."""
-    target = max(avg / 4.0, 1.0)
+    """Lanes per row: about four nonzeros per lane on large scalar levels and
+    eight on three-column levels (fewer cross-lane reductions, more loads in
+    flight per lane; measured fastest on the 48^3 levels), more lanes while
+    a small level would leave the chip under-filled (fewer than ~1M lanes in
+    flight, ~0.5M for three columns); at least one lane per component."""
+    per_lane, fill = (8.0, 1 << 19) if nc == 3 else (4.0, 1 << 20)
+    target = max(avg / per_lane, 1.0)
     lanes = 1
     while lanes < target and lanes < 64:
         lanes *= 2
     if rows is not None:
-        while rows * lanes < (1 << 20) and lanes < 64 and lanes < avg:
+        while rows * lanes < fill and lanes < 64 and lanes < avg:
             lanes *= 2
     return max(lanes, 4 if nc == 3 else 1)
 
