@@ -267,7 +267,8 @@ class MPETTotalPressureSolver(object):
     the guess, off otherwise), "u_smoother_degree" / "p_smoother_degree"
     (Chebyshev degrees of the two AMG hierarchies), "u_level0_degree" /
     "p_level0_degree" (their degrees on the finest level, 0: the same),
-    "device"."""
+    "u_max_coarse" (rows below which the displacement hierarchy ends in a
+    dense solve), "device"."""
 
     def __init__(self, problem, params=None):
         self.problem = problem
@@ -309,6 +310,7 @@ class MPETTotalPressureSolver(object):
         params.add("p_smoother_degree", 2)
         params.add("u_level0_degree", 1)
         params.add("p_level0_degree", 0)
+        params.add("u_max_coarse", 2500)
         params.add("device", "cuda")
         return params
 
@@ -347,7 +349,8 @@ class MPETTotalPressureSolver(object):
                             outer_tol=float(p["outer_tol"]), inner_rtol=float(p["inner_rtol"]),
                             maxit=int(p["maxit"]), p0_weight=p["p0_weight"],
                             amg=AmgParams(smoother_degree=int(p["u_smoother_degree"]),
-                                          level0_degree=int(p["u_level0_degree"])),
+                                          level0_degree=int(p["u_level0_degree"]),
+                                          max_coarse=int(p["u_max_coarse"])),
                             pamg=AmgParams(smoother_degree=int(p["p_smoother_degree"]),
                                            level0_degree=int(p["p_level0_degree"])))
 
