@@ -143,7 +143,7 @@ typedef struct {
   const int32_t* coarse_blk;
   const int32_t* coarse_off;
   const int64_t* coarse_ioff;
-  int32_t degree0;               /* Chebyshev degree on level 0 (0: degree) */
+  int32_t level_degree[MPET_MAXLEV];  /* per-level Chebyshev degree (0: degree) */
 } MpetHierDesc;
 
 /* Block-diagonal preconditioner of prec (:279-283) with Dirichlet rows and

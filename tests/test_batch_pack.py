@@ -20,12 +20,12 @@ def _laplacian(n, shift=0.0):
     return L
 
 
-@pytest.mark.parametrize("deg,deg0", [(2, 0), (3, 1)])
-def test_packed_vcycle_equals_blockwise(deg, deg0):
+@pytest.mark.parametrize("deg,levdeg", [(2, ()), (3, (1,)), (2, (1, 3))])
+def test_packed_vcycle_equals_blockwise(deg, levdeg):
     from waterscapes_amd import amg
     from waterscapes_amd.amg import AmgParams
     from waterscapes_amd.batch import pack_hierarchy
-    prm = AmgParams(max_coarse=40, smoother_degree=deg, level0_degree=deg0)
+    prm = AmgParams(max_coarse=40, smoother_degree=deg, level_degrees=levdeg)
     mats = [_laplacian(4, 0.5), _laplacian(9), _laplacian(6, 0.1)]
     hiers = [amg.setup_host(A.indptr, A.indices, A.data, prm) for A in mats]
     depths = [len(h[0]) for h in hiers]

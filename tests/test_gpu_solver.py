@@ -45,7 +45,7 @@ def _setup(n=4, A=4, mode="fp64", seed=0, **kw):
 def _ref_vcycle(levels, cinv, A0, mask0, prm, b):
     """numpy V-cycle with Chebyshev smoothing (columns of b solved at once)."""
     from waterscapes_amd import amg
-    degs = [prm.level0_degree or prm.smoother_degree] + [prm.smoother_degree] * len(levels)
+    degs = [prm.degree(l) for l in range(len(levels))]
     ops = [A0] + [sp.csr_matrix((lev["A_v"], lev["A_ci"], lev["A_rp"]), shape=(lev["n"],) * 2)
                   for lev in levels[1:]]
     Ps = [sp.csr_matrix((lev["P_v"], lev["P_ci"], lev["P_rp"]), shape=(lev["n"], lev["nc"]))

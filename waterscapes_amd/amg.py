@@ -27,7 +27,11 @@ class AmgParams:
     power_iters: int = 30
     omega_scale: float = 4.0 / 3.0
     smoother_degree: int = 2          # Chebyshev degree (pre and post)
-    level0_degree: int = 0            # degree on the finest level (0: smoother_degree)
+    level_degrees: tuple = ()         # degrees of levels 0, 1, .. (0 / absent: smoother_degree)
+
+    def degree(self, level):
+        d = self.level_degrees[level] if level < len(self.level_degrees) else 0
+        return int(d) if d else int(self.smoother_degree)
     eig_ratio: float = 8.0            # Chebyshev interval [rho/ratio, 1.1 rho]
     eig_safety: float = 1.1
 

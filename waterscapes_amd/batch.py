@@ -32,7 +32,7 @@ import torch
 from . import _lib
 from ._lib import check, ptr, MAXF
 from .krylov import (MAXLEV, HierDesc, PrecDesc, SolveStats, SolverConfig, SolverOptions,
-                     _pick_lpr, compact_level)
+                     _pick_lpr, compact_level, set_level_degrees)
 from .solver import StepEngine
 
 
@@ -384,7 +384,7 @@ class MPETBatch:
         h.coarse_inv = ptr(cinv)
         h.coarse_blk, h.coarse_off, h.coarse_ioff = ptr(cblk), ptr(coff), ptr(cioff)
         h.degree = params.smoother_degree
-        h.degree0 = params.level0_degree
+        set_level_degrees(h, params)
         h.eig_ratio = params.eig_ratio
         h.eig_safety = params.eig_safety
         h.scale = 1.0

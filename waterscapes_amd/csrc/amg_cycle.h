@@ -565,7 +565,7 @@ static int build_hier(std::vector<void*>& allocs, const MpetHierDesc* d, int NC,
     L.lpr = ld.lpr ? ld.lpr : 8;
     L.lprP = ld.lprP ? ld.lprP : 4;
     L.lprR = ld.lprR ? ld.lprR : 8;
-    L.deg = (l == 0 && d->degree0 > 0) ? d->degree0 : H.deg;
+    L.deg = d->level_degree[l] > 0 ? d->level_degree[l] : H.deg;
     if (L.deg > MAXDEG) {
       mpet_set_error("invalid hierarchy description");
       return ERR_INVALID;

@@ -36,7 +36,8 @@ class HierDesc(C.Structure):
     _fields_ = [("nlev", C.c_int32), ("lev", AmgLevelDesc * MAXLEV), ("coarse_inv", C.c_void_p),
                 ("degree", C.c_int32), ("eig_ratio", C.c_double), ("eig_safety", C.c_double),
                 ("scale", C.c_double), ("mask0", C.c_void_p), ("coarse_blk", C.c_void_p),
-                ("coarse_off", C.c_void_p), ("coarse_ioff", C.c_void_p), ("degree0", C.c_int32)]
+                ("coarse_off", C.c_void_p), ("coarse_ioff", C.c_void_p),
+                ("level_degree", C.c_int32 * MAXLEV)]
 
 
 class PrecDesc(C.Structure):
@@ -92,7 +93,8 @@ class SolverConfig:
     use_csr: bool = None              # default: fp64 only
     # displacement hierarchy: one Chebyshev step on the P2 level (as many
     # MINRES iterations as two at 48^3, measured), three on the coarser ones
-    amg: AmgParams = field(default_factory=lambda: AmgParams(smoother_degree=3, level0_degree=1))
+    amg: AmgParams = field(default_factory=lambda: AmgParams(smoother_degree=3,
+                                                             level_degrees=(1,)))
     pamg: AmgParams = field(default_factory=AmgParams)
     # weight of the total-pressure mass in the preconditioner.  The reference
     # form prec uses the bare mass p0 w0 dx (mpettotalpressuresolver.py:282),
@@ -106,6 +108,14 @@ class SolverConfig:
     # fixed symmetric operator, so MINRES is unaffected beyond the slightly
     # perturbed preconditioner
     compact_level0: bool = True
+
+
+def set_level_degrees(h, params: AmgParams):
+    """Per-level Chebyshev degrees of a HierDesc from the AMG parameters."""
+    if len(params.level_degrees) > MAXLEV:
+        raise ValueError("more level degrees than hierarchy levels")
+    for l in range(MAXLEV):
+        h.level_degree[l] = params.degree(l)
 
 
 def _csr_i32(rp):
@@ -323,7 +333,7 @@ class KrylovSolver:
                 L.lprR = _pick_lpr(nnzp / max(lev["nc"], 1), nc, lev["nc"])
         h.coarse_inv = ptr(cinv)
         h.degree = params.smoother_degree
-        h.degree0 = params.level0_degree
+        set_level_degrees(h, params)
         h.eig_ratio = params.eig_ratio
         h.eig_safety = params.eig_safety
         h.scale = scale

@@ -265,8 +265,9 @@ class MPETTotalPressureSolver(object):
     "extrapolate" (initial guess 2 u_n - u_{n-1} instead of up_; None =
     on for "mixed", whose outer tolerance bounds the error independently of
     the guess, off otherwise), "u_smoother_degree" / "p_smoother_degree"
-    (Chebyshev degrees of the two AMG hierarchies), "u_level0_degree" /
-    "p_level0_degree" (their degrees on the finest level, 0: the same),
+    (Chebyshev degrees of the two AMG hierarchies), "u_level_degrees" /
+    "p_level_degrees" (their degrees on levels 0, 1, ..; 0 or absent: the
+    smoother degree),
     "u_max_coarse" (rows below which the displacement hierarchy ends in a
     dense solve), "device"."""
 
@@ -308,8 +309,8 @@ class MPETTotalPressureSolver(object):
         params.add("extrapolate", None)
         params.add("u_smoother_degree", 3)
         params.add("p_smoother_degree", 2)
-        params.add("u_level0_degree", 1)
-        params.add("p_level0_degree", 0)
+        params.add("u_level_degrees", (1,))
+        params.add("p_level_degrees", ())
         params.add("u_max_coarse", 2500)
         params.add("device", "cuda")
         return params
@@ -349,10 +350,10 @@ class MPETTotalPressureSolver(object):
                             outer_tol=float(p["outer_tol"]), inner_rtol=float(p["inner_rtol"]),
                             maxit=int(p["maxit"]), p0_weight=p["p0_weight"],
                             amg=AmgParams(smoother_degree=int(p["u_smoother_degree"]),
-                                          level0_degree=int(p["u_level0_degree"]),
+                                          level_degrees=tuple(p["u_level_degrees"]),
                                           max_coarse=int(p["u_max_coarse"])),
                             pamg=AmgParams(smoother_degree=int(p["p_smoother_degree"]),
-                                           level0_degree=int(p["p_level0_degree"])))
+                                           level_degrees=tuple(p["p_level_degrees"])))
 
     def engine(self, direct=None):
         direct = self.params["direct_solver"] if direct is None else direct
