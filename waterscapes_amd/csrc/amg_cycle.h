@@ -52,11 +52,24 @@ __device__ __forceinline__ double block_sum(double v) {
   return r;
 }
 
+// Fixed-order sum of p[lo, hi) by the threads of a block into four
+// accumulators per thread (four loads in flight); per-thread value.
+__device__ __forceinline__ double thread_sum(const double* __restrict__ p, int lo, int hi) {
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int i = lo + threadIdx.x;
+  for (; i + 3 * NT < hi; i += 4 * NT) {
+    a0 += p[i];
+    a1 += p[i + NT];
+    a2 += p[i + 2 * NT];
+    a3 += p[i + 3 * NT];
+  }
+  for (; i < hi; i += NT) a0 += p[i];
+  return (a0 + a1) + (a2 + a3);
+}
+
 // Fixed-order sum of n partials by one block; valid in thread 0.
 __device__ __forceinline__ double sum_partials(const double* __restrict__ p, int n) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += NT) s += p[i];
-  return block_sum(s);
+  return block_sum(thread_sum(p, 0, n));
 }
 
 static int grid_for(long long items, int per_block, int cap) {
@@ -72,6 +85,9 @@ static int grid_for(long long items, int per_block, int cap) {
 // the row group finishes component c.
 // MODE 0: y = A x        MODE 1: y = b - A x       MODE 2: y += A x
 // MODE 3: Chebyshev step d = cd d + cr Dinv (b - A x); y = x + d
+// MODE 4: restriction entering the coarse level's smoother: y = A x, then
+//         its first Chebyshev step from zero d = y2 = cr Dinv y (dinv and
+//         cr of the coarse level)
 // ---------------------------------------------------------------------------
 template <typename T, int NC, int LPR, int MODE, bool COMPACT = false>
 __global__ void __launch_bounds__(NT)
@@ -79,8 +95,8 @@ spmm_kernel(int nrows, const int* __restrict__ rp, const int* __restrict__ ci,
             const T* __restrict__ val, const T* __restrict__ x, const T* __restrict__ b,
             T* __restrict__ y, const T* __restrict__ dinv, T* __restrict__ d, T cd, T cr,
             const uint8_t* __restrict__ mask, Skip sk, const int* __restrict__ grp,
-            const __half* __restrict__ val16 = nullptr, const int16_t* __restrict__ dci = nullptr,
-            T vscale = (T)1) {
+            T* __restrict__ y2, const __half* __restrict__ val16 = nullptr,
+            const int16_t* __restrict__ dci = nullptr, T vscale = (T)1) {
   // COMPACT: values stored as half precision times 1/vscale, columns as
   // signed 16-bit offsets from the row (half the bytes per nonzero)
   static_assert(NC <= LPR, "one lane per component in the epilogue");
@@ -113,6 +129,7 @@ spmm_kernel(int nrows, const int* __restrict__ rp, const int* __restrict__ ci,
     pd = d[o];
     pdinv = dinv[r];
   }
+  if constexpr (MODE == 4) pdinv = dinv[r];
   const int mrow = mask ? (int)mask[r] : 0;
   // U nonzeros per lane per trip: every index, value and gather load of a
   // trip is issued before the first use (a missing nonzero reloads the
@@ -158,6 +175,11 @@ spmm_kernel(int nrows, const int* __restrict__ rp, const int* __restrict__ ci,
     y[o] = m ? (T)0 : pb - mine;
   } else if (MODE == 2) {
     if (!m) y[o] = px + mine;
+  } else if (MODE == 4) {
+    y[o] = mine;
+    const T v = cr * pdinv * mine;
+    d[o] = v;
+    y2[o] = v;
   } else {
     T dd = m ? (T)0 : cd * pd + cr * pdinv * (pb - mine);
     d[o] = dd;
@@ -380,8 +402,9 @@ static void free_fork(Precond<T>& P) {
 
 template <typename T, int NC, int MODE>
 static void spmm(const LevelDev<T>& L, int which, const T* x, const T* b, T* y, T* d, T cd,
-                 T cr, Skip sk, hipStream_t s) {
-  // which: 0 level operator, 1 prolongator (rows n), 2 restriction (rows nc)
+                 T cr, Skip sk, hipStream_t s, const T* dinv = nullptr, T* y2 = nullptr) {
+  // which: 0 level operator, 1 prolongator (rows n), 2 restriction (rows nc);
+  // dinv: the diagonal inverse used by the epilogue (default the level's)
   int nrows = which == 2 ? L.nc : L.n;
   const int* rp = which == 0 ? L.rp : (which == 1 ? L.Prp : L.Rrp);
   const int* ci = which == 0 ? L.ci : (which == 1 ? L.Pci : L.Rci);
@@ -390,23 +413,24 @@ static void spmm(const LevelDev<T>& L, int which, const T* x, const T* b, T* y, 
   if (lpr < NC) lpr = NC == 3 ? 4 : NC;
   const uint8_t* mask = which == 2 ? nullptr : L.mask;
   const int* grp = which == 2 ? L.grpc : L.grp;
+  if (!dinv) dinv = L.dinv;
   long long threads = (long long)nrows * lpr;
   int blocks = (int)((threads + NT - 1) / NT);
   if (blocks == 0) return;
   if (which == 0 && L.v16) {
-    if constexpr (sizeof(T) == 4) {
+    if constexpr (sizeof(T) == 4 && MODE != 4) {
 #define SPMM_CASE(LP)                                                                          \
   case LP:                                                                                     \
     hipLaunchKernelGGL((spmm_kernel<T, NC, (LP < NC ? 4 : LP), MODE, true>), dim3(blocks),     \
-                       dim3(NT), 0, s, nrows, rp, ci, v, x, b, y, L.dinv, d, cd, cr, mask, sk, grp, \
-                       L.v16, L.dci, (T)L.vscale);                                             \
+                       dim3(NT), 0, s, nrows, rp, ci, v, x, b, y, dinv, d, cd, cr, mask, sk, grp, \
+                       y2, L.v16, L.dci, (T)L.vscale);                                         \
     break;
       switch (lpr) {
         SPMM_CASE(4) SPMM_CASE(8) SPMM_CASE(16) SPMM_CASE(32) SPMM_CASE(64)
         default: hipLaunchKernelGGL((spmm_kernel<T, NC, (NC > 4 ? 8 : 4), MODE, true>),
                                     dim3((unsigned)(((long long)nrows * (NC > 4 ? 8 : 4) + NT - 1) / NT)),
-                                    dim3(NT), 0, s, nrows, rp, ci, v, x, b, y, L.dinv, d, cd, cr,
-                                    mask, sk, grp, L.v16, L.dci, (T)L.vscale);
+                                    dim3(NT), 0, s, nrows, rp, ci, v, x, b, y, dinv, d, cd, cr,
+                                    mask, sk, grp, y2, L.v16, L.dci, (T)L.vscale);
       }
 #undef SPMM_CASE
       return;
@@ -415,7 +439,7 @@ static void spmm(const LevelDev<T>& L, int which, const T* x, const T* b, T* y, 
 #define SPMM_CASE(LP)                                                                          \
   case LP:                                                                                     \
     hipLaunchKernelGGL((spmm_kernel<T, NC, (LP < NC ? 4 : LP), MODE>), dim3(blocks), dim3(NT), \
-                       0, s, nrows, rp, ci, v, x, b, y, L.dinv, d, cd, cr, mask, sk, grp);    \
+                       0, s, nrows, rp, ci, v, x, b, y, dinv, d, cd, cr, mask, sk, grp, y2);  \
     break;
   switch (lpr) {
     SPMM_CASE(1) SPMM_CASE(2) SPMM_CASE(4) SPMM_CASE(8) SPMM_CASE(16) SPMM_CASE(32)
@@ -425,11 +449,19 @@ static void spmm(const LevelDev<T>& L, int which, const T* x, const T* b, T* y, 
 #undef SPMM_CASE
 }
 
+// Buffer of the first Chebyshev step from zero on a level whose smoothing
+// result goes to out.
+template <typename T>
+static T* first_dst(const LevelDev<T>& L, T* out) {
+  return L.deg == 1 ? out : L.S1;
+}
+
 // k Chebyshev steps on level l; from zero (x0 == nullptr, b scaled by
-// bscale and copied to bcopy when given) or from x0; result in out.
+// bscale and copied to bcopy when given; entered: the first step is already
+// in first_dst, made by the restriction) or from x0; result in out.
 template <typename T, int NC>
 static void smooth(const Hier<T>& H, int l, const T* b, T bscale, T* bcopy, const T* x0,
-                   T* out, Skip sk, hipStream_t s) {
+                   T* out, Skip sk, hipStream_t s, bool entered = false) {
   const LevelDev<T>& L = H.lev[l];
   const int k = L.deg;
   const T* cur = x0;
@@ -438,10 +470,11 @@ static void smooth(const Hier<T>& H, int l, const T* b, T bscale, T* bcopy, cons
   int flip = 0;
   const T* bb = b;
   if (!x0) {
-    T* dst = (k == 1) ? out : bufs[flip];
+    T* dst = first_dst(L, out);
     int blocks = (int)(((long long)NC * L.n + NT - 1) / NT);
-    hipLaunchKernelGGL((cheb0_kernel<T, NC>), dim3(blocks), dim3(NT), 0, s, L.n, b, bscale,
-                       L.dinv, (T)L.cr[0], L.D, dst, bcopy, L.mask, sk, L.grp);
+    if (!entered)
+      hipLaunchKernelGGL((cheb0_kernel<T, NC>), dim3(blocks), dim3(NT), 0, s, L.n, b, bscale,
+                         L.dinv, (T)L.cr[0], L.D, dst, bcopy, L.mask, sk, L.grp);
     if (bcopy) bb = bcopy;
     cur = dst;
     i = 1;
@@ -458,7 +491,7 @@ static void smooth(const Hier<T>& H, int l, const T* b, T bscale, T* bcopy, cons
 
 template <typename T, int NC>
 static void vcycle(const Hier<T>& H, int l, const T* b, T bscale, T* out, Skip sk,
-                   hipStream_t s) {
+                   hipStream_t s, bool entered = false) {
   const LevelDev<T>& L = H.lev[l];
   if (l == H.nlev - 1) {
     const T* bb = b;
@@ -478,12 +511,20 @@ static void vcycle(const Hier<T>& H, int l, const T* b, T bscale, T* out, Skip s
     return;
   }
   T* bcopy = (bscale != (T)1) ? L.B : nullptr;
-  smooth<T, NC>(H, l, b, bscale, bcopy, nullptr, L.X, sk, s);
+  smooth<T, NC>(H, l, b, bscale, bcopy, nullptr, L.X, sk, s, entered);
   const T* bb = bcopy ? bcopy : b;
   spmm<T, NC, 1>(L, 0, L.X, bb, L.R, nullptr, 0, 0, sk, s);
   const LevelDev<T>& C = H.lev[l + 1];
-  spmm<T, NC, 0>(L, 2, L.R, nullptr, C.B, nullptr, 0, 0, sk, s);
-  vcycle<T, NC>(H, l + 1, C.B, (T)1, C.E, sk, s);
+  if (l + 1 < H.nlev - 1) {
+    // restriction fused with the coarse level's first smoothing step (its
+    // pre-smoothing writes to C.X)
+    spmm<T, NC, 4>(L, 2, L.R, nullptr, C.B, C.D, 0, (T)C.cr[0], sk, s, C.dinv,
+                   first_dst(C, C.X));
+    vcycle<T, NC>(H, l + 1, C.B, (T)1, C.E, sk, s, true);
+  } else {
+    spmm<T, NC, 0>(L, 2, L.R, nullptr, C.B, nullptr, 0, 0, sk, s);
+    vcycle<T, NC>(H, l + 1, C.B, (T)1, C.E, sk, s);
+  }
   spmm<T, NC, 2>(L, 1, C.E, nullptr, L.X, nullptr, 0, 0, sk, s);
   smooth<T, NC>(H, l, bb, (T)1, nullptr, L.X, out, sk, s);
 }

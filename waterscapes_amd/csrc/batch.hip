@@ -188,10 +188,8 @@ bop_p_kernel(BOp<T> op, const Tile* __restrict__ tiles, int part_off, const T* _
 __device__ __forceinline__ double inst_sum(const double* __restrict__ partial,
                                            const int* __restrict__ r0, const int* __restrict__ r1,
                                            int k) {
-  double s = 0.0;
-  for (int i = r0[k] + threadIdx.x; i < r0[k + 1]; i += NT) s += partial[i];
-  for (int i = r1[k] + threadIdx.x; i < r1[k + 1]; i += NT) s += partial[i];
-  return block_sum(s);
+  return block_sum(thread_sum(partial, r0[k], r0[k + 1]) +
+                   thread_sum(partial, r1[k], r1[k + 1]));
 }
 
 __global__ void __launch_bounds__(NT)
