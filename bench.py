@@ -360,7 +360,7 @@ def main():
     ap.add_argument("--n", type=int, default=48)
     ap.add_argument("--A", type=int, default=4)
     ap.add_argument("--outer-tol", type=float, default=3e-11)
-    ap.add_argument("--inner-rtol", type=float, default=3e-4)
+    ap.add_argument("--inner-rtol", type=float, default=1e-4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=8)
     ap.add_argument("--skip-fp64", action="store_true")
