@@ -20,8 +20,15 @@ boundary-condition logic of the reference:
   * direct LU (MUMPS in the reference; SuperLU here) .................... :365,403
   * preconditioned MINRES with the block-diagonal preconditioner ......... :448,493
 
+It also restates the two-field MPETSolver (mpet/mpetsolver.py): forms
+:196-201 with the stress of mpetproblem.py:18-24, Robin :252-254, prec
+:269-271, Dirichlet DOFs on sub(i+1) (:74-81), and its direct time loop
+:382-462 (OracleProblem(formulation="two_field")).
+
 Canonical numbering (inputs to every function): u DOF = 3*a + j for P2 node a,
-component j; p_f DOF = 3*nP2 + (A+1)*c + f for P1 node c, field f = 0..A.
+component j; p_f DOF = 3*nP2 + F*c + f for P1 node c, field f = 0..F-1 with
+F = A+1 (p_0, p_1..p_A) for the total-pressure and F = A (p_1..p_A) for the
+two-field formulation.
 
 Pinning (see tests/test_oracle.py): element tensors against sympy-exact
 fixtures (tests/golden/), P1 closed forms, kernels (rigid motions, constants),
@@ -159,6 +166,7 @@ def element_tensors(coords, cells):
       'B'   (nC,30,4)   int psi_c div(phi_a e_j)   (row 3a+j, col c);
       'M'   (nC,4,4)    P1 mass;
       'S'   (nC,4,4)    P1 stiffness;
+      'D'   (nC,30,30)  int div(phi_a e_i) div(phi_b e_j) (two-field lambda term);
     """
     _, det, Jinv = cell_geometry(coords, cells)
     qp, qw = tet_rule(2)
@@ -182,7 +190,9 @@ def element_tensors(coords, cells):
     M = np.einsum("cq,qa,qb->cab", wdet, psi, psi)
     g1 = np.einsum("ckm,ak->cam", Jinv, P1_REF_GRADS)      # nC,4,3
     S = (det / 6.0)[:, None, None] * np.einsum("cam,cbm->cab", g1, g1)
-    return {"E": E, "G": G, "B": B, "M": M, "S": S, "det": det}
+    # div(phi_b e_j) div(phi_a e_i) = d_i phi_a d_j phi_b  (row 3a+i, col 3b+j)
+    D = np.transpose(Gm, (0, 1, 3, 2, 4)).reshape(nC, 30, 30)
+    return {"E": E, "G": G, "B": B, "M": M, "S": S, "D": D, "det": det}
 
 
 # ---------------------------------------------------------------------------
@@ -228,17 +238,25 @@ def pressure_coefficients(alpha, c, S, lmbda, dt, theta):
 
 
 class Layout:
-    def __init__(self, nP2, nP1, A):
+    """Canonical numbering with F pressure fields per P1 node: F = A + 1
+    (total pressure: p_0, p_1..p_A) or F = A (two-field: p_1..p_A)."""
+
+    def __init__(self, nP2, nP1, A, F=None):
         self.nP2, self.nP1, self.A = nP2, nP1, A
+        self.F = A + 1 if F is None else F
         self.nu = 3 * nP2
-        self.n = self.nu + (A + 1) * nP1
+        self.n = self.nu + self.F * nP1
+
+    def net(self, i):
+        """Layout field of network i (0-based)."""
+        return i + (self.F - self.A)
 
     def udofs(self, cell_p2):
         """(nC,30) local u DOFs 3a+j."""
         return (3 * cell_p2[:, :, None] + np.arange(3)).reshape(cell_p2.shape[0], 30)
 
     def pdofs(self, cell_p1, f):
-        return self.nu + (self.A + 1) * cell_p1 + f
+        return self.nu + self.F * cell_p1 + f
 
 
 def _coo(rows, cols, vals, n):
@@ -326,6 +344,133 @@ def assemble_matrix(coords, cells, cell_p2, cell_p1, nP2, nP1, params,
 
 
 # ---------------------------------------------------------------------------
+# two-field formulation (MPETSolver)
+# ---------------------------------------------------------------------------
+
+
+def two_field_pressure_coefficients(c, S, dt, theta):
+    """A x A coefficients of the P1 mass in the pressure rows of the
+    two-field form (mpet/mpetsolver.py:198, 201): Cm for lhs(F), Bm for the
+    up_ part of rhs(F)."""
+    A = len(c)
+    Cm = np.zeros((A, A))
+    Bm = np.zeros((A, A))
+    for i in range(A):
+        for j in range(A):
+            if i == j:
+                tr = sum(S[i][k] for k in range(A) if k != i)
+                Cm[i, j] = -c[i] - dt * theta * tr
+                Bm[i, j] = -c[i] + dt * (1 - theta) * tr
+            else:
+                Cm[i, j] = dt * theta * S[i][j]
+                Bm[i, j] = -dt * (1 - theta) * S[i][j]
+    return Cm, Bm
+
+
+def assemble_two_field(coords, cells, cell_p2, cell_p1, nP2, nP1, params,
+                       dt, theta, K_cells=None, robin=None, which="A"):
+    """Global FP64 CSR of the two-field MPETSolver system, fields u,
+    p_1..p_A with F = A pressure fields per node (mpet/mpetsolver.py:196-201,
+    stress sigma = 2 mu eps + lambda div I of mpetproblem.py:18-24).
+
+    which == "A": lhs(F) + sum_i lhs(F2a_i) (:252-253, added at :336-338, :413-415):
+      (u,v)     int 2 mu eps(u):eps(v) + lambda div u div v       :196
+      (p_i,v)   -alpha_i int p_i div v                            :197
+      (u,q_i)   -alpha_i int div u q_i                            :199
+      (p_j,q_i) -c_i delta_ij M - dt theta delta_ij K_i S
+                - dt theta (delta_ij sum_k S_ik - S_ij) M          :198, 200-201
+                - dt theta delta_ij int_{ds_i(2)} beta_i p_i q_i   :252
+    which == "P": prec (:269-271): mu grad u : grad v;
+      (c_i + dt theta sum_{j != i} S_ij) M + dt theta K_i S (the reference's
+      ``dt*theta**inner(...)`` read as the product dt*theta*inner(...)).
+    """
+    A = int(params["J"])
+    mu, lmbda = lame(params["E"], params["nu"])
+    alpha, c, S = params["alpha"], params["c"], params["S"]
+    lay = Layout(nP2, nP1, A, A)
+    et = element_tensors(coords, cells)
+    nC = cells.shape[0]
+    if K_cells is None:
+        K_cells = np.tile(np.asarray(params["K"], float), (nC, 1))
+    ud = lay.udofs(cell_p2)
+    blocks = []
+    if which == "A":
+        blocks.append((ud, ud, 2 * mu * et["E"] + lmbda * et["D"]))
+        Cm, _ = two_field_pressure_coefficients(c, S, dt, theta)
+        for i in range(A):
+            pdi = lay.pdofs(cell_p1, i)
+            blocks.append((ud, pdi, -alpha[i] * et["B"]))
+            blocks.append((pdi, ud, -alpha[i] * np.transpose(et["B"], (0, 2, 1))))
+            for j in range(A):
+                v = Cm[i, j] * et["M"]
+                if i == j:
+                    v = v - dt * theta * K_cells[:, i, None, None] * et["S"]
+                blocks.append((pdi, lay.pdofs(cell_p1, j), v))
+    else:
+        G = mu * et["G"]
+        Gu = np.zeros((nC, 10, 3, 10, 3))
+        for j in range(3):
+            Gu[:, :, j, :, j] = G
+        blocks.append((ud, ud, Gu.reshape(nC, 30, 30)))
+        for i in range(A):
+            a_i = c[i] + sum(dt * theta * S[i][j] for j in range(A) if j != i)
+            v = a_i * et["M"] + dt * theta * K_cells[:, i, None, None] * et["S"]
+            pdi = lay.pdofs(cell_p1, i)
+            blocks.append((pdi, pdi, v))
+    rows, cols, vals = [], [], []
+    for (ld, rd, v) in blocks:
+        R, Cc = _blk(ld, rd, v)
+        rows.append(R.reshape(-1))
+        cols.append(Cc.reshape(-1))
+        vals.append(v.reshape(-1))
+    if which == "A" and robin is not None:
+        for i, rb in enumerate(robin):
+            if rb is None:
+                continue
+            tri_dofs, Mf = rb
+            pdi = lay.nu + A * tri_dofs + i
+            R, Cc = _blk(pdi, pdi, Mf)
+            rows.append(R.reshape(-1))
+            cols.append(Cc.reshape(-1))
+            vals.append((-dt * theta * Mf).reshape(-1))
+    M = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(lay.n, lay.n)).tocsr()
+    M.sum_duplicates()
+    M.sort_indices()
+    return M
+
+
+def two_field_rhs_matrix(mesh, params, dt, theta, K_cells, lay):
+    """Matrix of the up_-linear part of rhs(F) of the two-field form at t_n
+    (:196-201): pressure rows i get -alpha_i int div u_ q_i,
+    Bm_ij (M p_j_) and dt (1 - theta) K_i S p_i_."""
+    m = mesh
+    A = int(params["J"])
+    alpha = params["alpha"]
+    _, Bm = two_field_pressure_coefficients(params["c"], params["S"], dt, theta)
+    et = element_tensors(m["coords"], m["cells"])
+    nC = m["cells"].shape[0]
+    K = K_cells if K_cells is not None else np.tile(np.asarray(params["K"], float), (nC, 1))
+    ud = lay.udofs(m["cell_p2"])
+    rows, cols, vals = [], [], []
+    for i in range(A):
+        pdi = lay.pdofs(m["cell_p1"], i)
+        blocks = [(pdi, ud, -alpha[i] * np.transpose(et["B"], (0, 2, 1)))]
+        for j in range(A):
+            v = Bm[i, j] * et["M"]
+            if i == j:
+                v = v + dt * (1 - theta) * K[:, i, None, None] * et["S"]
+            blocks.append((pdi, lay.pdofs(m["cell_p1"], j), v))
+        for (ld, rd, v) in blocks:
+            R, Cc = _blk(ld, rd, v)
+            rows.append(R.reshape(-1))
+            cols.append(Cc.reshape(-1))
+            vals.append(v.reshape(-1))
+    return sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(lay.n, lay.n)).tocsr()
+
+
+# ---------------------------------------------------------------------------
 # sources (Expression(degree=k) interpolated into P_k, then exact quadrature)
 # ---------------------------------------------------------------------------
 
@@ -405,7 +550,8 @@ def dirichlet_dofs(mesh_facets, facet_cell, facet_local, cells, cell_p2, cell_p1
                    markers, field, lay):
     """DOFs in the closure of facets marked 0 (DOLFIN topological method).
     field 0 -> all u DOFs of the facet's 3 vertex + 3 edge P2 nodes;
-    field i >= 1 -> p_i DOFs of its 3 vertices."""
+    field k >= 1 -> the DOFs of layout pressure field k - 1 at its 3 vertices
+    (sub-space k of the mixed space)."""
     sel = np.nonzero(np.asarray(markers) == 0)[0]
     if sel.size == 0:
         return np.zeros(0, dtype=np.int64)
@@ -421,7 +567,7 @@ def dirichlet_dofs(mesh_facets, facet_cell, facet_local, cells, cell_p2, cell_p1
         nodes = np.unique(np.concatenate(nodes))
         return np.sort((3 * nodes[:, None] + np.arange(3)).reshape(-1))
     nodes = np.unique(np.concatenate([cell_p1[cl, verts[:, k]] for k in range(3)]))
-    return np.sort(lay.nu + (lay.A + 1) * nodes + field)
+    return np.sort(lay.nu + lay.F * nodes + (field - 1))
 
 
 def apply_row_replace(A, dofs):
@@ -531,7 +677,11 @@ class OracleProblem:
                  p_bar=None, f_degree=3, g_degree=3, K_cells=None,
                  momentum_markers=None, continuity_markers=None,
                  s=None, s_degree=3, I=None, I_degree=3, beta=None,
-                 p_robin=None, robin_degree=1, p_robin_degree=None):
+                 p_robin=None, robin_degree=1, p_robin_degree=None,
+                 formulation="total_pressure"):
+        if formulation not in ("total_pressure", "two_field"):
+            raise ValueError(formulation)
+        self.two_field = formulation == "two_field"
         self.m = mesh
         self.params = params
         self.A = int(params["J"])
@@ -546,18 +696,20 @@ class OracleProblem:
         self.beta, self.p_robin, self.robin_degree = beta, p_robin, robin_degree
         self.p_robin_degree = robin_degree if p_robin_degree is None else p_robin_degree
         self.K_cells = K_cells
-        self.lay = Layout(mesh["nP2"], mesh["nP1"], self.A)
+        self.lay = Layout(mesh["nP2"], mesh["nP1"], self.A,
+                          self.A if self.two_field else self.A + 1)
         nF = mesh["facets"].shape[0]
         self.mm = momentum_markers if momentum_markers is not None else np.full(nF, 2**63 - 1)
         self.cm = continuity_markers or [np.full(nF, 2**63 - 1) for _ in range(self.A)]
         self._robin_mass = self._robin_terms()
         m = mesh
-        self.A_mat = assemble_matrix(m["coords"], m["cells"], m["cell_p2"], m["cell_p1"],
-                                     m["nP2"], m["nP1"], params, self.dt, self.theta,
-                                     K_cells, robin=self._robin_mass, which="A")
-        self.P_mat = assemble_matrix(m["coords"], m["cells"], m["cell_p2"], m["cell_p1"],
-                                     m["nP2"], m["nP1"], params, self.dt, self.theta,
-                                     K_cells, which="P")
+        asm = assemble_two_field if self.two_field else assemble_matrix
+        self.A_mat = asm(m["coords"], m["cells"], m["cell_p2"], m["cell_p1"],
+                         m["nP2"], m["nP1"], params, self.dt, self.theta,
+                         K_cells, robin=self._robin_mass, which="A")
+        self.P_mat = asm(m["coords"], m["cells"], m["cell_p2"], m["cell_p1"],
+                         m["nP2"], m["nP1"], params, self.dt, self.theta,
+                         K_cells, which="P")
         self.bc_dofs = self._bc_dofs()
         self._B_mat = None
 
@@ -595,7 +747,7 @@ class OracleProblem:
         for i in range(self.A):
             out.append(dirichlet_dofs(m["facets"], m["facet_cell"], m["facet_local"],
                                       m["cells"], m["cell_p2"], m["cell_p1"],
-                                      self.cm[i], i + 1, self.lay))
+                                      self.cm[i], self.lay.net(i) + 1, self.lay))
         return out
 
     def bc_values(self, t):
@@ -614,7 +766,7 @@ class OracleProblem:
         for i in range(self.A):
             d = self.bc_dofs[i + 1]
             if d.size:
-                nodes = (d - lay.nu) // (self.A + 1)
+                nodes = (d - lay.nu) // lay.F
                 pb = self.p_bar[i](m["coords"][nodes], t) if self.p_bar is not None else np.zeros(nodes.size)
                 vals.append(np.asarray(pb, float).reshape(-1))
             else:
@@ -625,6 +777,10 @@ class OracleProblem:
     def B_matrix(self):
         """Matrix of the up_-linear part of b (rhs of F, evaluated at t_n)."""
         if self._B_mat is not None:
+            return self._B_mat
+        if self.two_field:
+            self._B_mat = two_field_rhs_matrix(self.m, self.params, self.dt, self.theta,
+                                               self.K_cells, self.lay)
             return self._B_mat
         m = self.m
         A = self.A
@@ -671,7 +827,7 @@ class OracleProblem:
                 loc = cell_source_vector(m["coords"], m["cells"], m["cell_p1"],
                                          lambda X, t, gi=self.g[i]: np.asarray(gi(X, t)).reshape(-1, 1),
                                          self.g_degree, t_theta, 1)
-                b += self.dt * scatter_vector(lay.n, lay.pdofs(m["cell_p1"], i + 1), loc[..., 0])
+                b += self.dt * scatter_vector(lay.n, lay.pdofs(m["cell_p1"], lay.net(i)), loc[..., 0])
         if self.I is not None:
             for i in range(self.A):
                 if self.I[i] is None:
@@ -684,7 +840,7 @@ class OracleProblem:
                 loc = facet_source_vector(m["coords"], tri_v, 3,
                                           lambda X, t, Ii=self.I[i]: np.asarray(Ii(X, t)).reshape(-1, 1),
                                           self.I_degree, t_theta)
-                b += self.dt * scatter_vector(lay.n, lay.pdofs(tri_n, i + 1), loc[..., 0])
+                b += self.dt * scatter_vector(lay.n, lay.pdofs(tri_n, lay.net(i)), loc[..., 0])
         # rhs(L2_i): dt beta((1-theta) p_i_ - p_robin) w_i ds_i(2) at t_theta
         if self.beta is not None:
             for i in range(self.A):
@@ -693,7 +849,7 @@ class OracleProblem:
                     continue
                 tri_v = np.stack([m["cells"][cl, verts[:, k]] for k in range(3)], 1)
                 tri_n = np.stack([m["cell_p1"][cl, verts[:, k]] for k in range(3)], 1)
-                pprev = up_prev[lay.pdofs(tri_n, i + 1)]                     # nF,3
+                pprev = up_prev[lay.pdofs(tri_n, lay.net(i))]                # nF,3
 
                 def fdat(X, t, bi=self.beta[i], pr=self.p_robin[i] if self.p_robin else None):
                     return np.asarray(bi(X, t)).reshape(-1, 1)
@@ -706,7 +862,7 @@ class OracleProblem:
                                               self.robin_degree, self.p_robin[i],
                                               self.p_robin_degree, t_theta)
                     loc = loc - pr
-                b += self.dt * scatter_vector(lay.n, lay.pdofs(tri_n, i + 1), loc)
+                b += self.dt * scatter_vector(lay.n, lay.pdofs(tri_n, lay.net(i)), loc)
         # L0: f . v dx + s . v ds_m(1) at t_{n+1}
         if self.f is not None:
             loc = cell_source_vector(m["coords"], m["cells"], m["cell_p2"], self.f,

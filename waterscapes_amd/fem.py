@@ -10,8 +10,11 @@ global numbering used by every backend and by the FP64 oracle:
   * u DOF     = 3 * a + j                          (a P2 node, j component);
   * p_f DOF   = 3 * nP2 + (A + 1) * c + f          (c P1 node, f = 0..A).
 
-Components of u and the A+1 pressures of one node are adjacent in memory, so
-3x3 blocks and (A+1)-vectors of a node are read with one contiguous load.
+The two-field space of MPETSolver (u, p_1..p_A; mpet/mpetsolver.py:100-131)
+uses the same numbering with F = A pressure fields per node in place of A+1.
+Components of u and the pressures of one node are adjacent in memory, so
+3x3 blocks and the pressure vector of a node are read with one contiguous
+load.
 
 ``Constant``, ``Expression``, ``Function``, ``interpolate``, ``assign`` and
 ``errornorm`` are the DOLFIN look-alikes the reference API exposes.
@@ -326,16 +329,21 @@ Expression.__mul__ = lambda self, other: (FacetNormalTraction(self)
 
 
 class MixedSpace:
-    """[P2]^3 x P1^(A+1) on a mesh, canonical numbering (module docstring)."""
+    """[P2]^3 x P1^F on a mesh, canonical numbering (module docstring):
+    F = A + 1 fields u, p_0, p_1..p_A of the total-pressure formulation, or
+    (total_pressure=False) F = A fields u, p_1..p_A of the two-field
+    MPETSolver (mpet/mpetsolver.py:100-131, no nullspace multipliers)."""
 
-    def __init__(self, mesh, A, dofmap=None):
+    def __init__(self, mesh, A, dofmap=None, total_pressure=True):
         self.mesh = mesh
         self.A = int(A)
+        self.total_pressure = bool(total_pressure)
+        self.F = self.A + 1 if self.total_pressure else self.A
         self.dofmap = dofmap if dofmap is not None else DofMap(mesh)
         self.nP2 = self.dofmap.nP2
         self.nP1 = self.dofmap.nP1
         self.nu = 3 * self.nP2
-        self.np = (self.A + 1) * self.nP1
+        self.np = self.F * self.nP1
         self.ndofs = self.nu + self.np
 
     def dim(self):
@@ -345,14 +353,14 @@ class MixedSpace:
         return SubSpace(self, i)
 
     def num_sub_spaces(self):
-        return self.A + 2
+        return self.F + 1
 
     def field_index(self, i):
-        """Global DOF indices of field i (0 = u (nP2*3, node-major), 1 = p_0,
-        2.. = p_1..p_A)."""
+        """Global DOF indices of field i (0 = u (nP2*3, node-major); then the
+        pressure fields in order: p_0, p_1..p_A, or p_1..p_A)."""
         if i == 0:
             return np.arange(self.nu)
-        return self.nu + np.arange(self.nP1) * (self.A + 1) + (i - 1)
+        return self.nu + np.arange(self.nP1) * self.F + (i - 1)
 
 
 class SubSpace:
@@ -458,7 +466,7 @@ class Function:
         sp = self._space
         if not isinstance(sp, MixedSpace):
             return (self,)
-        return tuple(FieldView(self, i) for i in range(sp.A + 2))
+        return tuple(FieldView(self, i) for i in range(sp.F + 1))
 
     def copy(self, deepcopy=True):
         d = self.data.copy() if isinstance(self.data, np.ndarray) else self.data.clone()
@@ -472,7 +480,7 @@ class Function:
             return a.reshape(-1, 3) if sp.i == 0 else a
         if i == 0:
             return a[: sp.nu].reshape(sp.nP2, 3)
-        return a[sp.nu:].reshape(sp.nP1, sp.A + 1)[:, i - 1]
+        return a[sp.nu:].reshape(sp.nP1, sp.F)[:, i - 1]
 
 
 class FieldView:

@@ -87,6 +87,59 @@ def preconditioner_coefficients(alpha, c, S, lmbda, dt, theta):
     return diag, stiff
 
 
+# ---------------------------------------------------------------------------
+# two-field formulation (MPETSolver, mpet/mpetsolver.py:196-201, 249-254, 269-271)
+# ---------------------------------------------------------------------------
+
+
+def two_field_matrix_coefficients(c, S, dt, theta):
+    """Pressure block of lhs(F) of the two-field form: C[i][j] multiplies the
+    P1 mass, stiff[i] the K_i-weighted stiffness (fields p_1..p_A only):
+
+      (p_j,q_i) -c_i delta_ij                                 (:198)
+                -dt theta (delta_ij sum_k S_ik - S_ij)        (:201)
+      stiffness -dt theta K_i                                 (:200)"""
+    A = len(c)
+    C = np.zeros((A, A))
+    for i in range(A):
+        for j in range(A):
+            if i == j:
+                C[i, j] = -float(c[i]) - dt * theta * sum(_s(S, i, k) for k in range(A) if k != i)
+            else:
+                C[i, j] = dt * theta * _s(S, i, j)
+    return C, np.full(A, -dt * theta)
+
+
+def two_field_rhs_coefficients(c, S, dt, theta):
+    """Part of rhs(F) of the two-field form linear in the pressures of up_
+    (at t_n): -c_i p_i_ + dt (1-theta) (K_i grad p_i_, grad q_i)
+    + dt (1-theta) sum_j S_ij (p_i_ - p_j_).  The displacement part,
+    -alpha_i (div u_, q_i) (:199), is added by the caller."""
+    A = len(c)
+    B = np.zeros((A, A))
+    for i in range(A):
+        for j in range(A):
+            if i == j:
+                B[i, j] = -float(c[i]) + dt * (1 - theta) * sum(_s(S, i, k)
+                                                                 for k in range(A) if k != i)
+            else:
+                B[i, j] = -dt * (1 - theta) * _s(S, i, j)
+    return B, np.full(A, dt * (1 - theta))
+
+
+def two_field_preconditioner_coefficients(c, S, dt, theta):
+    """Diagonal blocks of the two-field prec (:269-271): u mu grad:grad;
+    p_i (c_i + dt theta sum_{j != i} S_ij) mass + dt theta K_i stiffness.
+    The reference writes the stiffness weight as ``dt*theta**inner(...)``
+    (theta raised to the power of the integrand, :270), evidently meaning
+    dt*theta*inner(...); the product form is used here (its iterative path,
+    the only user of prec, is switched off in the reference, :308-314)."""
+    A = len(c)
+    diag = np.array([float(c[i]) + sum(dt * theta * _s(S, i, j) for j in range(A) if j != i)
+                     for i in range(A)])
+    return diag, np.full(A, dt * theta)
+
+
 def dirichlet_nodes(space, markers, field):
     """Nodes in the closure of exterior facets marked 0 (DOLFIN's
     topological DirichletBC): P2 nodes (3 vertices + 3 edges) for field 0,
@@ -109,6 +162,8 @@ def dirichlet_nodes(space, markers, field):
 
 
 def field_dofs(space, field, nodes):
+    """DOFs of sub-space `field` (0 = u, 1.. = the pressure fields in the
+    space's order) at the given nodes."""
     if field == 0:
         return (3 * nodes[:, None] + np.arange(3)).reshape(-1)
-    return space.nu + (space.A + 1) * nodes + (field - 1)
+    return space.nu + space.F * nodes + (field - 1)
