@@ -210,6 +210,28 @@ int mpet_expand_csr(const MpetPatterns* pat, const MpetPrimitives* prim, int nP2
                     int A, const MpetCoef* coef, int which, const long long* rowptr,
                     int* cols, double* vals, hipStream_t stream);
 
+/* Two-field system of MPETSolver (mpet/mpetsolver.py:196-201; fields u,
+ * p_1..p_A, F = A pressure fields per P1 node): coefficients of the
+ * pressure rows (C row-major, stride MPET_MAXF, times the P1 mass; stiff[f]
+ * times St_f), the Biot-Willis coefficients alpha_f and the Lame constants. */
+typedef struct {
+  double C[MPET_MAXF * MPET_MAXF];
+  double stiff[MPET_MAXF];
+  double alpha[MPET_MAXF];
+  double mu, lmbda;
+} MpetTwoFieldCoef;
+
+/* Scalar FP64 CSR (int64 rowptr computed by the caller) of the two-field
+ * system from the assembled primitives (Kuu, Lap, Dv, Dt, M1, St):
+ * which = 0 the matrix of lhs(F) + Robin (`assemble(a)` + `a_robin`,
+ * :334-338, :411-415), 1 the preconditioner prec (:269-271), 2 the matrix of
+ * the up_-linear part of rhs(F) (`assemble(L)`, :356, :433; pressure rows
+ * only).  The u-u block 2 mu eps:eps + lambda div div is formed from Kuu and
+ * Lap (no new element integrals). */
+int mpet_tf_expand_csr(const MpetPatterns* pat, const MpetPrimitives* prim, int nP2, int nP1,
+                       int A, const MpetTwoFieldCoef* coef, int which, const long long* rowptr,
+                       int* cols, double* vals, hipStream_t stream);
+
 /* Pressure rows of b = rhs(F) at t_n (linear in up_): replaces
  * `b = assemble(L)` (:461); zeroes the u rows.  coef.C = rhs coupling,
  * coef.stiff[f] = dt*(1-theta). */
@@ -303,6 +325,15 @@ void mpet_amg_destroy(void* h);
  * Borrows every device array of the descriptors (caller keeps them alive);
  * owns its workspaces.  op32/prec32 are needed for mode 1, prec64 for
  * mode 0, prec32 for mode 2. */
+/* FP64 MINRES on a scalar CSR operator only (no structured operator): the
+ * two-field MPETSolver system, F pressure fields per P1 node, replacing the
+ * LU solve of step / solve_direct (mpet/mpetsolver.py:347-379, :422-456)
+ * with the block preconditioner of prec.  opt->mode must be 0; dmask marks
+ * the Dirichlet DOFs (n = 3 nP2 + F nP1 entries). */
+int mpet_solver_create_csr(const MpetCsrDesc* csr64, const uint8_t* dmask, int nP2, int nP1,
+                           int F, const MpetPrecDesc* prec64, const MpetSolverOptions* opt,
+                           void** handle);
+
 int mpet_solver_create(const MpetOpDesc* op64, const MpetOpDesc* op32,
                        const MpetCsrDesc* csr64, const MpetPrecDesc* prec64,
                        const MpetPrecDesc* prec32, const MpetSolverOptions* opt,

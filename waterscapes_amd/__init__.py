@@ -1,5 +1,5 @@
 """waterscapes_amd — MI355X-native backend of the total-pressure MPET time
-step of meg-simula/waterscapes (`mpet`).
+step of meg-simula/waterscapes (`mpet`), and of the two-field MPETSolver.
 
 ``from waterscapes_amd import *`` exposes the same names the reference's
 ``from mpet import *`` does for this path (mpet/__init__.py:10-17), plus the
@@ -12,11 +12,13 @@ from .fem import (Constant, Expression, Function, FacetNormal, MixedSpace, inter
                   errornorm)
 from .problem import MPETProblem, convert_to_mu_lmbda, convert_to_E_nu
 from .solver import MPETTotalPressureSolver, DirichletBC, Parameters
+from .mpetsolver import MPETSolver
 from .krylov import SolverConfig
 from .amg import AmgParams
 
 __all__ = ["Mesh", "BoxMesh", "UnitCubeMesh", "MeshFunction", "FacetFunction", "SubDomain",
            "CompiledSubDomain", "DOLFIN_EPS", "INVALID", "Constant", "Expression", "Function",
            "FacetNormal", "MixedSpace", "interpolate", "assign", "errornorm", "MPETProblem",
-           "convert_to_mu_lmbda", "convert_to_E_nu", "MPETTotalPressureSolver", "DirichletBC",
+           "convert_to_mu_lmbda", "convert_to_E_nu", "MPETTotalPressureSolver", "MPETSolver",
+           "DirichletBC",
            "Parameters", "SolverConfig", "AmgParams"]

@@ -50,6 +50,23 @@ class OpDesc(C.Structure):
                 ("dmask", C.c_void_p), ("lanes_u", C.c_int32), ("lanes_p", C.c_int32)]
 
 
+class TwoFieldCoef(C.Structure):
+    _fields_ = [("C", C.c_double * (MAXF * MAXF)), ("stiff", C.c_double * MAXF),
+                ("alpha", C.c_double * MAXF), ("mu", C.c_double), ("lmbda", C.c_double)]
+
+
+def make_two_field_coef(Cm, stiff, alpha, mu, lmbda):
+    c = TwoFieldCoef()
+    F = Cm.shape[0]
+    for f in range(F):
+        for g in range(F):
+            c.C[f * MAXF + g] = float(Cm[f, g])
+        c.stiff[f] = float(stiff[f])
+        c.alpha[f] = float(alpha[f])
+    c.mu, c.lmbda = float(mu), float(lmbda)
+    return c
+
+
 def make_coef(Cm, stiff, mu=0.0):
     c = Coef()
     F = Cm.shape[0]
@@ -86,6 +103,8 @@ def lib():
         "mpet_upload_tables": ([P, C.c_int], C.c_int),
         "mpet_assemble_primitives": ([P, P, P, C.c_int, C.c_double, P, P], C.c_int),
         "mpet_expand_csr": ([P, P, C.c_int, C.c_int, C.c_int, P, C.c_int, P, P, P, P], C.c_int),
+        "mpet_tf_expand_csr": ([P, P, C.c_int, C.c_int, C.c_int, P, C.c_int, P, P, P, P],
+                               C.c_int),
         "mpet_rhs_pressure": ([P, P, C.c_int, C.c_int, C.c_int, P, P, P, P], C.c_int),
         "mpet_cell_source": ([P, P, C.c_int, P, C.c_int, P, C.c_int, C.c_double,
                               C.c_longlong, C.c_int, C.c_int, P, P], C.c_int),

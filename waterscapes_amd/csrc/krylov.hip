@@ -560,6 +560,39 @@ static int bnorm_M(MpetSolver* S, Minres<T>& M, const Precond<T>& P, const doubl
 template <typename T>
 MpetOpT<T> mpet_make_op(const MpetOpDesc* d);
 
+static int finish_create(MpetSolver* S, const MpetPrecDesc* prec64, const MpetPrecDesc* prec32,
+                         void** handle);
+
+extern "C" int mpet_solver_create_csr(const MpetCsrDesc* csr64, const uint8_t* dmask, int nP2,
+                                      int nP1, int F, const MpetPrecDesc* prec64,
+                                      const MpetSolverOptions* opt, void** handle) {
+  if (!csr64 || !csr64->rp || !csr64->ci || !csr64->v || !dmask || !prec64 || !opt ||
+      !handle || F < 1 || F > MPET_MAXF || nP2 < 1 || nP1 < 1 ||
+      csr64->nrows != 3ll * nP2 + (long long)F * nP1) {
+    mpet_set_error("mpet_solver_create_csr: invalid argument");
+    return ERR_INVALID;
+  }
+  if (opt->mode != 0) {
+    mpet_set_error("mpet_solver_create_csr: only FP64 MINRES (mode 0) runs on a CSR operator alone");
+    return ERR_INVALID;
+  }
+  MpetSolver* S = new MpetSolver();
+  S->opt = *opt;
+  S->opt.use_csr = 1;
+  S->F = F;
+  S->nu = 3ll * nP2;
+  S->n = S->nu + (long long)F * nP1;
+  S->op64.nP2 = nP2;
+  S->op64.nP1 = nP1;
+  S->opt_mask64 = dmask;
+  S->csr_rows = csr64->nrows;
+  S->csr_rp = csr64->rp;
+  S->csr_ci = csr64->ci;
+  S->csr_v = csr64->v;
+  S->csr_lpr = csr64->lpr ? csr64->lpr : 16;
+  return finish_create(S, prec64, nullptr, handle);
+}
+
 extern "C" int mpet_solver_create(const MpetOpDesc* op64, const MpetOpDesc* op32,
                                   const MpetCsrDesc* csr64, const MpetPrecDesc* prec64,
                                   const MpetPrecDesc* prec32, const MpetSolverOptions* opt,
@@ -600,6 +633,13 @@ extern "C" int mpet_solver_create(const MpetOpDesc* op64, const MpetOpDesc* op32
     S->csr_v = csr64->v;
     S->csr_lpr = csr64->lpr ? csr64->lpr : 16;
   }
+  return finish_create(S, prec64, prec32, handle);
+}
+
+// Preconditioners, MINRES workspaces and streams of a solver whose sizes,
+// operators and options are set; frees S on failure.
+static int finish_create(MpetSolver* S, const MpetPrecDesc* prec64, const MpetPrecDesc* prec32,
+                         void** handle) {
   int rc = OK;
   if (prec64 && (rc = build_precond(S->allocs, S->nu, prec64, S->pc64))) { mpet_solver_destroy(S); return rc; }
   if (prec32 && (rc = build_precond(S->allocs, S->nu, prec32, S->pc32))) { mpet_solver_destroy(S); return rc; }

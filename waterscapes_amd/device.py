@@ -124,6 +124,10 @@ class DeviceSystem:
         self.primK = _lib.Primitives(ptr(t["Kuu"]), ptr(t["Lap"]), ptr(t["Dv"]), ptr(t["Dt"]),
                                      ptr(t["M1"]), ptr(t["StK"]))
 
+    def st_row(self, f):
+        """Row of St (network) carrying the stiffness of pressure field f."""
+        return f - 1 if f >= 1 else None
+
     def _structured(self, dtype):
         """SoA copy of the primitives in float64 (dtype 0) or float32 (1)."""
         tdt = torch.float64 if dtype == 0 else torch.float32
@@ -222,4 +226,83 @@ class DeviceSystem:
         check(_lib.lib().mpet_rhs_pressure(C.byref(self.pat), C.byref(self.primK), self.nP2,
                                            self.nP1, self.A, C.byref(coef), ptr(up_prev),
                                            ptr(b), _lib.stream_ptr()))
+        return b
+
+
+class TwoFieldSystem(DeviceSystem):
+    """Assembled two-field MPET system (MPETSolver, mpet/mpetsolver.py:196-201,
+    fields u, p_1..p_A) of one instance on one GPU, as scalar FP64 CSR
+    matrices built on the device from the same primitives as the
+    total-pressure system (`mpet_tf_expand_csr`)."""
+
+    two_field = True
+
+    def __init__(self, space: MixedSpace, params, dt, theta, K_cells=None, device="cuda",
+                 robin=None):
+        if space.total_pressure:
+            raise ValueError("TwoFieldSystem needs a space without total pressure")
+        super().__init__(space, params, dt, theta, K_cells=K_cells, device=device,
+                         build_csr=False, need_fp32=False, structured=False, robin=robin)
+        A = space.A
+        self.F = A
+        c, S = params["c"], params["S"]
+        self.alpha = np.array([float(a) for a in params["alpha"]])
+        self.Cm, self.Cs = forms.two_field_matrix_coefficients(c, S, dt, theta)
+        self.Bm, self.Bs = forms.two_field_rhs_coefficients(c, S, dt, theta)
+        self.Pd, self.Ps = forms.two_field_preconditioner_coefficients(c, S, dt, theta)
+        self.csr = self.expand_csr(0)
+        self.Bcsr = self.expand_csr(2)
+
+    def st_row(self, f):
+        return f
+
+    def op_desc(self, dtype, masked=True, lanes=None):
+        raise NotImplementedError("the two-field system is held as scalar CSR only")
+
+    def row_lengths(self, which):
+        rp = {k: np.diff(v[0]) for k, v in self.pat_host.items()}
+        F = self.F
+        if which == 0:
+            lu = 3 * rp["22"] + F * rp["21"]
+        elif which == 1:
+            lu = 3 * rp["22"]
+        else:
+            lu = np.zeros(self.nP2, dtype=np.int64)
+        lens_u = np.repeat(lu, 3)
+        lp = np.empty((self.nP1, F), dtype=np.int64)
+        lp[:, :] = (rp["11"] if which == 1 else 3 * rp["12"] + F * rp["11"])[:, None]
+        return np.concatenate([lens_u, lp.reshape(-1)])
+
+    def expand_csr(self, which):
+        """Scalar CSR (rowptr int64, cols int32, vals f64): which = 0 the
+        matrix of lhs(F) with Robin, 1 the preconditioner prec, 2 the matrix
+        of the up_-linear part of rhs(F)."""
+        lens = self.row_lengths(which)
+        rowptr = np.zeros(self.n + 1, dtype=np.int64)
+        np.cumsum(lens, out=rowptr[1:])
+        nnz = int(rowptr[-1])
+        if nnz >= 2**31:
+            raise ValueError("scalar CSR too large for int32 column indices")
+        dev = self.device
+        rp = _dev(rowptr, torch.int64, dev)
+        cols = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)
+        vals = torch.empty(max(nnz, 1), dtype=torch.float64, device=dev)
+        if which == 0:
+            Cm, st, prim = self.Cm, self.Cs, self.prim
+        elif which == 1:
+            Cm, st, prim = np.diag(self.Pd), self.Ps, self.primK
+        else:
+            Cm, st, prim = self.Bm, self.Bs, self.primK
+        coef = _lib.make_two_field_coef(Cm, st, self.alpha, self.mu, self.lmbda)
+        check(_lib.lib().mpet_tf_expand_csr(C.byref(self.pat), C.byref(prim), self.nP2,
+                                            self.nP1, self.A, C.byref(coef), which, ptr(rp),
+                                            ptr(cols), ptr(vals), _lib.stream_ptr()))
+        return rp, cols, vals
+
+    def rhs_pressure(self, up_prev, b):
+        """b = (matrix of the up_-linear part of rhs(F)) up_ — `assemble(L)`
+        (:356, :433); the u rows are 0."""
+        rp, ci, vals = self.Bcsr
+        check(_lib.lib().mpet_csr_spmv(0, self.n, ptr(rp), ptr(ci), ptr(vals), ptr(up_prev),
+                                       None, None, ptr(b), 16, _lib.stream_ptr()))
         return b

@@ -355,6 +355,11 @@ class MixedSpace:
     def num_sub_spaces(self):
         return self.F + 1
 
+    def network_sub(self, i):
+        """Sub-space index of network pressure p_{i+1} (i 0-based): i + 2 with
+        a total pressure, i + 1 without."""
+        return i + 1 + (self.F - self.A)
+
     def field_index(self, i):
         """Global DOF indices of field i (0 = u (nP2*3, node-major); then the
         pressure fields in order: p_0, p_1..p_A, or p_1..p_A)."""

@@ -66,6 +66,9 @@ def _bind():
     P = C.c_void_p
     L.mpet_solver_create.argtypes = [P, P, P, P, P, P, C.POINTER(C.c_void_p)]
     L.mpet_solver_create.restype = C.c_int
+    L.mpet_solver_create_csr.argtypes = [P, P, C.c_int, C.c_int, C.c_int, P, P,
+                                         C.POINTER(C.c_void_p)]
+    L.mpet_solver_create_csr.restype = C.c_int
     L.mpet_solver_destroy.argtypes = [P]
     L.mpet_solver_destroy.restype = C.c_int
     L.mpet_solver_solve.argtypes = [P, P, P, P, P]
@@ -191,10 +194,13 @@ class KrylovSolver:
         pmask[pd] = 1
         self.pmask = torch.as_tensor(pmask).to(dev)
         self.Pd = np.array(system.Pd, dtype=np.float64)
-        if cfg.p0_weight == "robust":
-            self.Pd[0] = 0.5 / system.mu + 1.0 / system.lmbda
-        elif cfg.p0_weight != "reference":
+        two_field = getattr(system, "two_field", False)
+        if two_field and cfg.mode != "fp64":
+            raise ValueError("the two-field system is solved in FP64 (mode 'fp64') only")
+        if cfg.p0_weight not in ("robust", "reference"):
             raise ValueError("p0_weight must be 'robust' or 'reference'")
+        if cfg.p0_weight == "robust" and not two_field:
+            self.Pd[0] = 0.5 / system.mu + 1.0 / system.lmbda
         self._build_hierarchy()
         self._build_pressure()
         self.handle = None
@@ -205,6 +211,8 @@ class KrylovSolver:
         self.prec64 = self._prec_desc(0) if not pc32 else None
         self.prec32 = self._prec_desc(1) if pc32 else None
         use_csr = cfg.use_csr if cfg.use_csr is not None else (cfg.mode == "fp64")
+        if two_field:
+            use_csr = True
         self.csr_desc = None
         if use_csr and cfg.mode == "fp64":
             if system.csr is None:
@@ -218,9 +226,17 @@ class KrylovSolver:
         self.opt = SolverOptions(mode_id, 1 if self.csr_desc is not None else 0,
                                  cfg.rtol, cfg.maxit, check_every, 1 if graphs else 0,
                                  cfg.outer_tol, cfg.inner_rtol, cfg.max_outer)
+        h = C.c_void_p()
+        if two_field:
+            self.op64 = self.op32 = None
+            check(L.mpet_solver_create_csr(C.byref(self.csr_desc), ptr(system.dmask),
+                                           system.nP2, system.nP1, system.F,
+                                           C.byref(self.prec64), C.byref(self.opt),
+                                           C.byref(h)))
+            self.handle = h
+            return
         self.op64 = system.op_desc(0)
         self.op32 = system.op_desc(1) if mixed else None
-        h = C.c_void_p()
         check(L.mpet_solver_create(C.byref(self.op64),
                                    C.byref(self.op32) if self.op32 is not None else None,
                                    C.byref(self.csr_desc) if self.csr_desc is not None else None,
@@ -288,8 +304,9 @@ class KrylovSolver:
 
     def _build_pressure(self):
         """Hierarchy of the block-diagonal pressure preconditioner
-        diag(P_0, .., P_A), P_f = Pd_f M1 + Ps_f St_f (Dirichlet rows and
-        columns eliminated), in field-major order."""
+        diag(P_0, .., P_A) (two-field: diag(P_1, .., P_A)), P_f = Pd_f M1 +
+        Ps_f St_f (Dirichlet rows and columns eliminated), in field-major
+        order."""
         s = self.sys
         rp, ci = s.pat_host["11"]
         M1 = s.prim_t["M1"].cpu().numpy()
@@ -301,8 +318,9 @@ class KrylovSolver:
         blocks = []
         for f in range(F):
             Pf = self.Pd[f] * Mm
-            if f >= 1:
-                Pf = Pf + s.Ps[f] * sp.csr_matrix((St[f - 1], ci, rp), shape=(n1, n1))
+            k = s.st_row(f)
+            if k is not None:
+                Pf = Pf + s.Ps[f] * sp.csr_matrix((St[k], ci, rp), shape=(n1, n1))
             free = ~pmask[:, f]
             keep = sp.diags(free.astype(np.float64))
             blocks.append((keep @ Pf @ keep + sp.diags((~free).astype(np.float64))).tocsr())

@@ -30,7 +30,7 @@ import numpy as np
 import torch
 
 from . import forms, refel
-from .device import DeviceSystem
+from .device import DeviceSystem, TwoFieldSystem
 from .facets import FacetSet, FacetTerm
 from .fem import Constant, Function, MixedSpace
 from .amg import AmgParams
@@ -156,14 +156,24 @@ class StepEngine:
                 robin_lhs.append((i, fs, problem.beta[i], None))
         # batched: the batch packs the operator and hierarchies of all its
         # instances, so no per-instance structured copies or solver handle
-        self.system = DeviceSystem(space, params, dt, theta, K_cells=cellK, device=device,
-                                   build_csr=(not batched and config.mode == "fp64"
-                                              and config.use_csr is not False),
-                                   structured=not batched, robin=robin_lhs)
-        # Dirichlet conditions (create_dirichlet_bcs, :123-144)
+        if space.total_pressure:
+            self.system = DeviceSystem(space, params, dt, theta, K_cells=cellK, device=device,
+                                       build_csr=(not batched and config.mode == "fp64"
+                                                  and config.use_csr is not False),
+                                       structured=not batched, robin=robin_lhs)
+        else:
+            if batched:
+                raise NotImplementedError("batches of two-field instances")
+            self.system = TwoFieldSystem(space, params, dt, theta, K_cells=cellK,
+                                         device=device, robin=robin_lhs)
+        # the two-field solver (mpet/mpetsolver.py:362-374, :428-448) sets
+        # t_{n+1} = t_n + dt directly; the total-pressure one t_theta + (1-theta) dt
+        self.t1_from_t0 = not space.total_pressure
+        # Dirichlet conditions (create_dirichlet_bcs, :123-144; two-field
+        # mpetsolver.py:65-83, networks on sub(i+1))
         self.bcs = [DirichletBC(space, 0, problem.u_bar, problem.momentum_boundary_markers, 0)]
         for i in range(A):
-            self.bcs.append(DirichletBC(space, i + 2, problem.p_bar[i],
+            self.bcs.append(DirichletBC(space, space.network_sub(i), problem.p_bar[i],
                                         problem.continuity_boundary_markers[i], 0))
         self.bc_dofs_host = np.concatenate([bc.dofs for bc in self.bcs]).astype(np.int64)
         self.bc_dofs = torch.as_tensor(self.bc_dofs_host).to(self.device)
@@ -171,7 +181,7 @@ class StepEngine:
                                    create_handle=not batched)
         self.config = config
         # sources
-        nu, F = space.nu, space.A + 1
+        nu, F = space.nu, space.F
         self.f_src = None
         if not problem.f.is_zero():
             self.f_src = _Source(self, problem.f, 2, 3, 1.0, 0, 3, 0)
@@ -179,7 +189,8 @@ class StepEngine:
         for i in range(A):
             gi = problem.g[i]
             if not gi.is_zero():
-                self.g_src.append(_Source(self, gi, 1, 1, self.dt, nu, F, i + 1))
+                self.g_src.append(_Source(self, gi, 1, 1, self.dt, nu, F,
+                                          space.network_sub(i) - 1))
         # facet sources: traction s on ds_m(1) (P2 test, 3 components), fluxes
         # I_i on ds_i(1) and Robin data on ds_i(2) (P1 test, network i rows)
         self.s_term = None
@@ -194,11 +205,12 @@ class StepEngine:
             fs = FacetSet(space, problem.continuity_boundary_markers[i], NEUMANN_MARKER,
                           self.device)
             if fs.n:
-                self.I_terms.append((i, FacetTerm(fs, 1, nu, F, i + 1, 1)))
+                self.I_terms.append((i, FacetTerm(fs, 1, nu, F, space.network_sub(i) - 1, 1)))
         self.robin_terms = []
         for i, fs in self.robin:
-            gdof = torch.as_tensor((nu + F * fs.p1 + (i + 1)).astype(np.int32)).to(self.device)
-            self.robin_terms.append((i, FacetTerm(fs, 1, nu, F, i + 1, 1), gdof))
+            fi = space.network_sub(i) - 1
+            gdof = torch.as_tensor((nu + F * fs.p1 + fi).astype(np.int32)).to(self.device)
+            self.robin_terms.append((i, FacetTerm(fs, 1, nu, F, fi, 1), gdof))
         self.b = torch.empty(space.ndofs, dtype=torch.float64, device=self.device)
         self.stats = []
 
@@ -225,7 +237,7 @@ class StepEngine:
                 term.add_robin_prev(b, prob.beta[i], up_prev, gdof, dt * (1.0 - theta))
             if not _is_zero(prob.p_robin[i]):
                 term.add_product(b, prob.p_robin[i], prob.beta[i], -dt)
-        t_np1 = float(time) + (1.0 - theta) * dt
+        t_np1 = float(t_n) + dt if self.t1_from_t0 else float(time) + (1.0 - theta) * dt
         time.assign(t_np1)
         if self.f_src is not None:                  # f . v dx               (:313)
             self.f_src.add(b, None)
