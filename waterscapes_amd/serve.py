@@ -9,13 +9,16 @@ resident until the request completes.  At every scheduling tick it
   1. admits waiting requests in arrival order while their estimated HBM
      footprint fits the budget (a request larger than the whole budget is
      refused with MPET_ERR_BUDGET);
-  2. re-packs the batch if the set of instances changed (no assembly or AMG
+  2. re-packs the batch when requests were admitted (no assembly or AMG
      setup is repeated: each instance's device state is reused);
-  3. runs one batched step of every admitted instance (requests that
+  3. runs one batched step of every unfinished instance (requests that
      arrived while others were running join mid-flight: their steps overlap
-     on the device with the others');
-  4. retires finished requests (final state copied to the host) and frees
-     their memory.
+     on the device with the others'); finished instances stay packed but
+     inactive, so a retirement costs no re-pack;
+  4. retires finished requests (final state copied to the host); their
+     memory is released when the batch is next re-packed — at the next
+     admission, when half of the packed instances have finished, or when a
+     waiting request needs the memory.
 
 The reference has no serving layer (it steps one problem per
 MPETTotalPressureSolver, mpet/mpettotalpressuresolver.py:456-502); each
@@ -98,7 +101,8 @@ class MPETServer:
         self.finished = []
         self.used = 0
         self.batch = None
-        self._batch_ids = None
+        self.batch_reqs = []       # requests packed in self.batch, in order
+        self.pending_free = 0      # footprint of retired requests still packed
         self.t0 = clock()
         self.steps_run = 0
         self.ticks = 0
@@ -126,6 +130,8 @@ class MPETServer:
                 req.done = now
                 self.finished.append(req)
                 continue
+            if self.used + req.footprint > self.budget and self.pending_free:
+                self._drop_batch()         # release retired instances first
             if self.used + req.footprint > self.budget:
                 break                      # wait until memory frees up
             self.waiting.pop(0)
@@ -138,14 +144,22 @@ class MPETServer:
             changed = True
         return changed
 
-    def _repack(self):
+    def _drop_batch(self):
+        """Close the batch and release the memory of retired instances."""
         if self.batch is not None:
             self.batch.close()
             self.batch = None
+        self.batch_reqs = []
+        self.used -= self.pending_free
+        self.pending_free = 0
+        gc.collect()
+
+    def _repack(self):
+        self._drop_batch()
         if self.active:
             self.batch = MPETBatch([r.solver for r in self.active], precision=self.precision,
                                    **self.tol)
-        self._batch_ids = [id(r) for r in self.active]
+            self.batch_reqs = list(self.active)
 
     def _retire(self, now):
         keep = []
@@ -153,19 +167,16 @@ class MPETServer:
             if r.steps_done >= r.steps:
                 r.result = r.solver.up.vector().get_local().copy()
                 r.done = now
-                self.used -= r.footprint
-                r.solver._engine = None
+                self.pending_free += r.footprint
                 r.solver = None
                 self.finished.append(r)
             else:
                 keep.append(r)
         changed = len(keep) != len(self.active)
         self.active = keep
-        if changed:
-            if self.batch is not None:
-                self.batch.close()
-                self.batch = None
-            gc.collect()
+        if changed and (not keep or 2 * (len(self.batch_reqs) - len(keep))
+                        >= len(self.batch_reqs)):
+            self._drop_batch()
         return changed
 
     # -- scheduling ---------------------------------------------------------------
@@ -176,19 +187,23 @@ class MPETServer:
         changed = self._admit(now)
         if not self.active:
             return 0
-        if changed or self.batch is None or self._batch_ids != [id(r) for r in self.active]:
+        if changed or self.batch is None:
             self._repack()
         b = self.batch
-        stats = b.step()
-        b.commit()
-        for k, r in enumerate(self.active):
-            r.steps_done += 1
-            r.niter.append(int(stats[k].inner_iterations))
-        self.steps_run += len(self.active)
+        act = np.array([1 if r.steps_done < r.steps else 0 for r in self.batch_reqs],
+                       dtype=np.int32)
+        stats = b.step(act)
+        b.commit(None if act.all() else act)
+        for k, r in enumerate(self.batch_reqs):
+            if act[k]:
+                r.steps_done += 1
+                r.niter.append(int(stats[k].inner_iterations))
+        stepped = int(act.sum())
+        self.steps_run += stepped
         self.ticks += 1
         torch.cuda.synchronize()
         self._retire(self.clock() - self.t0)
-        return len(stats)
+        return stepped
 
     def run(self, until_idle=True, idle_sleep=1e-3):
         """Tick until every submitted request has finished."""
@@ -201,9 +216,7 @@ class MPETServer:
                     # the head request does not fit although nothing runs:
                     # only possible when it exceeds the budget (refused above)
                     raise BudgetError("request cannot be admitted")
-        if self.batch is not None:
-            self.batch.close()
-            self.batch = None
+        self._drop_batch()
         return self.finished
 
     def latencies(self):
