@@ -678,10 +678,15 @@ class OracleProblem:
                  momentum_markers=None, continuity_markers=None,
                  s=None, s_degree=3, I=None, I_degree=3, beta=None,
                  p_robin=None, robin_degree=1, p_robin_degree=None,
-                 formulation="total_pressure"):
+                 formulation="total_pressure", permc_spec="COLAMD"):
         if formulation not in ("total_pressure", "two_field"):
             raise ValueError(formulation)
         self.two_field = formulation == "two_field"
+        # column ordering of SuperLU; COLAMD factorises these row-replaced
+        # systems fastest (8^3 all-Dirichlet MMS: 8.6 s vs 133 s with MMD on
+        # A + A^T; benchmark workloads: 12 s vs 16 s total-pressure, 9 s vs
+        # 51 s two-field)
+        self.permc_spec = permc_spec
         self.m = mesh
         self.params = params
         self.A = int(params["J"])
@@ -891,9 +896,7 @@ class OracleProblem:
         A = self.A_mat.copy()
         alld = np.concatenate(self.bc_dofs)
         A = apply_row_replace(A, alld)
-        # COLAMD: ~15x faster factorisation than MMD on A + A^T for these
-        # row-replaced systems (8^3: 8.6 s vs 133 s), a quarter of the fill
-        lu = spla.splu(A.tocsc(), permc_spec="COLAMD")
+        lu = spla.splu(A.tocsc(), permc_spec=self.permc_spec)
         up_prev = up0.copy()
         t = self.t0
         while t < self.T - 1e-9:
