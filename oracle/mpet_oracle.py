@@ -904,6 +904,11 @@ class OracleProblem:
             for d, v in zip(self.bc_dofs, self.bc_values(t_np1)):
                 b[d] = v
             up = lu.solve(b)
+            # two steps of iterative refinement in FP64: the two-field systems
+            # with tiny K are ill-conditioned enough that the LU solution's
+            # weakest pressure field moves by ~1e-8 with the column ordering
+            for _ in range(2):
+                up += lu.solve(b - A @ up)
             yield up, t_np1
             up_prev = up
             t = t_np1
