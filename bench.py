@@ -148,7 +148,7 @@ def cpu_baseline(n_sample, A, steps, budget_s=30.0):
     t0 = time.perf_counter()
     prob = O.OracleProblem(m, params, dict(dt=0.0125, theta=1.0, T=1e9), K_cells=Kc,
                            u_bar=lambda X, t: np.zeros((X.shape[0], 3)), p_bar=pb,
-                           momentum_markers=mm, continuity_markers=[cm] * A)
+                           momentum_markers=mm, continuity_markers=[cm] * A, refine_steps=0)
     gen = prob.solve_direct(np.zeros(space.ndofs))
     next(gen)                                      # factorisation + first step
     setup = time.perf_counter() - t0
@@ -370,7 +370,7 @@ def two_field_cpu_baseline(n_sample, steps, budget_s=20.0):
     prob = O.OracleProblem(m, TF_PARAMS, dict(dt=0.0125, theta=1.0, T=1e9),
                            u_bar=lambda X, t: np.zeros((X.shape[0], 3)), p_bar=pb,
                            momentum_markers=mm, continuity_markers=[cm] * 2,
-                           formulation="two_field")
+                           formulation="two_field", refine_steps=0)
     gen = prob.solve_direct(np.zeros(space.ndofs))
     next(gen)
     setup = time.perf_counter() - t0

@@ -678,7 +678,7 @@ class OracleProblem:
                  momentum_markers=None, continuity_markers=None,
                  s=None, s_degree=3, I=None, I_degree=3, beta=None,
                  p_robin=None, robin_degree=1, p_robin_degree=None,
-                 formulation="total_pressure", permc_spec="COLAMD"):
+                 formulation="total_pressure", permc_spec="COLAMD", refine_steps=2):
         if formulation not in ("total_pressure", "two_field"):
             raise ValueError(formulation)
         self.two_field = formulation == "two_field"
@@ -687,6 +687,7 @@ class OracleProblem:
         # A + A^T; benchmark workloads: 12 s vs 16 s total-pressure, 9 s vs
         # 51 s two-field)
         self.permc_spec = permc_spec
+        self.refine_steps = int(refine_steps)
         self.m = mesh
         self.params = params
         self.A = int(params["J"])
@@ -904,10 +905,12 @@ class OracleProblem:
             for d, v in zip(self.bc_dofs, self.bc_values(t_np1)):
                 b[d] = v
             up = lu.solve(b)
-            # two steps of iterative refinement in FP64: the two-field systems
-            # with tiny K are ill-conditioned enough that the LU solution's
-            # weakest pressure field moves by ~1e-8 with the column ordering
-            for _ in range(2):
+            # iterative refinement in FP64 (two steps by default): the
+            # two-field systems with tiny K are ill-conditioned enough that the
+            # unrefined LU solution's weakest pressure field moves by ~1e-8
+            # with the column ordering; the timed CPU baseline (one LU solve
+            # per step, as the reference) uses none
+            for _ in range(self.refine_steps):
                 up += lu.solve(b - A @ up)
             yield up, t_np1
             up_prev = up
