@@ -100,6 +100,9 @@ typedef struct {
   MpetCoef coef;
   const uint8_t* dmask;
   int32_t lanes_u, lanes_p;   /* lanes per P2 / P1 node row: 4/8/16/32, 0 = 16 */
+  int32_t xcd_chunk;          /* >0: runs of this many consecutive row tiles per
+                                 XCD (the 8 XCDs take workgroups round-robin),
+                                 0: hardware order */
 } MpetOpDesc;
 
 /* One level of the displacement-block hierarchy (device arrays, int32

@@ -47,7 +47,8 @@ class OpDesc(C.Structure):
     _fields_ = [("nP2", C.c_int32), ("nP1", C.c_int32), ("A", C.c_int32), ("pat", Patterns),
                 ("dtype", C.c_int32), ("Kuu", C.c_void_p), ("Dv", C.c_void_p),
                 ("Dt", C.c_void_p), ("M1", C.c_void_p), ("St", C.c_void_p), ("coef", Coef),
-                ("dmask", C.c_void_p), ("lanes_u", C.c_int32), ("lanes_p", C.c_int32)]
+                ("dmask", C.c_void_p), ("lanes_u", C.c_int32), ("lanes_p", C.c_int32),
+                ("xcd_chunk", C.c_int32)]
 
 
 class TwoFieldCoef(C.Structure):

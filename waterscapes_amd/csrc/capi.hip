@@ -21,6 +21,7 @@ static MpetOpT<T> make_op(const MpetOpDesc* d) {
   op.dmask = d->dmask;
   op.lanes_u = d->lanes_u;
   op.lanes_p = d->lanes_p;
+  op.xcd_chunk = d->xcd_chunk;
   return op;
 }
 

@@ -86,7 +86,8 @@ k1_op_kernel(MpetOpT<T> op, const T* __restrict__ z, EpiK1<T> epi, const MinresS
   const int ntu = (op.nP2 + K1_ROWS - 1) / K1_ROWS, ntp = (op.nP1 + K1_ROWS - 1) / K1_ROWS;
   const long long n22 = op.nnz22, n21 = op.nnz21, n12 = op.nnz12, n11 = op.nnz11;
   double acc = 0.0;
-  for (int tile = blockIdx.x; tile < ntu + ntp; tile += gridDim.x) {
+  for (int t = blockIdx.x; t < ntu + ntp; t += gridDim.x) {
+    const int tile = xcd_tile(t, ntu + ntp, op.xcd_chunk);
     if (tile < ntu) {
       const int a = tile * K1_ROWS + threadIdx.x / K1_G;
       if (a < op.nP2) {

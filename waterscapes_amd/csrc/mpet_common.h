@@ -46,6 +46,22 @@ __device__ __forceinline__ T group_sum(T v) {
   return v;
 }
 
+// Workgroups go to the 8 XCDs round-robin (workgroup w on XCD w % 8), each
+// XCD with its own L2.  Map hardware tile t of n to a logical tile so that
+// every XCD works through runs of `chunk` consecutive logical tiles (rows
+// that share gathered vector entries), the runs dealt round-robin over the
+// XCDs; tiles past the last full round keep their index.  A bijection on
+// [0, n); chunk <= 0 is the identity.
+__device__ __forceinline__ int xcd_tile(int t, int n, int chunk) {
+  if (chunk <= 0) return t;
+  const int round = 8 * chunk;
+  const int full = (n / round) * round;
+  if (t >= full) return t;
+  const int x = t % 8, j = t / 8;           // j-th tile of XCD x
+  const int q = j / chunk, r = j % chunk;   // run q, position r
+  return (q * 8 + x) * chunk + r;
+}
+
 // Lower bound of `key` in sorted cols[lo, hi).
 __device__ __forceinline__ int lower_bound_i32(const int* __restrict__ cols, int lo, int hi,
                                                int key) {

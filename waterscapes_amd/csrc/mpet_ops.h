@@ -20,6 +20,7 @@ struct MpetOpT {
   double stiff[MPET_MAXF];
   const uint8_t* dmask;   // per DOF, 1 = Dirichlet (may be null)
   int lanes_u, lanes_p;   // lanes per P2 / P1 node row (0 = default)
+  int xcd_chunk;          // tiles per XCD run (0 = hardware order)
 };
 
 template <typename T>

@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(256)
 op_apply_kernel(MpetOpT<T> op, const T* __restrict__ x, const T* __restrict__ b,
                 T* __restrict__ y, int nbu) {
   const long long nu = 3ll * op.nP2;
-  const int bid = blockIdx.x;
+  const int bid = xcd_tile(blockIdx.x, gridDim.x, op.xcd_chunk);
   if (bid < nbu) {
     const int lane = threadIdx.x % GU;
     const int a = bid * (256 / GU) + threadIdx.x / GU;

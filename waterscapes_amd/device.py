@@ -143,14 +143,18 @@ class DeviceSystem:
         return out
 
     lanes = {0: (16, 16), 1: (16, 16)}     # lanes per (P2, P1) node row, per dtype
+    # row tiles per XCD run, per dtype (0: off); runs of 4 make the FP32
+    # apply 4.5% faster at 48^3 (0.303 vs 0.317 ms), flat from 2 to 128
+    xcd_chunk = {0: 4, 1: 4}
 
-    def op_desc(self, dtype, masked=True, lanes=None):
+    def op_desc(self, dtype, masked=True, lanes=None, xcd_chunk=None):
         vals = self.op64 if dtype == 0 else self.op32
         coef = _lib.make_coef(self.Cm, self.Cs, self.mu)
         lu, lp = lanes or self.lanes[dtype]
+        xc = self.xcd_chunk[dtype] if xcd_chunk is None else xcd_chunk
         return _lib.OpDesc(self.nP2, self.nP1, self.A, self.pat, dtype, ptr(vals["Kuu"]),
                            ptr(vals["Dv"]), ptr(vals["Dt"]), ptr(vals["M1"]), ptr(vals["St"]),
-                           coef, ptr(self.dmask) if masked else None, lu, lp)
+                           coef, ptr(self.dmask) if masked else None, lu, lp, xc)
 
     def row_lengths(self, which):
         rp = {k: np.diff(v[0]) for k, v in self.pat_host.items()}
