@@ -206,6 +206,14 @@ int mpet_assemble_primitives(const MpetMeshDesc* mesh, const MpetPatterns* pat,
                              const double* Kcell, int A, double mu,
                              const MpetPrimitives* out, hipStream_t stream);
 
+/* P2 mass int phi_a phi_b on the P2 node pattern (nnz22 doubles, device),
+ * the vector mass of the u block that the reference adds to the
+ * preconditioner with a displacement nullspace (inner(u, v) dx,
+ * mpet/mpettotalpressuresolver.py:290-293) and the source of the rigid-
+ * motion constraint rows.  Deterministic (colour order). */
+int mpet_assemble_p2_mass(const MpetMeshDesc* mesh, const MpetPatterns* pat, double* M2,
+                          hipStream_t stream);
+
 /* Scalar CSR (int64 rowptr computed by the caller) of the un-eliminated A
  * (which = 0) or P (which = 1) in canonical numbering — the PETSc AIJ matrix
  * that `assemble(a)` / `assemble(prec)` produce (:437, :444). */
@@ -336,6 +344,34 @@ void mpet_amg_destroy(void* h);
 int mpet_solver_create_csr(const MpetCsrDesc* csr64, const uint8_t* dmask, int nP2, int nP1,
                            int F, const MpetPrecDesc* prec64, const MpetSolverOptions* opt,
                            void** handle);
+
+/* FP64 MINRES on the bordered system [A B^T; B 0] of the nullspace
+ * multipliers (rigid motions of u, constants of chosen pressure fields:
+ * mpet/mpettotalpressuresolver.py:179-202, 285-306; mpet/mpetsolver.py:
+ * 107-128, 203-215), replacing the LU / MINRES solve of the augmented
+ * mixed space.  A is the CSR operator of mpet_solver_create_csr (F pressure
+ * fields); B (m x n, row-major, device, borrowed) the constraint rows of
+ * mpet_constraint_rows, m <= 16; omega > 0 the multiplier block of the
+ * preconditioner (omega I; the reference's int z r dx gives |Omega|).
+ * Vectors b and x of mpet_solver_solve then hold n + m entries (fields,
+ * then multipliers); the multiplier rows of b are the constraint values
+ * (0 in the reference).  opt->mode must be 0. */
+int mpet_solver_create_bordered(const MpetCsrDesc* csr64, const uint8_t* dmask, int nP2,
+                                int nP1, int F, const MpetPrecDesc* prec64, const double* B,
+                                int m, double omega, const MpetSolverOptions* opt,
+                                void** handle);
+
+/* Constraint rows B of the bordered system (device, (nz + nf) x n,
+ * row-major): rows k < nz = int Z_k . v for the P2 vector test functions,
+ * from the P2 mass M2 (nnz22, mpet_assemble_p2_mass) and Zt (nz x 3 nP2,
+ * the rigid motions at the P2 nodes, node-interleaved; nz = 0 or 6);
+ * rows nz + j = int q over the P1 test functions of pressure field
+ * fields[j] (host array, field numbering of the DOF layout), from the P1
+ * mass M1 (nnz11).  Restates the constraint terms of
+ * mpet/mpettotalpressuresolver.py:285-303 and mpet/mpetsolver.py:203-215. */
+int mpet_constraint_rows(const MpetPatterns* pat, const double* M2, const double* M1, int nP2,
+                         int nP1, int F, const double* Zt, int nz, const int* fields, int nf,
+                         double* B, hipStream_t stream);
 
 int mpet_solver_create(const MpetOpDesc* op64, const MpetOpDesc* op32,
                        const MpetCsrDesc* csr64, const MpetPrecDesc* prec64,

@@ -23,7 +23,11 @@ boundary-condition logic of the reference:
 It also restates the two-field MPETSolver (mpet/mpetsolver.py): forms
 :196-201 with the stress of mpetproblem.py:18-24, Robin :252-254, prec
 :269-271, Dirichlet DOFs on sub(i+1) (:74-81), and its direct time loop
-:382-462 (OracleProblem(formulation="two_field")).
+:382-462 (OracleProblem(formulation="two_field")), and the nullspace
+multipliers of both solvers (rigid motions of mpet/rm_basis_L2.py:10-74,
+constraint terms mpettotalpressuresolver.py:285-303, mpetsolver.py:203-215)
+as a bordered system solved by LU (OracleProblem(u_nullspace=...,
+p_nullspace=...)).
 
 Canonical numbering (inputs to every function): u DOF = 3*a + j for P2 node a,
 component j; p_f DOF = 3*nP2 + F*c + f for P1 node c, field f = 0..F-1 with
@@ -570,6 +574,73 @@ def dirichlet_dofs(mesh_facets, facet_cell, facet_local, cells, cell_p2, cell_p1
     return np.sort(lay.nu + lay.F * nodes + (field - 1))
 
 
+# ---------------------------------------------------------------------------
+# nullspace multipliers (R-space Lagrange multipliers)
+# ---------------------------------------------------------------------------
+
+
+def rigid_motions(coords, cells):
+    """mpet/rm_basis_L2.py:10-74, 3D branch: centroid c = int x / |Omega|,
+    rotational Gram matrix R_ij = int (x-c) x e_i . (x-c) x e_j (:41-48),
+    eigenpairs (w_k, v_k) of R (:51-53, numpy eigh, eigenvectors as rows);
+    translations v_k / sqrt|Omega| (:58), rotations (x-c) x v_k / sqrt(w_k)
+    (:64-72).  Integrals by degree-2 quadrature (exact for these integrands).
+    Returns (list of six callables X -> (n, 3), volume)."""
+    J, det, _ = cell_geometry(coords, cells)
+    qp, qw = tet_rule(2)
+    v0 = coords[cells[:, 0]]
+    X = v0[:, None, :] + np.einsum("cij,qj->cqi", J, qp)           # nC,nq,3
+    w = det[:, None] * qw[None, :]
+    vol = w.sum()
+    c = np.einsum("cq,cqi->i", w, X) / vol
+    R = np.zeros((3, 3))
+    E = np.eye(3)
+    for i in range(3):
+        for j in range(3):
+            ci = np.cross(X - c, E[i])
+            cj = np.cross(X - c, E[j])
+            R[i, j] = np.einsum("cq,cqk,cqk->", w, ci, cj)
+    eigw, eigv = np.linalg.eigh(R)
+    eigv = eigv.T
+    out = []
+    for v in eigv:
+        out.append(lambda P, v=v: np.broadcast_to(v / math.sqrt(vol), (len(P), 3)).copy())
+    for v, ew in zip(eigv, eigw):
+        out.append(lambda P, v=v, ew=ew: np.cross(np.asarray(P) - c, v) / math.sqrt(ew))
+    return out, vol
+
+
+def constraint_rows(coords, cells, cell_p2, cell_p1, lay, Z, fields):
+    """Rows of the multiplier constraints as a sparse (m, n) matrix:
+    int Z_k . v over the P2 vector test functions for each rigid motion Z_k
+    (mpettotalpressuresolver.py:287-288, mpetsolver.py:205-206), then
+    int q over the P1 test functions of each constrained pressure field
+    (layout field number; :297-299, mpetsolver.py:212-214).  Element
+    quadrature of degree 3 (exact: linear Z times quadratic phi)."""
+    J, det, _ = cell_geometry(coords, cells)
+    qp, qw = tet_rule(3)
+    v0 = coords[cells[:, 0]]
+    X = v0[:, None, :] + np.einsum("cij,qj->cqi", J, qp)
+    phi = p2_ref_values(qp)
+    psi = p1_ref_values(qp)
+    nC = cells.shape[0]
+    rows = []
+    for Zk in Z:
+        Zq = np.asarray(Zk(X.reshape(-1, 3)), float).reshape(nC, len(qp), 3)
+        loc = np.einsum("q,c,qa,cqi->cai", qw, det, phi, Zq)          # nC,10,3
+        dofs = 3 * cell_p2[:, :, None] + np.arange(3)[None, None, :]
+        rows.append(scatter_vector(lay.n, dofs, loc))
+    for f in fields:
+        loc = np.einsum("q,c,qa->ca", qw, det, psi)
+        rows.append(scatter_vector(lay.n, lay.pdofs(cell_p1, f), loc))
+    return sp.csr_matrix(np.array(rows)) if rows else sp.csr_matrix((0, lay.n))
+
+
+def bordered(A, B, C=None):
+    """[A B^T; B C] (C = 0 when None)."""
+    return sp.bmat([[A, B.T], [B, C]], format="csr")
+
+
 def apply_row_replace(A, dofs):
     """DirichletBC.apply(A): rows replaced by identity rows (not symmetric)."""
     A = A.tolil(copy=True)
@@ -678,7 +749,8 @@ class OracleProblem:
                  momentum_markers=None, continuity_markers=None,
                  s=None, s_degree=3, I=None, I_degree=3, beta=None,
                  p_robin=None, robin_degree=1, p_robin_degree=None,
-                 formulation="total_pressure", permc_spec="COLAMD", refine_steps=2):
+                 formulation="total_pressure", permc_spec="COLAMD", refine_steps=2,
+                 u_nullspace=False, p_nullspace=None):
         if formulation not in ("total_pressure", "two_field"):
             raise ValueError(formulation)
         self.two_field = formulation == "two_field"
@@ -718,6 +790,22 @@ class OracleProblem:
                          K_cells, which="P")
         self.bc_dofs = self._bc_dofs()
         self._B_mat = None
+        # nullspace multipliers (mpettotalpressuresolver.py:179-202, 285-306;
+        # mpetsolver.py:107-128, 203-215): p_nullspace[k] constrains the k-th
+        # pressure trial function of the reference's mixed space, i.e. layout
+        # field k (p_0 first in the total-pressure formulation)
+        self.u_nullspace = bool(u_nullspace)
+        self.p_fields = [k for k, f in enumerate(p_nullspace or []) if f]
+        self.nmult = 0
+        if self.u_nullspace or self.p_fields:
+            Z = []
+            if self.u_nullspace:
+                Z, _ = rigid_motions(m["coords"], m["cells"])
+            self.Z = Z
+            self.C_mat = constraint_rows(m["coords"], m["cells"], m["cell_p2"], m["cell_p1"],
+                                         self.lay, Z, self.p_fields)
+            self.nmult = self.C_mat.shape[0]
+            self.A_mat = bordered(self.A_mat, self.C_mat)
 
     # -- boundary facets -----------------------------------------------------
     def _marked_facets(self, markers, value):
@@ -893,15 +981,21 @@ class OracleProblem:
 
     # -- time loops -----------------------------------------------------------
     def solve_direct(self, up0):
-        """Generator mirroring solve_direct (:330-411): LU of row-replaced A."""
+        """Generator mirroring solve_direct (:330-411): LU of row-replaced A
+        (bordered by the multiplier rows when the problem has nullspaces;
+        up0 and the yielded states then hold the multipliers after the
+        fields)."""
         A = self.A_mat.copy()
         alld = np.concatenate(self.bc_dofs)
         A = apply_row_replace(A, alld)
         lu = spla.splu(A.tocsc(), permc_spec=self.permc_spec)
         up_prev = up0.copy()
         t = self.t0
+        n = self.lay.n
         while t < self.T - 1e-9:
-            b, t_theta, t_np1 = self.rhs(up_prev, t)
+            b, t_theta, t_np1 = self.rhs(up_prev[:n], t)
+            if self.nmult:
+                b = np.concatenate([b, np.zeros(self.nmult)])
             for d, v in zip(self.bc_dofs, self.bc_values(t_np1)):
                 b[d] = v
             up = lu.solve(b)
@@ -918,6 +1012,8 @@ class OracleProblem:
 
     def eliminated_system(self, b, t_np1):
         """Symmetric elimination of all BCs in order (:487-489)."""
+        if self.nmult:
+            raise NotImplementedError("the oracle solves bordered systems directly only")
         A = self.A_mat.copy()
         for d, v in zip(self.bc_dofs, self.bc_values(t_np1)):
             b[d] = v
@@ -925,6 +1021,8 @@ class OracleProblem:
         return A, b
 
     def preconditioner(self):
+        if self.nmult:
+            raise NotImplementedError("the oracle solves bordered systems directly only")
         P = self.P_mat.copy()
         for d in self.bc_dofs:
             P = apply_symmetric(P, d)

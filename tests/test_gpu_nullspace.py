@@ -1,0 +1,186 @@
+"""Nullspace multipliers on the GPU (bordered FP64 MINRES) vs the oracle's
+bordered LU and known answers.
+
+Problems are set up through the product API the way the reference's
+test_donut.py does (u_has_nullspace / p_has_nullspace, traction
+Expression("t") * FacetNormal on every facet), on a box whose rigid-motion
+Gram matrix has distinct eigenvalues.
+
+* uniform pressures (u nullspace, p_j = -t Dirichlet, traction
+  -sum alpha_j p_j n): u = 0, p_j = -t (and p_0 = sum alpha_j t) — the 3D
+  version of test_donut.py:16-71 — in both formulations, both solve paths of
+  the total-pressure solver;
+* uniform dilation with pure-Neumann pressures (u and p nullspaces,
+  test_donut.py:73-131 in 3D): u = t/(2 mu + 3 lambda) (x - c), p_j = 0,
+  pressure multipliers 3 alpha_j (gamma_{n+1} - gamma_n);
+* a loaded total-pressure case with p_has_nullspace[0] (the reference's p[0]
+  is the total pressure) and a body force that is not orthogonal to the
+  rigid motions, so the displacement multipliers are nonzero: every field
+  matches the oracle to 1e-8 relative, the constraints hold on the GPU
+  output, and the multiplier norms (independent of the choice of basis
+  inside the rigid-motion space) match the oracle's."""
+import numpy as np
+import pytest
+
+from conftest import oracle_mesh
+from oracle import mpet_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+BOX = ((0.0, 0.0, 0.0), (1.0, 1.5, 0.7))
+
+
+def _problem(A, alpha, K, c=None, E=500.0, nu=0.35, n=(2, 3, 2)):
+    from waterscapes_amd import MPETProblem, Constant, BoxMesh
+    mesh = BoxMesh(BOX[0], BOX[1], *n)
+    time = Constant(0.0)
+    params = dict(J=A, alpha=alpha, K=K, S=[[0.0] * A for _ in range(A)],
+                  c=c if c is not None else (0.0,) * A, nu=nu, E=E)
+    return MPETProblem(mesh, time, params=params), params
+
+
+def _markers(problem):
+    mm = np.asarray(problem.momentum_boundary_markers.array(), dtype=np.uint64)
+    cm = [np.asarray(c.array(), dtype=np.uint64) for c in problem.continuity_boundary_markers]
+    return mm, cm
+
+
+def _run(solver, direct=None):
+    if direct is None:
+        gen = solver.solve()
+    else:
+        gen = solver.solve_direct() if direct else solver.solve_iterative()
+    return [(up.vector().get_local().copy(), t) for up, t in gen]
+
+
+@pytest.mark.parametrize("case", ["tp_direct", "tp_iterative", "two_field"])
+def test_uniform_pressure_known_answer(case):
+    from waterscapes_amd import (MPETSolver, MPETTotalPressureSolver, Expression,
+                                 FacetNormal, CompiledSubDomain)
+    tp = case.startswith("tp")
+    A = 2
+    problem, params = _problem(A, (1.0, 0.5), (1e-2, 1e-1))
+    sa = sum(params["alpha"])
+    n = FacetNormal(problem.mesh)
+    problem.s = Expression("sa*t", sa=sa, t=problem.time, degree=0) * n
+    problem.u_has_nullspace = True
+    CompiledSubDomain("on_boundary").mark(problem.momentum_boundary_markers, 1)
+    problem.p_bar = [Expression("-t", t=problem.time, degree=0) for _ in range(A)]
+    for i in range(A):
+        CompiledSubDomain("on_boundary").mark(problem.continuity_boundary_markers[i], 0)
+    sp_ = dict(dt=0.1, theta=1.0, T=0.2)
+    solver = MPETTotalPressureSolver(problem, sp_) if tp else MPETSolver(problem, sp_)
+    space = solver.up.function_space()
+    assert space.nmult == 6 and space.num_sub_spaces() == space.F + 2
+    steps = _run(solver, direct=None if not tp else case == "tp_direct")
+    assert len(steps) == 2
+    m = oracle_mesh(space)
+    mm, cm = _markers(problem)
+    ref = O.OracleProblem(m, params, dict(dt=0.1, theta=1.0, T=0.2),
+                          u_bar=lambda X, t: np.zeros((X.shape[0], 3)),
+                          p_bar=[lambda X, t: np.full(X.shape[0], -t) for _ in range(A)],
+                          s=lambda X, t, nrm: sa * t * nrm, s_degree=0,
+                          momentum_markers=mm, continuity_markers=cm,
+                          formulation="total_pressure" if tp else "two_field", u_nullspace=True)
+    ref_steps = list(ref.solve_direct(np.zeros(space.dim())))
+    nu, F = space.nu, space.F
+    for (x, t), (xr, tr) in zip(steps, ref_steps):
+        assert abs(t - tr) < 1e-14
+        print(case, t, np.abs(x[:nu]).max(), np.abs(x[space.ndofs:]).max(), flush=True)
+        assert np.abs(x[:nu]).max() < 1e-9
+        for i in range(A):
+            p = x[nu:space.ndofs].reshape(-1, F)[:, space.network_sub(i) - 1]
+            assert np.abs(p + t).max() < 1e-9
+        if tp:
+            assert np.abs(x[nu:space.ndofs].reshape(-1, F)[:, 0] - sa * t).max() < 1e-9
+        assert np.abs(x[space.ndofs:]).max() < 1e-9
+        assert np.abs(x - xr).max() < 1e-9
+
+
+def test_dilation_with_pressure_nullspaces():
+    from waterscapes_amd import MPETSolver, Expression, FacetNormal, CompiledSubDomain
+    A = 2
+    problem, params = _problem(A, (1.0, 0.5), (1e-2, 1e-1))
+    n = FacetNormal(problem.mesh)
+    problem.s = Expression("t", t=problem.time, degree=0) * n
+    problem.u_has_nullspace = True
+    problem.p_has_nullspace = (True, True)
+    CompiledSubDomain("on_boundary").mark(problem.momentum_boundary_markers, 1)
+    CompiledSubDomain("on_boundary").mark(problem.continuity_boundary_markers[0], 1)
+    solver = MPETSolver(problem, dict(dt=0.1, theta=1.0, T=0.2))
+    space = solver.up.function_space()
+    assert space.nmult == 8
+    steps = _run(solver)
+    m = oracle_mesh(space)
+    mm, cm = _markers(problem)
+    ref = O.OracleProblem(m, params, dict(dt=0.1, theta=1.0, T=0.2),
+                          s=lambda X, t, nrm: t * nrm, s_degree=0, momentum_markers=mm,
+                          continuity_markers=cm, formulation="two_field", u_nullspace=True,
+                          p_nullspace=(True, True))
+    ref_steps = list(ref.solve_direct(np.zeros(space.dim())))
+    mu, lmbda = O.lame(params["E"], params["nu"])
+    cen = 0.5 * (np.array(BOX[0]) + np.array(BOX[1]))
+    nu, nd = space.nu, space.ndofs
+    t_prev = 0.0
+    for (x, t), (xr, _) in zip(steps, ref_steps):
+        u = (t / (2 * mu + 3 * lmbda)) * (space.dofmap.x2 - cen)
+        err_u = np.abs(x[:nu] - u.reshape(-1)).max() / np.abs(u).max()
+        want = 3 * np.asarray(params["alpha"]) * (t - t_prev) / (2 * mu + 3 * lmbda)
+        print("dilation", t, err_u, x[nd + 6:], want, flush=True)
+        assert err_u < 1e-8
+        assert np.abs(x[nu:nd]).max() < 1e-8 * np.abs(want).max()
+        assert np.abs(x[nd:nd + 6]).max() < 1e-8 * np.abs(want).max()
+        assert np.abs(x[nd + 6:] - want).max() < 1e-8 * np.abs(want).max()
+        assert np.abs(x[:nu] - xr[:nu]).max() < 1e-8 * np.abs(u).max()
+        assert np.abs(x[nd + 6:] - xr[nd + 6:]).max() < 1e-8 * np.abs(want).max()
+        t_prev = t
+
+
+@pytest.mark.parametrize("direct", [True, False])
+def test_loaded_total_pressure_with_p0_constraint(direct):
+    """Body force with a net force and torque, traction-free boundary, u
+    nullspace, p_has_nullspace = (True, False): the reference's p[0] is
+    p_0; network 1 Dirichlet on x = 0, network 2 Neumann."""
+    from waterscapes_amd import MPETTotalPressureSolver, Expression, CompiledSubDomain
+    A = 2
+    problem, params = _problem(A, (0.5, 0.3), (1e-2, 1e-3), c=(1e-3, 2e-3), E=1e3, nu=0.3,
+                               n=(3, 3, 2))
+    problem.f = Expression(("1.0 + x[1]", "0.5*x[0]*t", "-0.2"), t=problem.time, degree=1)
+    problem.g = [Expression("x[2]", degree=1), Expression("0.1*t", t=problem.time, degree=0)]
+    problem.u_has_nullspace = True
+    problem.p_has_nullspace = (True, False)
+    CompiledSubDomain("on_boundary").mark(problem.momentum_boundary_markers, 1)
+    CompiledSubDomain("on_boundary && near(x[0], 0.0)").mark(
+        problem.continuity_boundary_markers[0], 0)
+    problem.p_bar = [Expression("0.3*t", t=problem.time, degree=0), Expression("0.0", degree=0)]
+    solver = MPETTotalPressureSolver(problem, dict(dt=0.05, theta=1.0, T=0.1))
+    space = solver.up.function_space()
+    assert space.nmult == 7
+    steps = _run(solver, direct=direct)
+    m = oracle_mesh(space)
+    mm, cm = _markers(problem)
+    ref = O.OracleProblem(
+        m, params, dict(dt=0.05, theta=1.0, T=0.1),
+        f=lambda X, t: np.stack([1.0 + X[:, 1], 0.5 * X[:, 0] * t, np.full(X.shape[0], -0.2)],
+                                1), f_degree=1,
+        g=[lambda X, t: X[:, 2], lambda X, t: np.full(X.shape[0], 0.1 * t)], g_degree=1,
+        u_bar=lambda X, t: np.zeros((X.shape[0], 3)),
+        p_bar=[lambda X, t: np.full(X.shape[0], 0.3 * t), lambda X, t: np.zeros(X.shape[0])],
+        momentum_markers=mm, continuity_markers=cm, u_nullspace=True, p_nullspace=(True, False))
+    ref_steps = list(ref.solve_direct(np.zeros(space.dim())))
+    nd = space.ndofs
+    for (x, t), (xr, _) in zip(steps, ref_steps):
+        errs = []
+        for i in range(space.F + 1):
+            idx = space.field_index(i)
+            errs.append(np.linalg.norm(x[idx] - xr[idx]) / np.linalg.norm(xr[idx]))
+        cons = ref.C_mat @ x[:nd]
+        zt, zr = x[nd:], xr[nd:]
+        print("loaded", direct, t, errs, np.abs(cons).max(), zt, zr, flush=True)
+        assert max(errs) < 1e-8
+        assert np.abs(cons).max() < 1e-10 * np.abs(x[:nd]).max()
+        for a, b in ((0, 3), (3, 6)):
+            assert abs(np.linalg.norm(zt[a:b]) - np.linalg.norm(zr[a:b])) < \
+                1e-8 * np.linalg.norm(zr[:6])
+        assert np.linalg.norm(zr[:6]) > 1e-3
+        assert abs(zt[6] - zr[6]) < 1e-8 * max(abs(zr[6]), np.linalg.norm(zr[:6]))

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libmpet_hip.so")
 
 MAXF = 8
-TAB_SIZE = 1760
+TAB_SIZE = 1860
 
 
 class MeshDesc(C.Structure):
@@ -106,6 +106,9 @@ def lib():
         "mpet_expand_csr": ([P, P, C.c_int, C.c_int, C.c_int, P, C.c_int, P, P, P, P], C.c_int),
         "mpet_tf_expand_csr": ([P, P, C.c_int, C.c_int, C.c_int, P, C.c_int, P, P, P, P],
                                C.c_int),
+        "mpet_assemble_p2_mass": ([P, P, P, P], C.c_int),
+        "mpet_constraint_rows": ([P, P, P, C.c_int, C.c_int, C.c_int, P, C.c_int, P, C.c_int,
+                                  P, P], C.c_int),
         "mpet_rhs_pressure": ([P, P, C.c_int, C.c_int, C.c_int, P, P, P, P], C.c_int),
         "mpet_cell_source": ([P, P, C.c_int, P, C.c_int, P, C.c_int, C.c_double,
                               C.c_longlong, C.c_int, C.c_int, P, P], C.c_int),

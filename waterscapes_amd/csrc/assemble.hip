@@ -153,6 +153,29 @@ assemble_colour_kernel(const double* __restrict__ X, const int* __restrict__ cel
   }
 }
 
+// P2 mass int phi_a phi_b on the P2 pattern, one colour of cells: lanes
+// 0..9 of each 16 own the cell's P2 rows (same ownership and order as
+// assemble_colour_kernel, so the sums are reproducible).
+__global__ void __launch_bounds__(256)
+p2_mass_colour_kernel(const double* __restrict__ X, const int* __restrict__ cells,
+                      const int* __restrict__ cell_p2, const int* __restrict__ colour_cells,
+                      int cbeg, int cend, const int* __restrict__ rp22,
+                      const int* __restrict__ ci22, double* __restrict__ M2) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int ci = cbeg + t / 16;
+  int a = t % 16;
+  if (ci >= cend || a >= 10) return;
+  int e = colour_cells[ci];
+  CellGeom G;
+  cell_geometry(X, cells, e, G);
+  const int row = cell_p2[10 * e + a];
+  const int rb = rp22[row], re = rp22[row + 1];
+  for (int b = 0; b < 10; ++b) {
+    int pos = lower_bound_i32(ci22, rb, re, cell_p2[10 * e + b]);
+    M2[pos] += G.vol * c_tab[MPET_TAB_M2 + a * 10 + b];
+  }
+}
+
 // Scalar CSR of A (un-eliminated) or P (block diagonal, un-eliminated) in
 // the canonical numbering; one thread per row, columns emitted in ascending
 // order.  rowptr is computed on the host from the node-pattern row lengths.
@@ -329,6 +352,26 @@ extern "C" int mpet_assemble_primitives(const MpetMeshDesc* mesh, const MpetPatt
     hipLaunchKernelGGL(assemble_colour_kernel, dim3(blocks), dim3(256), 0, stream,
                        mesh->coords, mesh->cells, mesh->cell_p2, mesh->colour_cells, cb, ce,
                        Kcell, A, mu, *pat, *out);
+  }
+  HIP_TRY(hipGetLastError());
+  return OK;
+}
+
+extern "C" int mpet_assemble_p2_mass(const MpetMeshDesc* mesh, const MpetPatterns* pat,
+                                     double* M2, hipStream_t stream) {
+  if (!mesh || !pat || !M2 || pat->nnz22 < 1) {
+    mpet_set_error("mpet_assemble_p2_mass: invalid argument");
+    return ERR_INVALID;
+  }
+  HIP_TRY(hipMemsetAsync(M2, 0, sizeof(double) * (long long)pat->nnz22, stream));
+  for (int k = 0; k < mesh->ncolours; ++k) {
+    int cb = (int)mesh->colour_offsets[k], ce = (int)mesh->colour_offsets[k + 1];
+    if (ce <= cb) continue;
+    long long threads = 16ll * (ce - cb);
+    int blocks = (int)((threads + 255) / 256);
+    hipLaunchKernelGGL(p2_mass_colour_kernel, dim3(blocks), dim3(256), 0, stream, mesh->coords,
+                       mesh->cells, mesh->cell_p2, mesh->colour_cells, cb, ce, pat->rp22,
+                       pat->ci22, M2);
   }
   HIP_TRY(hipGetLastError());
   return OK;

@@ -310,6 +310,141 @@ __global__ void cvt_up_kernel(long long n, const float* __restrict__ a, double* 
   if (i < n) b[i] = (double)a[i];
 }
 
+// ---------------------------------------------------------------------------
+// Bordered systems [A B^T; B 0] (nullspace multipliers): m constraint rows
+// B (m x n, row-major, dense) after the n field rows; Krylov vectors carry
+// the m multipliers after the fields.
+// ---------------------------------------------------------------------------
+constexpr int MAXB = 16;   // constraint rows
+
+// K1 epilogue on the field rows: y = s (A z + B^T z_t) - c1 r1 (Dirichlet
+// rows 0), v = s z; partials of v . y (row 0 of partial) and of B z (row
+// 1 + k), stride gridDim.x.  az = A z.
+__global__ void __launch_bounds__(NT)
+k1b_kernel(long long n, int m, const double* __restrict__ B, const double* __restrict__ az,
+           const double* __restrict__ z, const double* __restrict__ r1,
+           const uint8_t* __restrict__ mask, double* __restrict__ y, double* __restrict__ v,
+           const MinresState* st, const int* __restrict__ skip, double* __restrict__ partial) {
+  if (*skip) return;
+  double s, c1;
+  k1_scalars(st, s, c1);
+  double zt[MAXB], bz[MAXB];
+#pragma unroll
+  for (int k = 0; k < MAXB; ++k) {
+    zt[k] = k < m ? z[n + k] : 0.0;
+    bz[k] = 0.0;
+  }
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < n;
+       i += (long long)gridDim.x * NT) {
+    double ax = az[i];
+    const double zi = z[i];
+#pragma unroll
+    for (int k = 0; k < MAXB; ++k) {
+      if (k < m) {
+        const double bk = B[k * n + i];
+        ax += bk * zt[k];
+        bz[k] += bk * zi;
+      }
+    }
+    const double yy = (mask && mask[i]) ? 0.0 : s * ax - c1 * r1[i];
+    const double vv = s * zi;
+    y[i] = yy;
+    v[i] = vv;
+    acc += vv * yy;
+  }
+  double t = block_sum(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+#pragma unroll
+  for (int k = 0; k < MAXB; ++k) {
+    if (k < m) {
+      __syncthreads();
+      t = block_sum(bz[k]);
+      if (threadIdx.x == 0) partial[(long long)(k + 1) * gridDim.x + blockIdx.x] = t;
+    }
+  }
+}
+
+// K1 finalize of a bordered system: the multiplier rows y_t = s B z - c1
+// r1_t, v_t = s z_t, and alfa = v . y over fields and multipliers.
+__global__ void __launch_bounds__(NT)
+k1bf_kernel(const double* __restrict__ partial, int np, int m, long long n,
+            const double* __restrict__ z, const double* __restrict__ r1, double* __restrict__ y,
+            double* __restrict__ v, MinresState* st, const int* __restrict__ skip) {
+  if (*skip) return;
+  double s, c1;
+  k1_scalars(st, s, c1);
+  double alfa = sum_partials(partial, np);
+  for (int k = 0; k < m; ++k) {
+    __syncthreads();
+    const double bzk = sum_partials(partial + (long long)(k + 1) * np, np);
+    if (threadIdx.x == 0) {
+      const double yy = s * bzk - c1 * r1[n + k];
+      const double vv = s * z[n + k];
+      y[n + k] = yy;
+      v[n + k] = vv;
+      alfa += vv * yy;
+    }
+  }
+  if (threadIdx.x == 0) st->alfa = alfa;
+}
+
+// z_t = sc r_t on the multiplier rows (their preconditioner block is
+// omega I, sc = 1 / omega).
+__global__ void tail_scale_kernel(int m, const double* __restrict__ r, double* __restrict__ z,
+                                  double sc, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  if ((int)threadIdx.x < m) z[threadIdx.x] = sc * r[threadIdx.x];
+}
+
+// Residual of a bordered system, field rows: r -= B^T x_t (r holds
+// Pi (b - A x); Dirichlet rows stay 0); partials of B x (row k, stride
+// gridDim.x).
+__global__ void __launch_bounds__(NT)
+bres_kernel(long long n, int m, const double* __restrict__ B, const double* __restrict__ x,
+            const uint8_t* __restrict__ mask, double* __restrict__ r,
+            double* __restrict__ partial) {
+  double xt[MAXB], bx[MAXB];
+#pragma unroll
+  for (int k = 0; k < MAXB; ++k) {
+    xt[k] = k < m ? x[n + k] : 0.0;
+    bx[k] = 0.0;
+  }
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < n;
+       i += (long long)gridDim.x * NT) {
+    double ri = r[i];
+    const double xi = x[i];
+#pragma unroll
+    for (int k = 0; k < MAXB; ++k) {
+      if (k < m) {
+        const double bk = B[k * n + i];
+        ri -= bk * xt[k];
+        bx[k] += bk * xi;
+      }
+    }
+    if (!(mask && mask[i])) r[i] = ri;
+  }
+#pragma unroll
+  for (int k = 0; k < MAXB; ++k) {
+    if (k < m) {
+      if (k) __syncthreads();
+      const double t = block_sum(bx[k]);
+      if (threadIdx.x == 0) partial[(long long)k * gridDim.x + blockIdx.x] = t;
+    }
+  }
+}
+
+// Multiplier rows of the residual: r_t = b_t - B x.
+__global__ void __launch_bounds__(NT)
+bresf_kernel(const double* __restrict__ partial, int np, int m, long long n,
+             const double* __restrict__ b, double* __restrict__ r) {
+  for (int k = 0; k < m; ++k) {
+    if (k) __syncthreads();
+    const double t = sum_partials(partial + (long long)k * np, np);
+    if (threadIdx.x == 0) r[n + k] = b[n + k] - t;
+  }
+}
+
 }  // namespace mpet
 
 // ===========================================================================
@@ -363,7 +498,16 @@ struct MpetSolver {
   double* scal = nullptr;                 // device scalar
   double* scal_host = nullptr;            // pinned
   double* partial = nullptr;
+  // bordered system (nullspace multipliers): mb constraint rows B (mb x n),
+  // multiplier block of the preconditioner omega I (tail_inv = 1 / omega)
+  int mb = 0;
+  const double* B = nullptr;
+  double tail_inv = 0.0;
+  long long nv = 0;                       // Krylov vector length n + mb
+  double* yraw = nullptr;                 // A z of the field rows
 };
+
+static int bgrid(long long n) { return grid_for(n, NT * 4, MAX_PARTIALS / (MAXB + 1)); }
 
 // z = M^-1 r.  Mode 2 applies the FP32 preconditioner to FP64 vectors
 // (convert, V-cycles in FP32, convert back).
@@ -380,6 +524,12 @@ static void apply_M(MpetSolver* S, const Precond<T>& P, const T* r, T* z, const 
                          skip);
       return;
     }
+    if (S->mb) {
+      precond_apply(P, r, z, skip, s);
+      hipLaunchKernelGGL(tail_scale_kernel, dim3(1), dim3(64), 0, s, S->mb, r + S->n, z + S->n,
+                         S->tail_inv, skip);
+      return;
+    }
   }
   precond_apply(P, r, z, skip, s);
 }
@@ -387,7 +537,7 @@ static void apply_M(MpetSolver* S, const Precond<T>& P, const T* r, T* z, const 
 
 template <typename T>
 static int build_minres(MpetSolver* S, Minres<T>& M) {
-  M.n = S->n;
+  M.n = S->nv;
   M.nu = S->nu;
   int rc;
   if ((rc = alloc_dev(S->allocs, &M.R[0], M.n)) || (rc = alloc_dev(S->allocs, &M.R[1], M.n)) ||
@@ -419,7 +569,17 @@ static void minres_iter(MpetSolver* S, Minres<T>& M, const Precond<T>& P, bool c
   int eb = (int)((n + NT - 1) / NT);
   int np = 1;
   if constexpr (sizeof(T) == 8) {
-    if (csr) {
+    if (csr && S->mb) {
+      launch_csr_spmv<double>(S->csr_rows, S->csr_rp, S->csr_ci, S->csr_v, M.z,
+                              (const double*)nullptr, (const uint8_t*)nullptr, S->yraw,
+                              S->csr_lpr, s);
+      const int bg = bgrid(S->n);
+      hipLaunchKernelGGL(k1b_kernel, dim3(bg), dim3(NT), 0, s, S->n, S->mb, S->B,
+                         (const double*)S->yraw, (const double*)M.z, (const double*)Ra,
+                         S->opt_mask64, M.y, M.v, M.st, skip, M.partial);
+      hipLaunchKernelGGL(k1bf_kernel, dim3(1), dim3(NT), 0, s, M.partial, bg, S->mb, S->n,
+                         (const double*)M.z, (const double*)Ra, M.y, M.v, M.st, skip);
+    } else if (csr) {
       epi.mask = S->opt_mask64;
       int lpr = S->csr_lpr;
       int rows_per_block = NT / lpr;
@@ -448,7 +608,8 @@ static void minres_iter(MpetSolver* S, Minres<T>& M, const Precond<T>& P, bool c
     switch (S->F) { K1OP(2) K1OP(3) K1OP(4) K1OP(5) K1OP(6) K1OP(7) K1OP(8) default: break; }
 #undef K1OP
   }
-  hipLaunchKernelGGL(k1f_kernel, dim3(1), dim3(NT), 0, s, M.partial, np, M.st, skip);
+  if (!(csr && S->mb))
+    hipLaunchKernelGGL(k1f_kernel, dim3(1), dim3(NT), 0, s, M.partial, np, M.st, skip);
   hipLaunchKernelGGL((k2_kernel<T>), dim3(eb), dim3(NT), 0, s, n, M.y, Rb, Ra, M.st, skip);
   apply_M(S, P, Ra, M.z, skip, s);
   int dg = dot_grid(n);
@@ -524,6 +685,7 @@ static int minres_solve(MpetSolver* S, Minres<T>& M, const Precond<T>& P, bool c
   return minres_continue(S, M, P, csr, rtol, maxit, check, graph, s, iters, conv);
 }
 
+// r = Pi (b - K x) in FP64 (K = A, or the bordered [A B^T; B 0]).
 static void op_residual64(MpetSolver* S, const double* b, const double* x, double* r,
                           hipStream_t s) {
   if (S->opt.mode == 0 && S->csr_rp && S->opt.use_csr)
@@ -531,6 +693,13 @@ static void op_residual64(MpetSolver* S, const double* b, const double* x, doubl
                             r, S->csr_lpr, s);
   else
     launch_op_apply<double>(S->op64, x, b, r, s);
+  if (S->mb) {
+    const int bg = bgrid(S->n);
+    hipLaunchKernelGGL(bres_kernel, dim3(bg), dim3(NT), 0, s, S->n, S->mb, S->B, x,
+                       S->opt_mask64, r, S->partial);
+    hipLaunchKernelGGL(bresf_kernel, dim3(1), dim3(NT), 0, s, (const double*)S->partial, bg,
+                       S->mb, S->n, b, r);
+  }
 }
 
 // ||Pi (b - A g)||_{M^-1}: the preconditioned norm of the right-hand side of
@@ -550,7 +719,7 @@ static int bnorm_M(MpetSolver* S, Minres<T>& M, const Precond<T>& P, const doubl
   if constexpr (sizeof(T) == 4)
     hipLaunchKernelGGL(f64_to_f32_kernel, dim3(eb), dim3(NT), 0, s, n, S->r64, M.b);
   else
-    HIPCHK(hipMemcpyAsync(M.b, S->r64, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(M.b, S->r64, sizeof(double) * M.n, hipMemcpyDeviceToDevice, s));
   double beta;
   int rc = minres_start(S, M, P, 1.0, 1, s, &beta);   // synchronises
   if (rc) return rc;
@@ -564,17 +733,21 @@ MpetOpT<T> mpet_make_op(const MpetOpDesc* d);
 static int finish_create(MpetSolver* S, const MpetPrecDesc* prec64, const MpetPrecDesc* prec32,
                          void** handle);
 
-extern "C" int mpet_solver_create_csr(const MpetCsrDesc* csr64, const uint8_t* dmask, int nP2,
-                                      int nP1, int F, const MpetPrecDesc* prec64,
-                                      const MpetSolverOptions* opt, void** handle) {
+static int create_csr(const char* who, const MpetCsrDesc* csr64, const uint8_t* dmask, int nP2,
+                      int nP1, int F, const MpetPrecDesc* prec64, const double* B, int m,
+                      double omega, const MpetSolverOptions* opt, void** handle) {
+  char msg[160];
   if (!csr64 || !csr64->rp || !csr64->ci || !csr64->v || !dmask || !prec64 || !opt ||
       !handle || F < 1 || F > MPET_MAXF || nP2 < 1 || nP1 < 1 ||
-      csr64->nrows != 3ll * nP2 + (long long)F * nP1) {
-    mpet_set_error("mpet_solver_create_csr: invalid argument");
+      csr64->nrows != 3ll * nP2 + (long long)F * nP1 || m < 0 || m > MAXB ||
+      (m > 0 && (!B || !(omega > 0.0)))) {
+    snprintf(msg, sizeof(msg), "%s: invalid argument", who);
+    mpet_set_error(msg);
     return ERR_INVALID;
   }
   if (opt->mode != 0) {
-    mpet_set_error("mpet_solver_create_csr: only FP64 MINRES (mode 0) runs on a CSR operator alone");
+    snprintf(msg, sizeof(msg), "%s: only FP64 MINRES (mode 0) runs on a CSR operator alone", who);
+    mpet_set_error(msg);
     return ERR_INVALID;
   }
   MpetSolver* S = new MpetSolver();
@@ -583,6 +756,10 @@ extern "C" int mpet_solver_create_csr(const MpetCsrDesc* csr64, const uint8_t* d
   S->F = F;
   S->nu = 3ll * nP2;
   S->n = S->nu + (long long)F * nP1;
+  S->mb = m;
+  S->B = B;
+  S->tail_inv = m > 0 ? 1.0 / omega : 0.0;
+  S->nv = S->n + m;
   S->op64.nP2 = nP2;
   S->op64.nP1 = nP1;
   S->opt_mask64 = dmask;
@@ -592,6 +769,25 @@ extern "C" int mpet_solver_create_csr(const MpetCsrDesc* csr64, const uint8_t* d
   S->csr_v = csr64->v;
   S->csr_lpr = csr64->lpr ? csr64->lpr : 16;
   return finish_create(S, prec64, nullptr, handle);
+}
+
+extern "C" int mpet_solver_create_csr(const MpetCsrDesc* csr64, const uint8_t* dmask, int nP2,
+                                      int nP1, int F, const MpetPrecDesc* prec64,
+                                      const MpetSolverOptions* opt, void** handle) {
+  return create_csr("mpet_solver_create_csr", csr64, dmask, nP2, nP1, F, prec64, nullptr, 0,
+                    0.0, opt, handle);
+}
+
+extern "C" int mpet_solver_create_bordered(const MpetCsrDesc* csr64, const uint8_t* dmask,
+                                           int nP2, int nP1, int F, const MpetPrecDesc* prec64,
+                                           const double* B, int m, double omega,
+                                           const MpetSolverOptions* opt, void** handle) {
+  if (m < 1) {
+    mpet_set_error("mpet_solver_create_bordered: needs at least one constraint row");
+    return ERR_INVALID;
+  }
+  return create_csr("mpet_solver_create_bordered", csr64, dmask, nP2, nP1, F, prec64, B, m,
+                    omega, opt, handle);
 }
 
 extern "C" int mpet_solver_create(const MpetOpDesc* op64, const MpetOpDesc* op32,
@@ -621,6 +817,7 @@ extern "C" int mpet_solver_create(const MpetOpDesc* op64, const MpetOpDesc* op32
   S->F = op64->A + 1;
   S->nu = 3ll * op64->nP2;
   S->n = S->nu + (long long)S->F * op64->nP1;
+  S->nv = S->n;
   S->op64 = mpet_make_op<double>(op64);
   S->opt_mask64 = op64->dmask;
   if (op32) {
@@ -658,10 +855,18 @@ static int finish_create(MpetSolver* S, const MpetPrecDesc* prec64, const MpetPr
     mpet_solver_destroy(S);
     return rc;
   }
-  if ((rc = alloc_dev(S->allocs, &S->r64, S->n)) || (rc = alloc_dev(S->allocs, &S->g64, S->n)) ||
-      (rc = alloc_dev(S->allocs, &S->scal, 1)) || (rc = alloc_dev(S->allocs, &S->partial, MAX_PARTIALS))) {
+  if ((rc = alloc_dev(S->allocs, &S->r64, S->nv)) ||
+      (rc = alloc_dev(S->allocs, &S->g64, S->nv)) || (rc = alloc_dev(S->allocs, &S->scal, 1)) ||
+      (rc = alloc_dev(S->allocs, &S->partial, MAX_PARTIALS)) ||
+      (S->mb && (rc = alloc_dev(S->allocs, &S->yraw, S->n)))) {
     mpet_solver_destroy(S);
     return rc;
+  }
+  // multiplier rows of the Dirichlet part g of an initial guess are zero
+  if (S->mb && hipMemset(S->g64 + S->n, 0, sizeof(double) * S->mb) != hipSuccess) {
+    mpet_set_error("mpet_solver_create: HIP setup failed");
+    mpet_solver_destroy(S);
+    return ERR_HIP;
   }
   if (hipHostMalloc((void**)&S->scal_host, sizeof(double), hipHostMallocDefault) != hipSuccess) {
     mpet_set_error("mpet_solver_create: pinned allocation failed");
@@ -723,7 +928,8 @@ extern "C" int mpet_solver_solve(void* h, const double* b, double* x, MpetSolveS
                            S->opt.rtol, S->opt.maxit, S->opt.check_every,
                            S->opt.use_graphs ? &S->graph64 : nullptr, s, &it, &conv);
     if (dbg) fprintf(stderr, "[mpet] fp64 ||b||_M^-1 %.6e  ||r0|| %.6e  its %d\n", bn, beta1, it);
-    hipLaunchKernelGGL(add_f64_kernel, dim3(eb), dim3(NT), 0, s, n, S->m64.x, x);
+    hipLaunchKernelGGL(add_f64_kernel, dim3((int)((S->nv + NT - 1) / NT)), dim3(NT), 0, s,
+                       S->nv, S->m64.x, x);
     st.inner_iterations = it;
     st.outer_iterations = 1;
     st.converged = conv > 0;

@@ -9,9 +9,11 @@
 // by waterscapes_amd/refel.py:tables_float64():
 //   W[a][b][k][l]  (10x10x4x4)  (1/|K|) int dphi_a/dlam_k dphi_b/dlam_l
 //   V[b][c][k]     (10x4x4)     (1/|K|) int psi_c dphi_b/dlam_k
+//   M2[a][b]       (10x10)      (1/|K|) int phi_a phi_b
 #define MPET_TAB_W 0
 #define MPET_TAB_V 1600
-#define MPET_TAB_SIZE 1760
+#define MPET_TAB_M2 1760
+#define MPET_TAB_SIZE 1860
 
 namespace mpet {
 

@@ -291,8 +291,9 @@ class Expression:
 
 
 class FacetNormalTraction:
-    """s = sigma . n for a tensor-valued Expression sigma; produced by
-    ``tensor_expression * FacetNormal(mesh)`` (reference test :135-138)."""
+    """s = sigma . n for a tensor-valued Expression sigma, or sigma n for a
+    scalar one; produced by ``expression * FacetNormal(mesh)`` (reference
+    test_convergence_totalpressuresolver.py:135-138, test_donut.py:45)."""
 
     def __init__(self, sigma):
         self.sigma = sigma
@@ -305,8 +306,10 @@ class FacetNormalTraction:
         return self.sigma.depends_on_time()
 
     def eval_normal(self, X, normals, t=None):
-        s = self.sigma.eval(X, t)                      # n,3,3
-        return np.einsum("nij,nj->ni", s, normals)
+        s = np.asarray(self.sigma.eval(X, t))
+        if s.ndim == 1:                                # scalar * n (test_donut.py:45)
+            return s[:, None] * normals
+        return np.einsum("nij,nj->ni", s, normals)     # n,3,3
 
 
 class FacetNormal:
@@ -332,9 +335,17 @@ class MixedSpace:
     """[P2]^3 x P1^F on a mesh, canonical numbering (module docstring):
     F = A + 1 fields u, p_0, p_1..p_A of the total-pressure formulation, or
     (total_pressure=False) F = A fields u, p_1..p_A of the two-field
-    MPETSolver (mpet/mpetsolver.py:100-131, no nullspace multipliers)."""
+    MPETSolver (mpet/mpetsolver.py:100-131).
 
-    def __init__(self, mesh, A, dofmap=None, total_pressure=True):
+    Nullspace multipliers (the R-space components the reference appends to
+    the mixed element, mpettotalpressuresolver.py:182-202, mpetsolver.py:
+    107-128): u_nullspace adds one 6-vector (rigid motions), p_fields lists
+    the pressure fields (field numbering of the layout: the reference's
+    p[k]) that get one scalar multiplier each.  Their coefficients follow the
+    ndofs field DOFs; dim() counts both."""
+
+    def __init__(self, mesh, A, dofmap=None, total_pressure=True, u_nullspace=False,
+                 p_fields=()):
         self.mesh = mesh
         self.A = int(A)
         self.total_pressure = bool(total_pressure)
@@ -345,15 +356,21 @@ class MixedSpace:
         self.nu = 3 * self.nP2
         self.np = self.F * self.nP1
         self.ndofs = self.nu + self.np
+        self.u_nullspace = bool(u_nullspace)
+        self.p_fields = tuple(int(f) for f in p_fields)
+        if any(f < 0 or f >= self.F for f in self.p_fields):
+            raise ValueError("p_fields: pressure field index out of range")
+        self.mult_groups = ([6] if self.u_nullspace else []) + [1] * len(self.p_fields)
+        self.nmult = sum(self.mult_groups)
 
     def dim(self):
-        return self.ndofs
+        return self.ndofs + self.nmult
 
     def sub(self, i):
         return SubSpace(self, i)
 
     def num_sub_spaces(self):
-        return self.F + 1
+        return self.F + 1 + len(self.mult_groups)
 
     def network_sub(self, i):
         """Sub-space index of network pressure p_{i+1} (i 0-based): i + 2 with
@@ -365,6 +382,10 @@ class MixedSpace:
         pressure fields in order: p_0, p_1..p_A, or p_1..p_A)."""
         if i == 0:
             return np.arange(self.nu)
+        if i > self.F:
+            j = i - self.F - 1
+            off = self.ndofs + sum(self.mult_groups[:j])
+            return np.arange(off, off + self.mult_groups[j])
         return self.nu + np.arange(self.nP1) * self.F + (i - 1)
 
 
@@ -377,7 +398,7 @@ class SubSpace:
         return CollapsedSpace(self.parent, self.i)
 
     def dim(self):
-        return self.parent.nu if self.i == 0 else self.parent.nP1
+        return len(self.parent.field_index(self.i))
 
 
 class CollapsedSpace:
@@ -471,7 +492,7 @@ class Function:
         sp = self._space
         if not isinstance(sp, MixedSpace):
             return (self,)
-        return tuple(FieldView(self, i) for i in range(sp.F + 1))
+        return tuple(FieldView(self, i) for i in range(sp.num_sub_spaces()))
 
     def copy(self, deepcopy=True):
         d = self.data.copy() if isinstance(self.data, np.ndarray) else self.data.clone()
@@ -485,7 +506,9 @@ class Function:
             return a.reshape(-1, 3) if sp.i == 0 else a
         if i == 0:
             return a[: sp.nu].reshape(sp.nP2, 3)
-        return a[sp.nu:].reshape(sp.nP1, sp.F)[:, i - 1]
+        if i > sp.F:                                  # nullspace multipliers
+            return a[sp.field_index(i)]
+        return a[sp.nu: sp.ndofs].reshape(sp.nP1, sp.F)[:, i - 1]
 
 
 class FieldView:

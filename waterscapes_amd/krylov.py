@@ -69,6 +69,9 @@ def _bind():
     L.mpet_solver_create_csr.argtypes = [P, P, C.c_int, C.c_int, C.c_int, P, P,
                                          C.POINTER(C.c_void_p)]
     L.mpet_solver_create_csr.restype = C.c_int
+    L.mpet_solver_create_bordered.argtypes = [P, P, C.c_int, C.c_int, C.c_int, P, P, C.c_int,
+                                              C.c_double, P, C.POINTER(C.c_void_p)]
+    L.mpet_solver_create_bordered.restype = C.c_int
     L.mpet_solver_destroy.argtypes = [P]
     L.mpet_solver_destroy.restype = C.c_int
     L.mpet_solver_solve.argtypes = [P, P, P, P, P]
@@ -166,9 +169,15 @@ def compact_level(pattern_host, values):
 class KrylovSolver:
     """Owns the device preconditioner data and the native solver handle."""
 
-    def __init__(self, system, bc_dofs, config: SolverConfig = None, create_handle=True):
+    def __init__(self, system, bc_dofs, config: SolverConfig = None, create_handle=True,
+                 constraints=None):
         self.sys = system
         self.cfg = cfg = config or SolverConfig()
+        # nullspace multipliers (nullspace.Constraints): bordered FP64 system
+        self.cons = constraints
+        if constraints is not None and cfg.mode != "fp64":
+            raise ValueError("systems with nullspace multipliers are solved in FP64 "
+                             "(mode 'fp64') only")
         dev = system.device
         self.keep = []                       # tensors the native handle borrows
         if cfg.mode not in ("fp64", "mixed", "fp64-pc32"):
@@ -201,6 +210,11 @@ class KrylovSolver:
             raise ValueError("p0_weight must be 'robust' or 'reference'")
         if cfg.p0_weight == "robust" and not two_field:
             self.Pd[0] = 0.5 / system.mu + 1.0 / system.lmbda
+        if constraints is not None:
+            # constrained pressure fields: prec += p[k] q[k] dx
+            # (mpettotalpressuresolver.py:304-305, mpetsolver.py:274)
+            for f in constraints.fields:
+                self.Pd[f] += 1.0
         self._build_hierarchy()
         self._build_pressure()
         self.handle = None
@@ -211,7 +225,7 @@ class KrylovSolver:
         self.prec64 = self._prec_desc(0) if not pc32 else None
         self.prec32 = self._prec_desc(1) if pc32 else None
         use_csr = cfg.use_csr if cfg.use_csr is not None else (cfg.mode == "fp64")
-        if two_field:
+        if two_field or constraints is not None:
             use_csr = True
         self.csr_desc = None
         if use_csr and cfg.mode == "fp64":
@@ -230,6 +244,15 @@ class KrylovSolver:
                                  cfg.rtol, cfg.maxit, check_every, 1 if graphs else 0,
                                  cfg.outer_tol, cfg.inner_rtol, cfg.max_outer)
         h = C.c_void_p()
+        if constraints is not None:
+            self.op64 = self.op32 = None
+            check(L.mpet_solver_create_bordered(C.byref(self.csr_desc), ptr(system.dmask),
+                                                system.nP2, system.nP1, system.F,
+                                                C.byref(self.prec64), ptr(constraints.B),
+                                                constraints.m, constraints.omega,
+                                                C.byref(self.opt), C.byref(h)))
+            self.handle = h
+            return
         if two_field:
             self.op64 = self.op32 = None
             check(L.mpet_solver_create_csr(C.byref(self.csr_desc), ptr(system.dmask),
@@ -253,7 +276,13 @@ class KrylovSolver:
     def _build_hierarchy(self):
         s = self.sys
         rp, ci = s.pat_host["22"]
-        lap = s.prim_t["Lap"].cpu().numpy()
+        lap_t = s.prim_t["Lap"]
+        if self.cons is not None and self.cons.M2 is not None:
+            # displacement nullspace: prec += inner(u, v) dx
+            # (mpettotalpressuresolver.py:290-293); the hierarchy is scaled by mu
+            lap_t = lap_t + self.cons.M2 / s.mu
+        self.lap0 = lap_t
+        lap = lap_t.cpu().numpy()
         L = sp.csr_matrix((lap, ci, rp), shape=(s.nP2, s.nP2))
         m = np.zeros(s.nP2, dtype=bool)
         m[self.dir_nodes] = True
@@ -293,11 +322,11 @@ class KrylovSolver:
         if dtype in self.level_dev:
             return self.level_dev[dtype]
         s = self.sys
-        lap = s.prim_t["Lap"] if dtype == 0 else s.prim_t["Lap"].to(torch.float32)
+        lap = self.lap0 if dtype == 0 else self.lap0.to(torch.float32)
         u = self._upload_levels(self.levels, self.cinv, dtype,
                                 level0=(s.pat_dev["22"][0], s.pat_dev["22"][1], lap))
         if dtype == 1 and self.cfg.compact_level0:
-            c = compact_level(s.pat_host["22"], s.prim_t["Lap"])
+            c = compact_level(s.pat_host["22"], self.lap0)
             if c is not None:
                 u[0][0].update(c)
                 self.compact = c

@@ -28,6 +28,7 @@ import torch
 from .amg import AmgParams
 from .fem import Constant, Function, MixedSpace
 from .krylov import SolverConfig
+from .nullspace import pressure_fields
 from .solver import DirichletBC, FormDescriptor, Parameters, StepEngine
 
 
@@ -69,13 +70,14 @@ class MPETSolver(object):
 
     # -- set-up ---------------------------------------------------------------
     def create_function_spaces(self, mesh):
-        """[P2]^3 x P1^A (:100-131); the R-space multipliers of the nullspace
-        options are not implemented."""
-        if self.problem.u_has_nullspace or any(self.problem.p_has_nullspace):
-            raise NotImplementedError("nullspace (rigid-motion / constant) handling")
+        """[P2]^3 x P1^A (:100-131), plus the R-space multipliers of the
+        nullspace options (:107-128): a 6-vector for the rigid motions, one
+        scalar per pressure with p_has_nullspace, after the field DOFs."""
         if self.params["u_degree"] != 2 or self.params["p_degree"] != 1:
             raise NotImplementedError("the GPU backend implements P2 displacement / P1 pressures")
-        return MixedSpace(mesh, int(self.problem.params["J"]), total_pressure=False)
+        return MixedSpace(mesh, int(self.problem.params["J"]), total_pressure=False,
+                          u_nullspace=bool(self.problem.u_has_nullspace),
+                          p_fields=pressure_fields(self.problem))
 
     def create_variational_forms(self, include_preconditioner=False):
         """Fields and form descriptors of :134-298; returns (forms, prec,
@@ -83,8 +85,8 @@ class MPETSolver(object):
         VQ = self.create_function_spaces(self.problem.mesh)
         A = VQ.A
         dev = torch.device(self.params["device"])
-        self.up_ = Function(VQ, torch.zeros(VQ.ndofs, dtype=torch.float64, device=dev))
-        self.up = Function(VQ, torch.zeros(VQ.ndofs, dtype=torch.float64, device=dev))
+        self.up_ = Function(VQ, torch.zeros(VQ.dim(), dtype=torch.float64, device=dev))
+        self.up = Function(VQ, torch.zeros(VQ.dim(), dtype=torch.float64, device=dev))
         self.dt = Constant(self.params["dt"])
         self.a = FormDescriptor("a", terms=":196-201 lhs")
         self.a_robin = [FormDescriptor(f"a_robin[{i}]", terms=":252-253") for i in range(A)]

@@ -210,6 +210,14 @@ def mass_p1():
 
 
 @lru_cache(maxsize=None)
+def mass_p2():
+    """(1/|K|) int phi_a phi_b for the P2 basis (10 x 10)."""
+    phi = p2_basis()
+    return np.array([[float((phi[a] * phi[b]).mean()) for b in range(10)]
+                     for a in range(10)])
+
+
+@lru_cache(maxsize=None)
 def source_weights(test_degree, k):
     """Q[i, m] = (1/|K|) int v_i * phi^(k)_m  for the test space P_test_degree
     (1 or 2) and the P_k interpolant of a source (Expression(degree=k))."""
@@ -249,4 +257,5 @@ def tables_float64():
     with fixed offsets (see csrc/mpet_common.h)."""
     W = grad_weights_p2p2().reshape(-1)      # 1600
     V = div_weights_p2p1().reshape(-1)       # 160
-    return np.ascontiguousarray(np.concatenate([W, V]))
+    M2 = mass_p2().reshape(-1)               # 100
+    return np.ascontiguousarray(np.concatenate([W, V, M2]))

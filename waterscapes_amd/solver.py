@@ -21,6 +21,11 @@ direct_solver=True (the reference default, LU with MUMPS, :330-411) has no
 sparse LU on the GPU: it runs the same FP64 Krylov solve to rtol 1e-12,
 which reproduces the LU solution to round-off level; row replacement and
 symmetric elimination give the same solution (tested on the oracle).
+
+Nullspaces (problem.u_has_nullspace, p_has_nullspace; :179-202, 285-306):
+the multipliers of the reference's R spaces follow the field DOFs of up
+(up.split() returns u, p_0..p_A, then the multipliers), and the bordered
+system runs in FP64 MINRES (waterscapes_amd/nullspace.py).
 """
 from __future__ import annotations
 
@@ -35,6 +40,7 @@ from .facets import FacetSet, FacetTerm
 from .fem import Constant, Function, MixedSpace
 from .amg import AmgParams
 from .krylov import KrylovSolver, SolverConfig
+from .nullspace import Constraints, has_nullspace, pressure_fields
 from .problem import resolved_params
 
 NEUMANN_MARKER = 1
@@ -140,8 +146,9 @@ class StepEngine:
         params, cellK = resolved_params(problem)
         self.params = params
         A = space.A
-        if problem.u_has_nullspace or any(problem.p_has_nullspace):
-            raise NotImplementedError("nullspace (rigid-motion / constant) handling")
+        self.nullspace = has_nullspace(problem)
+        if self.nullspace and batched:
+            raise NotImplementedError("batches of instances with nullspace multipliers")
         # exterior-facet terms (markers 1 = Neumann, 2 = Robin, :309-324)
         self.t0 = float(problem.time)
         self.robin = []
@@ -177,8 +184,15 @@ class StepEngine:
                                         problem.continuity_boundary_markers[i], 0))
         self.bc_dofs_host = np.concatenate([bc.dofs for bc in self.bcs]).astype(np.int64)
         self.bc_dofs = torch.as_tensor(self.bc_dofs_host).to(self.device)
+        self.constraints = None
+        if self.nullspace:
+            if space.nmult != 6 * bool(problem.u_has_nullspace) + len(pressure_fields(problem)):
+                raise ValueError("the function space's multipliers do not match the "
+                                 "problem's nullspace flags")
+            self.constraints = Constraints(self.system, bool(problem.u_has_nullspace),
+                                           pressure_fields(problem))
         self.krylov = KrylovSolver(self.system, [bc.dofs for bc in self.bcs], config,
-                                   create_handle=not batched)
+                                   create_handle=not batched, constraints=self.constraints)
         self.config = config
         # sources
         nu, F = space.nu, space.F
@@ -211,7 +225,9 @@ class StepEngine:
             fi = space.network_sub(i) - 1
             gdof = torch.as_tensor((nu + F * fs.p1 + fi).astype(np.int32)).to(self.device)
             self.robin_terms.append((i, FacetTerm(fs, 1, nu, F, fi, 1), gdof))
-        self.b = torch.empty(space.ndofs, dtype=torch.float64, device=self.device)
+        # multiplier rows of b (after the ndofs field rows) stay 0: the
+        # constraints are homogeneous (the forms have no multiplier rhs)
+        self.b = torch.zeros(space.dim(), dtype=torch.float64, device=self.device)
         self.stats = []
 
     def bc_values(self, t=None):
@@ -342,10 +358,11 @@ class MPETTotalPressureSolver(object):
             raise NotImplementedError("the GPU backend implements P2 displacement / P1 pressures")
         mesh = self.problem.mesh
         A = int(self.problem.params["J"])
-        VW = MixedSpace(mesh, A)
+        VW = MixedSpace(mesh, A, u_nullspace=bool(self.problem.u_has_nullspace),
+                        p_fields=pressure_fields(self.problem))
         dev = torch.device(self.params["device"])
-        up_ = Function(VW, torch.zeros(VW.ndofs, dtype=torch.float64, device=dev))
-        up = Function(VW, torch.zeros(VW.ndofs, dtype=torch.float64, device=dev))
+        up_ = Function(VW, torch.zeros(VW.dim(), dtype=torch.float64, device=dev))
+        up = Function(VW, torch.zeros(VW.dim(), dtype=torch.float64, device=dev))
         F = FormDescriptor("F", terms=":267-274")
         L0 = FormDescriptor("L0", terms=":313")
         L1 = [FormDescriptor(f"L1[{i}]", terms=":322") for i in range(A)]
@@ -355,7 +372,8 @@ class MPETTotalPressureSolver(object):
 
     def _config(self, direct):
         p = self.params
-        if direct:
+        if direct or (has_nullspace(self.problem) and p["precision"] != "fp64"):
+            # the bordered systems of nullspace multipliers run in FP64 only
             return SolverConfig(mode="fp64", rtol=1e-12, maxit=int(p["maxit"]),
                                 p0_weight=p["p0_weight"])
         return SolverConfig(mode=p["precision"], rtol=float(p["krylov_rtol"]),
