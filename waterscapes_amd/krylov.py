@@ -211,10 +211,14 @@ class KrylovSolver:
         if cfg.p0_weight == "robust" and not two_field:
             self.Pd[0] = 0.5 / system.mu + 1.0 / system.lmbda
         if constraints is not None:
-            # constrained pressure fields: prec += p[k] q[k] dx
-            # (mpettotalpressuresolver.py:304-305, mpetsolver.py:274)
+            # constrained pressure fields: the reference adds p[k] q[k] dx
+            # (mpettotalpressuresolver.py:304-305, mpetsolver.py:274) so that
+            # a pure-Neumann block without a mass term stays definite; a block
+            # that already has one keeps its weight (adding the bare mass to
+            # the "robust" p_0 weight ~1/(2 mu) would swamp it)
             for f in constraints.fields:
-                self.Pd[f] += 1.0
+                if not self.Pd[f] > 0.0:
+                    self.Pd[f] = 1.0
         self._build_hierarchy()
         self._build_pressure()
         self.handle = None
