@@ -437,6 +437,119 @@ def main_two_field(args):
     print(json.dumps(out))
 
 
+def make_nullspace_solver(n, A, seed=2024):
+    """The 48^3 A = 4 network workload without the displacement Dirichlet
+    condition: traction-free everywhere, rigid motions removed by the
+    multipliers of u_has_nullspace (mpettotalpressuresolver.py:179-202,
+    285-293); network pressures oscillating on z = 1.  FP64 MINRES to rtol
+    1e-10 on the bordered system (iterative path)."""
+    from waterscapes_amd import (MPETProblem, MPETTotalPressureSolver, Constant, Expression,
+                                 UnitCubeMesh, CompiledSubDomain)
+    mesh = UnitCubeMesh(n, n, n)
+    time_c = Constant(0.0)
+    params = workload_params(A)
+    Kc = lognormal_cells(mesh.num_cells(), A, seed)
+    params["K"] = tuple(Kc[:, j] for j in range(A))
+    problem = MPETProblem(mesh, time_c, params=params)
+    problem.u_has_nullspace = True
+    problem.p_bar = [Expression(f"{P0[j]} + 5.0*sin(2.0*M_PI*t)", t=time_c, degree=1)
+                     for j in range(A)]
+    for j in range(A):
+        CompiledSubDomain("on_boundary && near(x[2], 1.0)").mark(
+            problem.continuity_boundary_markers[j], 0)
+    return MPETTotalPressureSolver(problem, dict(dt=0.0125, theta=1.0, T=1e9,
+                                                 direct_solver=False, precision="fp64",
+                                                 krylov_rtol=1e-10))
+
+
+def nullspace_cpu_baseline(n_sample, A, steps, budget_s=20.0):
+    """The oracle's bordered LU (u nullspace) on the same workload at a
+    reduced mesh, per step with the factorisation amortised over 80 steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle import mpet_oracle as O
+    from waterscapes_amd.fem import MixedSpace
+    from waterscapes_amd.mesh import UnitCubeMesh
+    from conftest import oracle_mesh
+    mesh = UnitCubeMesh(n_sample, n_sample, n_sample)
+    space = MixedSpace(mesh, A, u_nullspace=True)
+    m = oracle_mesh(space)
+    params = workload_params(A)
+    Kc = lognormal_cells(mesh.num_cells(), A, 7)
+    mm = np.full(mesh.num_facets(), 2**63 - 1, dtype=np.uint64)
+    cm = np.full(mesh.num_facets(), 2**63 - 1, dtype=np.uint64)
+    mid = mesh.facet_midpoints()
+    cm[mesh.exterior & (np.abs(mid[:, 2] - 1) < 1e-12)] = 0
+    pb = [lambda X, t, j=j: np.full(X.shape[0], P0[j] + 5.0 * np.sin(2 * np.pi * t))
+          for j in range(A)]
+    t0 = time.perf_counter()
+    prob = O.OracleProblem(m, params, dict(dt=0.0125, theta=1.0, T=1e9), K_cells=Kc,
+                           u_bar=lambda X, t: np.zeros((X.shape[0], 3)), p_bar=pb,
+                           momentum_markers=mm, continuity_markers=[cm] * A, refine_steps=0,
+                           u_nullspace=True)
+    gen = prob.solve_direct(np.zeros(space.dim()))
+    next(gen)
+    setup = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    k = 0
+    while k < steps and time.perf_counter() - t1 < budget_s:
+        next(gen)
+        k += 1
+    per_step = (time.perf_counter() - t1) / max(k, 1)
+    return dict(ndofs=space.dim(), per_step=per_step, setup=setup, steps=k)
+
+
+def main_nullspace(args):
+    """Total-pressure steps/s with a displacement nullspace (bordered FP64
+    MINRES, six rigid-motion multipliers), 48^3 A = 4 traction-free."""
+    import torch
+    K, W = min(args.steps, 10), args.warmup
+    t_setup = time.perf_counter()
+    s = make_nullspace_solver(args.n, args.A)
+    eng = s.engine()
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t_setup
+    total, lat, states, its, _ = run_steps(s, W, K, keep=True)
+    sysd = eng.system
+    B = eng.constraints.rows()
+    x = states[-1]
+    cons = float((B @ x[: sysd.n]).abs().max() / x[: sysd.n].abs().max())
+    xr = torch.randn(sysd.n, device="cuda", dtype=torch.float64)
+    lpr = eng.krylov.csr_desc.lpr
+    ms = time_kernel(lambda: sysd.csr_apply(xr, lpr=lpr, masked=False), reps=20)
+    nbytes = csr_bytes(sysd)
+    ach = nbytes / (ms * 1e-3) / 1e9
+    out = {"metric": "MPET steps/s with a displacement nullspace (bordered FP64 MINRES, "
+                     "rtol 1e-10)",
+           "value": round(K / total, 4), "unit": "steps/s", "n_gpus": 1, "steps": K,
+           "warmup": W, "ms_per_step": round(1e3 * total / K, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"MPET total pressure, unit cube {args.n}^3 x 6 tets, "
+                                  f"A={args.A}, traction-free, u_has_nullspace, theta=1, "
+                                  "dt=0.0125", "dofs": int(sysd.n),
+                      "multipliers": int(eng.constraints.m)},
+           "dof_per_s": round(sysd.n * K / total, 1), "iterations": its,
+           "max_constraint_residual_rel": cons,
+           "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
+                          "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
+           "setup_s": round(setup, 2),
+           "roofline": {"bound": "hbm", "kernel": f"csr_spmv_kernel<double,{lpr}> (A of the "
+                                                  "bordered system)",
+                        "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
+                        "frac": round(ach / 8000.0, 4), "traffic": None,
+                        "bytes_per_launch": int(nbytes), "ms_per_launch": round(ms, 4)}}
+    if not args.no_cpu_baseline:
+        cb = nullspace_cpu_baseline(args.cpu_n, args.A, 20)
+        per_full = (cb["per_step"] + cb["setup"] / 80.0) * sysd.n / cb["ndofs"]
+        out["cpu_baseline"] = {
+            "value": round(1.0 / per_full, 6), "unit": "steps/s (extrapolated by DOF count)",
+            "cores": 1, "kind": "port",
+            "sample": f"FP64 oracle bordered LU (numpy/scipy SuperLU, u nullspace) at "
+                      f"{args.cpu_n}^3 ({cb['ndofs']} DOFs), {cb['steps']} steps after "
+                      f"factorisation, {cb['per_step']:.3f} s/step, setup {cb['setup']:.1f} s "
+                      "amortised over 80 steps"}
+    print(json.dumps(out))
+
+
 def main_probe(args):
     """Short run for hardware-counter passes: 5 launches of the FP32
     structured operator apply of the 48^3 A = 4 workload and 5 launches of a
@@ -478,7 +591,8 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-reference", action="store_true")
     ap.add_argument("--tf-n", type=int, default=32)
-    ap.add_argument("--workload", choices=("single", "batch", "serve", "probe", "two_field"),
+    ap.add_argument("--workload", choices=("single", "batch", "serve", "probe", "two_field",
+                                           "nullspace"),
                     default="single")
     ap.add_argument("--instances", type=int, default=96)
     ap.add_argument("--sample", type=int, default=8)
@@ -499,6 +613,8 @@ def main():
         return main_serve(args)
     if args.workload == "probe":
         return main_probe(args)
+    if args.workload == "nullspace":
+        return main_nullspace(args)
     if args.workload == "two_field":
         return main_two_field(args)
     import torch
