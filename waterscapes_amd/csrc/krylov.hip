@@ -507,7 +507,9 @@ struct MpetSolver {
   double* yraw = nullptr;                 // A z of the field rows
 };
 
-static int bgrid(long long n) { return grid_for(n, NT * 4, MAX_PARTIALS / (MAXB + 1)); }
+// Grid of the bordered kernels: one partial per block for each of the m + 1
+// reductions they produce.
+static int bgrid(long long n, int m) { return grid_for(n, NT * 4, MAX_PARTIALS / (m + 1)); }
 
 // z = M^-1 r.  Mode 2 applies the FP32 preconditioner to FP64 vectors
 // (convert, V-cycles in FP32, convert back).
@@ -573,7 +575,7 @@ static void minres_iter(MpetSolver* S, Minres<T>& M, const Precond<T>& P, bool c
       launch_csr_spmv<double>(S->csr_rows, S->csr_rp, S->csr_ci, S->csr_v, M.z,
                               (const double*)nullptr, (const uint8_t*)nullptr, S->yraw,
                               S->csr_lpr, s);
-      const int bg = bgrid(S->n);
+      const int bg = bgrid(S->n, S->mb);
       hipLaunchKernelGGL(k1b_kernel, dim3(bg), dim3(NT), 0, s, S->n, S->mb, S->B,
                          (const double*)S->yraw, (const double*)M.z, (const double*)Ra,
                          S->opt_mask64, M.y, M.v, M.st, skip, M.partial);
@@ -694,7 +696,7 @@ static void op_residual64(MpetSolver* S, const double* b, const double* x, doubl
   else
     launch_op_apply<double>(S->op64, x, b, r, s);
   if (S->mb) {
-    const int bg = bgrid(S->n);
+    const int bg = bgrid(S->n, S->mb);
     hipLaunchKernelGGL(bres_kernel, dim3(bg), dim3(NT), 0, s, S->n, S->mb, S->B, x,
                        S->opt_mask64, r, S->partial);
     hipLaunchKernelGGL(bresf_kernel, dim3(1), dim3(NT), 0, s, (const double*)S->partial, bg,
