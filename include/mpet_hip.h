@@ -206,6 +206,39 @@ int mpet_assemble_primitives(const MpetMeshDesc* mesh, const MpetPatterns* pat,
                              const double* Kcell, int A, double mu,
                              const MpetPrimitives* out, hipStream_t stream);
 
+/* A-posteriori error indicators of the two-field MPET step
+ * (AdaptiveMPETSolver, mpet/adaptivempetsolver.py:82-185).  Mesh arrays as
+ * in MpetMeshDesc (device); x, xp the current and previous two-field states
+ * (u then p_1..p_A, F = A pressure fields per P1 node); constant material
+ * parameters (S row-major, stride MPET_MAXF).  Cell quadrature: nq
+ * barycentric points qbary (nq x 4) with weights qw normalised to sum 1
+ * (exact for the integrand degree); sources as values at the P_k nodes of
+ * every cell (Expression(degree=k) semantics): f, fp (f at the previous
+ * time, the reference's f_) nC x nkf x 3, g nC x nkg x A (null = 0), with
+ * the P_k basis at the quadrature points Lf (nq x nkf), Lg (nq x nkg). */
+typedef struct {
+  int32_t nC, nP2, nP1, A;
+  const double* coords;
+  const int32_t *cells, *cell_p2;
+  double mu, lmbda, dt;
+  double alpha[MPET_MAXF], c[MPET_MAXF], K[MPET_MAXF], S[MPET_MAXF * MPET_MAXF];
+  int32_t nq;
+  const double *qbary, *qw;
+  int32_t nkf, nkg;
+  const double *Lf, *Lg;
+  const double *f, *fp, *g;
+} MpetIndDesc;
+
+/* Per-cell indicators out (4 x nC, device): rows zeta_p, zeta_u, zeta_u_dt
+ * (eta_p_K_m, eta_u_K_m, eta_u_dt_K_m of estimate_error_at_t, :168-171) and
+ * the cell contributions to R4p (:146-153).  fac (nF x 4: cell a, local
+ * face of a, cell b, local face of b) lists the interior facets, cell_f
+ * (nC x 4) the interior facet of each local face (-1 on the boundary);
+ * tmp holds 3 nF doubles.  Deterministic. */
+int mpet_error_indicators(const MpetIndDesc* d, const double* x, const double* xp,
+                          const int32_t* fac, int nF, const int32_t* cell_f, double* tmp,
+                          double* out, hipStream_t stream);
+
 /* P2 mass int phi_a phi_b on the P2 node pattern (nnz22 doubles, device),
  * the vector mass of the u block that the reference adds to the
  * preconditioner with a displacement nullspace (inner(u, v) dx,

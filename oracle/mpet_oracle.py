@@ -27,7 +27,8 @@ It also restates the two-field MPETSolver (mpet/mpetsolver.py): forms
 multipliers of both solvers (rigid motions of mpet/rm_basis_L2.py:10-74,
 constraint terms mpettotalpressuresolver.py:285-303, mpetsolver.py:203-215)
 as a bordered system solved by LU (OracleProblem(u_nullspace=...,
-p_nullspace=...)).
+p_nullspace=...)); and the a-posteriori error indicators of
+AdaptiveMPETSolver (mpet/adaptivempetsolver.py:82-185, error_indicators).
 
 Canonical numbering (inputs to every function): u DOF = 3*a + j for P2 node a,
 component j; p_f DOF = 3*nP2 + F*c + f for P1 node c, field f = 0..F-1 with
@@ -634,6 +635,155 @@ def constraint_rows(coords, cells, cell_p2, cell_p1, lay, Z, fields):
         loc = np.einsum("q,c,qa->ca", qw, det, psi)
         rows.append(scatter_vector(lay.n, lay.pdofs(cell_p1, f), loc))
     return sp.csr_matrix(np.array(rows)) if rows else sp.csr_matrix((0, lay.n))
+
+
+# ---------------------------------------------------------------------------
+# a-posteriori error indicators (AdaptiveMPETSolver, two-field)
+# ---------------------------------------------------------------------------
+
+
+def p2_ref_hess():
+    """(10, 3, 3) constant reference Hessians of the P2 basis."""
+    dlam = np.array([[-1, -1, -1], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+    H = [4 * np.outer(dlam[i], dlam[i]) for i in range(4)]
+    H += [4 * (np.outer(dlam[i], dlam[j]) + np.outer(dlam[j], dlam[i])) for (i, j) in EDGES]
+    return np.array(H)
+
+
+def error_indicators(m, params, x, xp, dt, f=None, fp=None, g=None, f_degree=0, g_degree=0,
+                     t=0.0):
+    """Cell indicators of mpet/adaptivempetsolver.py:119-153 for two-field
+    states x (current) and xp (previous), layout F = A:
+      zeta_p = sum_j h^2 ||RK_p,j||^2_K + sum_F avg(h)/2 ||RE_p,j||^2_F
+      zeta_u = h^2 ||RK_u||^2_K + sum_F avg(h)/2 ||RE_u||^2_F,  zeta_u_dt alike
+    with the residuals of :121-133 (h = longest edge, F interior facets) and
+    the cell parts of R4p (:146-153).  Sources are callables (X, t) with
+    Expression(degree) semantics (P_k interpolation per cell).  Cell
+    integrals by Stroud conical-product quadrature, facet integrals by a
+    collapsed Gauss rule; second derivatives from the reference Hessians.
+    Returns (zeta_p, zeta_u, zeta_u_dt, r4) per cell."""
+    A = int(params["J"])
+    mu, lmbda = lame(params["E"], params["nu"])
+    alpha = np.asarray(params["alpha"], float)
+    c = np.asarray(params["c"], float)
+    K = np.asarray(params["K"], float)
+    S = np.asarray(params["S"], float)
+    coords, cells, cp2 = m["coords"], m["cells"], m["cell_p2"]
+    nC = cells.shape[0]
+    nu = 3 * m["nP2"]
+    lay = Layout(m["nP2"], m["nP1"], A, A)
+    J, det, Jinv = cell_geometry(coords, cells)
+    v = coords[cells]
+    h = np.max([np.linalg.norm(v[:, i] - v[:, j], axis=1)
+                for i in range(4) for j in range(i + 1, 4)], axis=0)
+    vol = det / 6.0
+
+    def ufield(z):
+        return z[:nu].reshape(-1, 3)[cp2]                        # nC,10,3
+
+    def pfield(z):
+        return z[nu:lay.n].reshape(-1, A)[cells]                 # nC,4,A
+
+    Href = p2_ref_hess()
+    # physical Hessian: J^-T H_ref J^-1
+    Hph = np.einsum("cki,akl,clj->caij", Jinv, Href, Jinv)      # nC,10,3,3
+    g1 = np.einsum("ckm,ak->cam", Jinv, P1_REF_GRADS)           # nC,4,3
+
+    def divsig(U):
+        lap = np.einsum("caii,cak->ck", Hph, U)
+        gd = np.einsum("caij,caj->ci", Hph, U)
+        return mu * lap + (mu + lmbda) * gd
+
+    u, up_ = ufield(x), ufield(xp)
+    p, pp = pfield(x), pfield(xp)
+    gp = np.einsum("caj,cam->cjm", p, g1)                        # nC,A,3
+    gpp = np.einsum("caj,cam->cjm", pp, g1)
+    Cu = divsig(u) - np.einsum("j,cjm->cm", alpha, gp)
+    Cup = divsig(up_) - np.einsum("j,cjm->cm", alpha, gpp)
+    deg = 2 * max(f_degree if (f is not None or fp is not None) else 0,
+                  g_degree if g is not None else 0, 1)
+    qp, qw = tet_rule(deg)
+    X = coords[cells[:, 0]][:, None, :] + np.einsum("cij,qj->cqi", J, qp)
+
+    def interp(fun, k, ncomp):
+        nodes = lagrange_nodes(k)
+        Xn = coords[cells[:, 0]][:, None, :] + np.einsum("cij,nj->cni", J, nodes)
+        vals = np.asarray(fun(Xn.reshape(-1, 3), t), float).reshape(nC, len(nodes), -1)
+        L = lagrange_eval(k, qp) if k > 0 else np.ones((len(qp), 1))
+        return np.einsum("qm,cmk->cqk", L, vals)[:, :, :ncomp]
+
+    fq = interp(f, f_degree, 3) if f is not None else np.zeros((nC, len(qp), 3))
+    fpq = interp(fp, f_degree, 3) if fp is not None else np.zeros((nC, len(qp), 3))
+    RK = fq + Cu[:, None, :]
+    RKp = fpq + Cup[:, None, :]
+    wq = det[:, None] * qw[None, :]
+    zu = h ** 2 * np.einsum("cq,cqi,cqi->c", wq, RK, RK)
+    Rd = (RK - RKp) / dt
+    zudt = h ** 2 * np.einsum("cq,cqi,cqi->c", wq, Rd, Rd)
+    # pressure residuals at the quadrature points
+    psi = p1_ref_values(qp)                                      # nq,4
+    pq = np.einsum("qa,caj->cqj", psi, p)
+    ppq = np.einsum("qa,caj->cqj", psi, pp)
+    gr = np.einsum("ckm,qak->cqam", Jinv, p2_ref_grads(qp))      # nC,nq,10,3
+    divu = np.einsum("cqai,cai->cq", gr, u - up_)
+    gq = interp(g, g_degree, A) if g is not None else np.zeros((nC, len(qp), A))
+    RKp_ = (-gq - c[None, None, :] * (pq - ppq) / dt - alpha[None, None, :] * divu[:, :, None] / dt
+            - np.einsum("ij,cqj->cqj", S, pq) + np.einsum("ij,cqi->cqj", S, pq))
+    zp = h ** 2 * np.einsum("cq,cqj,cqj->c", wq, RKp_, RKp_)
+    dpq = pq - ppq
+    r4 = np.einsum("j,cjm,cjm->c", K, gp - gpp, gp - gpp) * vol
+    for j in range(A):
+        for i in range(A):
+            if S[i, j] > 3.0e-16:
+                r4 += np.einsum("cq,cq->c", wq, S[i, j] * (dpq[:, :, j] - dpq[:, :, i]) * dpq[:, :, j])
+    # interior facets
+    cf = m["cell_facets"].reshape(-1)
+    ext = m["exterior"]
+    slots = np.nonzero(~ext[cf])[0]
+    order = np.argsort(cf[slots], kind="stable")
+    pair = slots[order].reshape(-1, 2)
+    tq, tw = tri_rule(2)
+    for sa, sb in pair:
+        ca, la, cb = sa // 4, sa % 4, sb // 4
+        P = coords[cells[ca, list(FACES[la])]]
+        e1, e2 = P[1] - P[0], P[2] - P[0]
+        nrm = np.cross(e1, e2)
+        area = 0.5 * np.linalg.norm(nrm)
+        nrm /= np.linalg.norm(nrm)
+        if np.dot(coords[cells[ca, la]] - P[0], nrm) > 0:
+            nrm = -nrm
+        havg = 0.5 * (h[ca] + h[cb])
+        Xq = P[0] + np.outer(tq[:, 0], e1) + np.outer(tq[:, 1], e2)
+        wts = tw * 2.0 * area
+
+        def local(cell, z):
+            ref = (Xq - coords[cells[cell, 0]]) @ Jinv[cell].T       # nq,3 reference coords
+            G = np.einsum("km,qak->qam", Jinv[cell], p2_ref_grads(ref))
+            U = ufield(z)[cell]
+            Gu = np.einsum("ai,qaj->qij", U, G)
+            sig = mu * (Gu + np.transpose(Gu, (0, 2, 1))) + \
+                lmbda * np.einsum("qii->q", Gu)[:, None, None] * np.eye(3)
+            pv = p1_ref_values(ref) @ pfield(z)[cell]                # nq,A
+            gpl = np.einsum("aj,am->jm", pfield(z)[cell], g1[cell])  # A,3
+            return sig, pv, gpl
+
+        res = []
+        for z in (x, xp):
+            sa_, pa_, ga_ = local(ca, z)
+            sb_, pb_, gb_ = local(cb, z)
+            RE = (np.einsum("j,qj->q", alpha, pa_ - pb_)[:, None] * nrm
+                  - np.einsum("qij,j->qi", sa_ - sb_, nrm))
+            REp = -K * ((ga_ - gb_) @ nrm)
+            res.append((RE, REp))
+        (RE, REp), (REo, _) = res
+        Fu = havg * np.einsum("q,qi,qi->", wts, RE, RE)
+        Fd = havg * np.einsum("q,qi,qi->", wts, (RE - REo) / dt, (RE - REo) / dt)
+        Fp = havg * area * np.sum(REp ** 2)
+        for cell in (ca, cb):
+            zu[cell] += 0.5 * Fu
+            zudt[cell] += 0.5 * Fd
+            zp[cell] += 0.5 * Fp
+    return zp, zu, zudt, r4
 
 
 def bordered(A, B, C=None):

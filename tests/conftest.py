@@ -40,7 +40,7 @@ def oracle_mesh(space):
                 cell_p2=dm.cell_p2.astype(np.int64), cell_p1=dm.cell_p1.astype(np.int64),
                 nP2=dm.nP2, nP1=dm.nP1, x2=dm.x2, facets=mesh.facets,
                 facet_cell=mesh.facet_cell, facet_local=mesh.facet_local,
-                exterior=mesh.exterior)
+                exterior=mesh.exterior, cell_facets=mesh.cell_facets)
 
 
 CFG1 = dict(J=2, E=1e4, nu=0.35, alpha=(0.5, 0.5), c=(1e-4, 1e-4), K=(1e-3, 1e-4),

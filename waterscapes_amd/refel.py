@@ -252,6 +252,40 @@ def face_local_p2_nodes(face):
     return list(vs) + edge_ids
 
 
+def bpoly_eval(p, bary):
+    """Values of a barycentric polynomial at points (npts, n) of
+    barycentric coordinates."""
+    bary = np.asarray(bary, dtype=np.float64)
+    out = np.zeros(bary.shape[0])
+    for e, v in p.c.items():
+        out += float(v) * np.prod(bary ** np.asarray(e, dtype=np.float64), axis=1)
+    return out
+
+
+@lru_cache(maxsize=None)
+def gm_rule(degree):
+    """Grundmann-Moeller rule on the tetrahedron (A. Grundmann, H. M.
+    Moeller, SIAM J. Numer. Anal. 15 (1978) 282-290), exact for polynomials
+    of degree <= 2 s + 1 >= degree.  Returns (bary (nq, 4), w (nq,)) with the
+    weights normalised to sum 1, i.e. (1/|K|) int_K f ~ sum_q w_q f(x_q).
+    Some weights are negative (as in the published rule)."""
+    n = 3
+    s = max(0, (int(degree) - 1 + 1) // 2)
+    d = 2 * s + 1
+    pts, wts = [], []
+    for i in range(s + 1):
+        w = Fraction((-1) ** i * (d + n - 2 * i) ** d,
+                     2 ** (2 * s) * factorial(i) * factorial(d + n - i))
+        for beta in product(range(s - i + 1), repeat=n + 1):
+            if sum(beta) != s - i:
+                continue
+            pts.append([Fraction(2 * b + 1, d + n - 2 * i) for b in beta])
+            wts.append(w)
+    tot = sum(wts)
+    return (np.array([[float(x) for x in p] for p in pts]),
+            np.array([float(w / tot) for w in wts]))
+
+
 def tables_float64():
     """All tables needed by the device kernels, packed in one float64 array
     with fixed offsets (see csrc/mpet_common.h)."""
