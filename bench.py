@@ -550,6 +550,88 @@ def main_nullspace(args):
     print(json.dumps(out))
 
 
+def main_indicators(args):
+    """Two-field error indicators (AdaptiveMPETSolver, cell residuals and
+    interior-facet jumps) per second on the two-field workload after two
+    steps; the indicator kernels timed with HIP events."""
+    import torch
+    from waterscapes_amd import Expression
+    from waterscapes_amd.adaptive import ErrorEstimator
+    n = args.tf_n
+    s = make_two_field_solver(n)
+    prob = s.problem
+    prob.f = Expression(("x[0]*x[1] + t", "sin(x[2])", "x[0]*x[0]"), t=prob.time, degree=2)
+    f_ = Expression(("x[0]*x[1]", "sin(x[2])", "x[0]*x[0]"), degree=2)
+    gen = s.solve()
+    next(gen)
+    next(gen)                                  # up = step 2, up_ = step 1
+    est = ErrorEstimator(prob, s, f_)
+    est.indicators()
+    torch.cuda.synchronize()
+    K = max(args.steps, 5)
+    L = __import__("waterscapes_amd._lib", fromlist=["lib"])
+    d = est.desc
+    x, xp = s.up.data, s.up_.data
+
+    def launch():
+        L.check(L.lib().mpet_error_indicators(C.byref(d), L.ptr(x), L.ptr(xp), L.ptr(est.fac),
+                                              est.nF, L.ptr(est.cell_f), L.ptr(est.tmp),
+                                              L.ptr(est.out), L.stream_ptr()))
+    ms = time_kernel(launch, reps=K)
+    t0 = time.perf_counter()
+    for _ in range(K):
+        est.estimate_error_at_t()
+    total = time.perf_counter() - t0
+    nC = prob.mesh.num_cells()
+    # algorithmic bytes: per cell the 10 P2 and 4 P1 nodes of both states,
+    # geometry, the P_k source values at the cell's nodes, 4 outputs; per
+    # interior facet the two cells' nodes again and 3 outputs + the gather
+    A = 2
+    per_cell = (2 * (30 + 4 * A) * 8 + 4 * 3 * 8 + 14 * 4 + 2 * 6 * 3 * 8 + 4 * 8)
+    per_facet = (2 * (2 * (30 + 4 * A) * 8 + 4 * 3 * 8 + 14 * 4) + 4 * 4 + 3 * 8 * 2)
+    nbytes = nC * per_cell + est.nF * per_facet
+    ach = nbytes / (ms * 1e-3) / 1e9
+    out = {"metric": "two-field error indicators per second (cells/s, GPU kernels)",
+           "value": round(nC / (ms * 1e-3), 1), "unit": "cells/s", "n_gpus": 1, "steps": K,
+           "warmup": 1, "ms_per_step": round(ms, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"two-field MPET, unit cube {n}^3 x 6 tets, A=2, indicators "
+                                  "after two steps", "cells": nC, "interior_facets": est.nF},
+           "estimate_error_at_t_ms": round(1e3 * total / K, 3),
+           "roofline": {"bound": "hbm", "kernel": "ind_cell_kernel + ind_facet_kernel + "
+                                                  "ind_gather_kernel",
+                        "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
+                        "frac": round(ach / 8000.0, 4), "traffic": None,
+                        "bytes_per_launch": int(nbytes), "ms_per_launch": round(ms, 4)}}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle import mpet_oracle as O
+        from conftest import oracle_mesh
+        from waterscapes_amd.fem import MixedSpace
+        from waterscapes_amd.mesh import UnitCubeMesh
+        ns = min(args.cpu_n, 6)             # the oracle's facet loop is per-facet Python
+        mesh = UnitCubeMesh(ns, ns, ns)
+        sp_ = MixedSpace(mesh, 2, total_pressure=False)
+        m = oracle_mesh(sp_)
+        rng = np.random.default_rng(1)
+        xs, xps = rng.standard_normal(sp_.ndofs), rng.standard_normal(sp_.ndofs)
+        fcb = lambda X, t: np.stack([X[:, 0] * X[:, 1] + t, np.sin(X[:, 2]), X[:, 0] ** 2], 1)
+        t0 = time.perf_counter()
+        reps = 0
+        while reps < 1 or time.perf_counter() - t0 < 5.0:
+            O.error_indicators(m, TF_PARAMS, xs, xps, 0.0125, f=fcb, fp=fcb, f_degree=2)
+            reps += 1
+            if reps >= 20:
+                break
+        per = (time.perf_counter() - t0) / reps
+        out["cpu_baseline"] = {
+            "value": round(mesh.num_cells() / per, 1), "unit": "cells/s", "cores": 1,
+            "kind": "port",
+            "sample": f"oracle error_indicators (numpy, FP64) at {ns}^3 ({mesh.num_cells()} "
+                      f"cells), {reps} evaluations, {per:.3f} s each"}
+    print(json.dumps(out))
+
+
 def main_probe(args):
     """Short run for hardware-counter passes: 5 launches of the FP32
     structured operator apply of the 48^3 A = 4 workload and 5 launches of a
@@ -592,7 +674,7 @@ def main():
     ap.add_argument("--skip-reference", action="store_true")
     ap.add_argument("--tf-n", type=int, default=32)
     ap.add_argument("--workload", choices=("single", "batch", "serve", "probe", "two_field",
-                                           "nullspace"),
+                                           "nullspace", "indicators"),
                     default="single")
     ap.add_argument("--instances", type=int, default=96)
     ap.add_argument("--sample", type=int, default=8)
@@ -613,6 +695,8 @@ def main():
         return main_serve(args)
     if args.workload == "probe":
         return main_probe(args)
+    if args.workload == "indicators":
+        return main_indicators(args)
     if args.workload == "nullspace":
         return main_nullspace(args)
     if args.workload == "two_field":

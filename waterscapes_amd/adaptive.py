@@ -184,7 +184,7 @@ class ErrorEstimator:
         if expr is None or expr.is_zero():
             return None
         v = self._nodal(expr, k, ncomp)
-        return torch.as_tensor(np.ascontiguousarray(v.reshape(-1))).to(self.device)
+        return torch.as_tensor(np.array(v.reshape(-1), dtype=np.float64)).to(self.device)
 
     def indicators(self):
         """(zeta_p, zeta_u, zeta_u_dt, r4) per cell (numpy) for the solver's
