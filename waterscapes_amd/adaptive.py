@@ -181,10 +181,20 @@ class ErrorEstimator:
         return v
 
     def _values(self, expr, k, ncomp):
+        """Device P_k nodal values of expr on every cell; kept between calls
+        for an expression that does not depend on time."""
         if expr is None or expr.is_zero():
             return None
+        key = (id(expr), k, ncomp)
+        cache = self.__dict__.setdefault("_cache", {})
+        dep = getattr(expr, "depends_on_time", lambda: True)()
+        if not dep and key in cache:
+            return cache[key]
         v = self._nodal(expr, k, ncomp)
-        return torch.as_tensor(np.array(v.reshape(-1), dtype=np.float64)).to(self.device)
+        t = torch.as_tensor(np.array(v.reshape(-1), dtype=np.float64)).to(self.device)
+        if not dep:
+            cache[key] = t
+        return t
 
     def indicators(self):
         """(zeta_p, zeta_u, zeta_u_dt, r4) per cell (numpy) for the solver's
