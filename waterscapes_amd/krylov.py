@@ -92,7 +92,7 @@ class SolverConfig:
     rtol: float = 1e-10
     maxit: int = 5000
     outer_tol: float = 3e-11
-    inner_rtol: float = 1e-4
+    inner_rtol: float = 1e-6
     max_outer: int = 30
     check_every: int = 0              # 0: 1 for fp64, 10 for mixed
     use_graphs: bool = None           # default: mixed only

@@ -87,7 +87,7 @@ def footprint_bytes(space, precision="mixed", robin=False):
 class MPETServer:
     """Continuous-batching server over one GPU (see module docstring)."""
 
-    def __init__(self, hbm_budget=200e9, precision="mixed", outer_tol=3e-11, inner_rtol=1e-4,
+    def __init__(self, hbm_budget=200e9, precision="mixed", outer_tol=3e-11, inner_rtol=1e-6,
                  rtol=1e-10, max_instances=512, clock=time.perf_counter):
         if not torch.cuda.is_available():
             raise RuntimeError("MPETServer needs a GPU (no CPU fallback)")

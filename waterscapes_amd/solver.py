@@ -331,7 +331,7 @@ class MPETTotalPressureSolver(object):
         params.add("precision", "fp64")
         params.add("krylov_rtol", 1e-10)
         params.add("outer_tol", 3e-11)
-        params.add("inner_rtol", 1e-4)
+        params.add("inner_rtol", 1e-6)
         params.add("maxit", 5000)
         params.add("p0_weight", "robust")
         params.add("extrapolate", None)
