@@ -185,3 +185,36 @@ def test_mixed_space_multiplier_layout():
                           sp_.nu + 1 + sp_.F * np.arange(sp_.nP1))
     with pytest.raises(ValueError):
         MixedSpace(mesh, 2, p_fields=[3])
+
+
+def test_new_entry_points_reject_invalid_arguments():
+    """Argument checks of the constraint-row, P2-mass, bordered-solver and
+    indicator entry points return MPET_ERR_INVALID with a message before
+    any device work (host-only calls)."""
+    import ctypes as C
+    import os
+    from waterscapes_amd import _lib
+    if not os.path.exists(_lib.LIBPATH):
+        pytest.skip("library not built")
+    L = _lib.lib()
+    P = C.c_void_p
+    L.mpet_solver_create_bordered.argtypes = [P, P, C.c_int, C.c_int, C.c_int, P, P, C.c_int,
+                                              C.c_double, P, C.POINTER(C.c_void_p)]
+    h = C.c_void_p()
+    msg = C.create_string_buffer(256)
+
+    def err(rc):
+        assert rc == 1                       # MPET_ERR_INVALID
+        L.mpet_last_error(msg, 256)
+        return msg.value.decode()
+
+    assert "at least one constraint" in err(
+        L.mpet_solver_create_bordered(None, None, 1, 1, 1, None, None, 0, 1.0, None, C.byref(h)))
+    assert "invalid argument" in err(
+        L.mpet_solver_create_bordered(None, None, 1, 1, 1, None, None, 6, 1.0, None, C.byref(h)))
+    assert "invalid argument" in err(
+        L.mpet_constraint_rows(None, None, None, 1, 1, 1, None, 6, None, 0, None, None))
+    assert "invalid argument" in err(L.mpet_assemble_p2_mass(None, None, None, None))
+    assert "invalid argument" in err(
+        L.mpet_error_indicators(None, None, None, None, 0, None, None, None, None))
+    assert h.value is None
