@@ -671,6 +671,7 @@ def main():
     # 48^3 against 197.8 ms at 1e-5, ~200 ms at 1e-4, 201-206 ms at 1e-7 /
     # 3e-7; the 96-instance batch 62.5 instance-steps/s against 60.3 at 1e-4)
     ap.add_argument("--inner-rtol", type=float, default=None)
+    ap.add_argument("--ex-order", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=8)
     ap.add_argument("--skip-fp64", action="store_true")
@@ -710,7 +711,8 @@ def main():
     from waterscapes_amd import _lib
     K, W = args.steps, args.warmup
     t_setup = time.perf_counter()
-    mixed = make_solver(args.n, args.A, "mixed", outer_tol=args.outer_tol,
+    mixed = make_solver(args.n, args.A, "mixed", extrapolate_order=args.ex_order,
+                        outer_tol=args.outer_tol,
                         inner_rtol=args.inner_rtol)
     mixed.engine()
     torch.cuda.synchronize()

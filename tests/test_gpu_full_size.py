@@ -91,10 +91,10 @@ def test_full_size_step_is_reproducible(workload):
     t = float(mixed.problem.time)
     x1 = torch.empty_like(up_prev)
     x2 = torch.empty_like(up_prev)
-    eng.prev = None
+    eng.hist = []
     eng.step(up_prev, x1, t)
     mixed.problem.time.assign(t)
-    eng.prev = None
+    eng.hist = []
     eng.step(up_prev, x2, t)
     mixed.problem.time.assign(t)
     torch.cuda.synchronize()

@@ -33,7 +33,7 @@ from . import _lib
 from ._lib import check, ptr, MAXF
 from .krylov import (MAXLEV, HierDesc, PrecDesc, SolveStats, SolverConfig, SolverOptions,
                      _pick_lpr, compact_level, set_level_degrees)
-from .solver import StepEngine
+from .solver import StepEngine, extrapolated_guess, push_history
 
 
 class BatchCoef(C.Structure):
@@ -187,7 +187,7 @@ class MPETBatch:
 
     def __init__(self, solvers, precision="mixed", rtol=1e-10, outer_tol=3e-11,
                  inner_rtol=1e-6, maxit=5000, max_outer=30, check_every=10, use_graphs=True,
-                 extrapolate=None):
+                 extrapolate=None, extrapolate_order=3):
         if not torch.cuda.is_available():
             raise RuntimeError("MPETBatch needs a GPU (no CPU fallback)")
         if precision not in ("mixed", "fp64"):
@@ -196,9 +196,11 @@ class MPETBatch:
         if not self.solvers:
             raise ValueError("empty batch")
         self.precision = precision
-        # initial guess 2 u_n - u_{n-1} (the mixed path's outer tolerance
-        # bounds the error independently of the guess)
+        # initial guess extrapolated in time from u_n and the previous states
+        # (polynomial degree extrapolate_order; the mixed path's outer
+        # tolerance bounds the error independently of the guess)
         self.extrapolate = (precision == "mixed") if extrapolate is None else bool(extrapolate)
+        self.ex_order = int(extrapolate_order)
         self.K = K = len(self.solvers)
         dev = torch.device(self.solvers[0].params["device"])
         self.device = dev
@@ -487,9 +489,7 @@ class MPETBatch:
             e, s = self.engines[k], self.solvers[k]
             t_next[k] = e.assemble_rhs(s.up_.data, self.t[k])
             x = s.up.data
-            x.copy_(s.up_.data)
-            if self.extrapolate and e.prev is not None:
-                x.add_(s.up_.data).sub_(e.prev)
+            extrapolated_guess(e.hist if self.extrapolate else [], self.ex_order, s.up_.data, x)
             if e.bc_dofs.numel():
                 g = torch.as_tensor(e.bc_values(None)).to(self.device)
                 x.index_copy_(0, e.bc_dofs, g)
@@ -512,10 +512,7 @@ class MPETBatch:
         ks = range(self.K) if active is None else np.nonzero(np.asarray(active))[0]
         if self.extrapolate:
             for k in ks:
-                e = self.engines[k]
-                if e.prev is None:
-                    e.prev = torch.empty_like(self.solvers[k].up_.data)
-                e.prev.copy_(self.solvers[k].up_.data)
+                push_history(self.engines[k].hist, self.ex_order, self.solvers[k].up_.data)
         if active is None:
             self.Xprev_i.copy_(self.X_i)
         else:
