@@ -128,8 +128,10 @@ def test_dilation_with_pressure_nullspaces():
         want = 3 * np.asarray(params["alpha"]) * (t - t_prev) / (2 * mu + 3 * lmbda)
         print("dilation", t, err_u, x[nd + 6:], want, flush=True)
         assert err_u < 1e-8
-        assert np.abs(x[nu:nd]).max() < 1e-8 * np.abs(want).max()
-        assert np.abs(x[nd:nd + 6]).max() < 1e-8 * np.abs(want).max()
+        # p_j = 0 and the rigid-motion multipliers = 0 to the solve's
+        # tolerance (rtol 1e-12 of the right-hand side, loads of order 0.1)
+        assert np.abs(x[nu:nd]).max() < 1e-9
+        assert np.abs(x[nd:nd + 6]).max() < 1e-9
         assert np.abs(x[nd + 6:] - want).max() < 1e-8 * np.abs(want).max()
         assert np.abs(x[:nu] - xr[:nu]).max() < 1e-8 * np.abs(u).max()
         assert np.abs(x[nd + 6:] - xr[nd + 6:]).max() < 1e-8 * np.abs(want).max()

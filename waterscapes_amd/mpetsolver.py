@@ -50,7 +50,10 @@ class MPETSolver(object):
         """mpet/mpetsolver.py:86-98 plus backend keys: "krylov_rtol" (relative
         preconditioned-residual tolerance of the MINRES solve standing in for
         LU), "maxit", "u_smoother_degree", "u_level_degrees",
-        "p_smoother_degree", "device"."""
+        "p_smoother_degree", "extrapolate" / "extrapolate_order" (initial
+        guess of a step extrapolated in time from the previous states; the
+        tolerance is relative to the right-hand side, so only the iteration
+        count depends on it), "device"."""
         params = Parameters("MPETSolver")
         params.add("dt", 0.1)
         params.add("t", 0.0)
@@ -65,6 +68,8 @@ class MPETSolver(object):
         params.add("u_smoother_degree", 3)
         params.add("u_level_degrees", (1,))
         params.add("p_smoother_degree", 2)
+        params.add("extrapolate", True)
+        params.add("extrapolate_order", 3)
         params.add("device", "cuda")
         return params
 
@@ -122,7 +127,9 @@ class MPETSolver(object):
         if self._engine is None or self._engine.dt != dt:
             self._engine = StepEngine(self.problem, self.up_.function_space(), dt,
                                       self.params["theta"], self._config(),
-                                      self.params["device"])
+                                      self.params["device"],
+                                      extrapolate=bool(self.params["extrapolate"]),
+                                      extrapolate_order=int(self.params["extrapolate_order"]))
         return self._engine
 
     # -- time stepping ----------------------------------------------------------
