@@ -442,7 +442,8 @@ def make_nullspace_solver(n, A, seed=2024):
     condition: traction-free everywhere, rigid motions removed by the
     multipliers of u_has_nullspace (mpettotalpressuresolver.py:179-202,
     285-293); network pressures oscillating on z = 1.  FP64 MINRES to rtol
-    1e-10 on the bordered system (iterative path)."""
+    1e-10 on the bordered system (iterative path), initial guess
+    extrapolated in time (cubic)."""
     from waterscapes_amd import (MPETProblem, MPETTotalPressureSolver, Constant, Expression,
                                  UnitCubeMesh, CompiledSubDomain)
     mesh = UnitCubeMesh(n, n, n)
@@ -459,7 +460,7 @@ def make_nullspace_solver(n, A, seed=2024):
             problem.continuity_boundary_markers[j], 0)
     return MPETTotalPressureSolver(problem, dict(dt=0.0125, theta=1.0, T=1e9,
                                                  direct_solver=False, precision="fp64",
-                                                 krylov_rtol=1e-10))
+                                                 krylov_rtol=1e-10, extrapolate=True))
 
 
 def nullspace_cpu_baseline(n_sample, A, steps, budget_s=20.0):

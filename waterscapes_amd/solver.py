@@ -316,7 +316,7 @@ class MPETTotalPressureSolver(object):
     "inner_rtol", "maxit", "p0_weight" ("robust" / "reference"),
     "extrapolate" (initial guess extrapolated in time from up_ and the
     previous states instead of up_, polynomial degree "extrapolate_order",
-    default 3; None =
+    default 3; None = on for "mixed" and the direct path,
     on for "mixed", whose outer tolerance bounds the error independently of
     the guess, off otherwise), "u_smoother_degree" / "p_smoother_degree"
     (Chebyshev degrees of the two AMG hierarchies), "u_level_degrees" /
@@ -417,7 +417,11 @@ class MPETTotalPressureSolver(object):
         if self._engine is None:
             cfg = self._config(direct)
             ex = self.params["extrapolate"]
-            ex = (cfg.mode == "mixed") if ex is None else bool(ex)
+            # default: on for the mixed path and for the direct path (whose
+            # tolerance is relative to the right-hand side, so the guess only
+            # moves the iteration count); the plain FP64 iterative path keeps
+            # the reference's guess up_ (:452-454)
+            ex = (cfg.mode == "mixed" or direct) if ex is None else bool(ex)
             self._engine = StepEngine(self.problem, self.up_.function_space(),
                                       self.params["dt"], self.params["theta"], cfg,
                                       self.params["device"], extrapolate=ex,
