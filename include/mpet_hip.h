@@ -378,6 +378,17 @@ int mpet_solver_create_csr(const MpetCsrDesc* csr64, const uint8_t* dmask, int n
                            int F, const MpetPrecDesc* prec64, const MpetSolverOptions* opt,
                            void** handle);
 
+/* Mixed precision on a CSR operator alone (the two-field system): FP32
+ * MINRES with the FP32 block preconditioner prec32 on an FP32 copy of the
+ * CSR values (made here, owned by the handle), inside FP64 iterative
+ * refinement with the FP64 CSR residual, to opt->outer_tol as in
+ * mpet_solver_create's mode 1.  opt->mode must be 1.  No counterpart in the
+ * reference (which factorises the two-field system with MUMPS,
+ * mpet/mpetsolver.py:347-379); an alternative to mpet_solver_create_csr. */
+int mpet_solver_create_csr_mixed(const MpetCsrDesc* csr64, const uint8_t* dmask, int nP2,
+                                 int nP1, int F, const MpetPrecDesc* prec32,
+                                 const MpetSolverOptions* opt, void** handle);
+
 /* FP64 MINRES on the bordered system [A B^T; B 0] of the nullspace
  * multipliers (rigid motions of u, constants of chosen pressure fields:
  * mpet/mpettotalpressuresolver.py:179-202, 285-306; mpet/mpetsolver.py:

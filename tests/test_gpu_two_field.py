@@ -129,6 +129,28 @@ def test_two_field_solve_matches_oracle_direct(facets):
     assert len(solver.up.split()) == CFG1["J"] + 1
 
 
+@pytest.mark.parametrize("facets", [False, True])
+def test_two_field_mixed_matches_oracle(facets):
+    """precision="mixed": FP32 MINRES / preconditioner on the FP32 copy of
+    the CSR system inside FP64 refinement to outer_tol 1e-12 — the same
+    steps as the oracle's LU to 1e-8 per field."""
+    problem, solver, sym = build(4, facets=facets, precision="mixed")
+    ref_prob = oracle_problem(solver, CFG1, sym, 0.05, 1.0, 0.1, facets)
+    up0 = solver.up_.vector().get_local()
+    ref = [(x.copy(), t) for x, t in ref_prob.solve_direct(up0)]
+    space = solver.up_.function_space()
+    k = 0
+    for (up, t) in solver.solve():
+        errs = field_errors(space, up.vector().get_local(), ref[k][0])
+        st = solver.engine().stats[-1]
+        print("two-field mixed", facets, k, errs, st.outer_iterations, st.inner_iterations,
+              flush=True)
+        assert st.outer_iterations >= 1 and st.converged
+        assert max(errs) <= 1e-8, (k, errs)
+        k += 1
+    assert k == len(ref) == 2
+
+
 def test_two_field_step_api():
     """step(dt) (:317-379): one step from the current time; up and time move,
     up_ does not."""

@@ -505,6 +505,10 @@ struct MpetSolver {
   double tail_inv = 0.0;
   long long nv = 0;                       // Krylov vector length n + mb
   double* yraw = nullptr;                 // A z of the field rows
+  // CSR-only solvers (no structured operator): the operator of every mode
+  // is the CSR matrix; mixed mode iterates on its FP32 copy
+  bool csr_only = false;
+  float* csr_v32 = nullptr;
 };
 
 // Grid of the bordered kernels: one partial per block for each of the m + 1
@@ -591,6 +595,22 @@ static void minres_iter(MpetSolver* S, Minres<T>& M, const Precond<T>& P, bool c
   case LP:                                                                                     \
     hipLaunchKernelGGL((k1_csr_kernel<double, LP>), dim3(np), dim3(NT), 0, s, S->csr_rows,     \
                        S->csr_rp, S->csr_ci, S->csr_v, M.z, epi, M.st, skip, M.partial);      \
+    break;
+      switch (lpr) { K1CSR(4) K1CSR(8) K1CSR(16) K1CSR(32) K1CSR(64) default: break; }
+#undef K1CSR
+    }
+  } else {
+    if (csr) {                      // FP32 copy of a CSR-only operator
+      epi.mask = S->opt_mask64;
+      int lpr = S->csr_lpr;
+      int rows_per_block = NT / lpr;
+      long long ntiles = (S->csr_rows + rows_per_block - 1) / rows_per_block;
+      np = grid_for(ntiles, 1, MAX_PARTIALS);
+#define K1CSR(LP)                                                                              \
+  case LP:                                                                                     \
+    hipLaunchKernelGGL((k1_csr_kernel<float, LP>), dim3(np), dim3(NT), 0, s, S->csr_rows,      \
+                       S->csr_rp, S->csr_ci, (const float*)S->csr_v32, M.z, epi, M.st, skip,   \
+                       M.partial);                                                             \
     break;
       switch (lpr) { K1CSR(4) K1CSR(8) K1CSR(16) K1CSR(32) K1CSR(64) default: break; }
 #undef K1CSR
@@ -690,7 +710,7 @@ static int minres_solve(MpetSolver* S, Minres<T>& M, const Precond<T>& P, bool c
 // r = Pi (b - K x) in FP64 (K = A, or the bordered [A B^T; B 0]).
 static void op_residual64(MpetSolver* S, const double* b, const double* x, double* r,
                           hipStream_t s) {
-  if (S->opt.mode == 0 && S->csr_rp && S->opt.use_csr)
+  if (S->csr_rp && (S->csr_only || (S->opt.mode == 0 && S->opt.use_csr)))
     launch_csr_spmv<double>(S->csr_rows, S->csr_rp, S->csr_ci, S->csr_v, x, b, S->opt_mask64,
                             r, S->csr_lpr, s);
   else
@@ -736,19 +756,22 @@ static int finish_create(MpetSolver* S, const MpetPrecDesc* prec64, const MpetPr
                          void** handle);
 
 static int create_csr(const char* who, const MpetCsrDesc* csr64, const uint8_t* dmask, int nP2,
-                      int nP1, int F, const MpetPrecDesc* prec64, const double* B, int m,
-                      double omega, const MpetSolverOptions* opt, void** handle) {
+                      int nP1, int F, const MpetPrecDesc* prec64, const MpetPrecDesc* prec32,
+                      const double* B, int m, double omega, const MpetSolverOptions* opt,
+                      void** handle) {
   char msg[160];
-  if (!csr64 || !csr64->rp || !csr64->ci || !csr64->v || !dmask || !prec64 || !opt ||
-      !handle || F < 1 || F > MPET_MAXF || nP2 < 1 || nP1 < 1 ||
+  if (!csr64 || !csr64->rp || !csr64->ci || !csr64->v || !dmask || !(prec64 || prec32) ||
+      !opt || !handle || F < 1 || F > MPET_MAXF || nP2 < 1 || nP1 < 1 ||
       csr64->nrows != 3ll * nP2 + (long long)F * nP1 || m < 0 || m > MAXB ||
       (m > 0 && (!B || !(omega > 0.0)))) {
     snprintf(msg, sizeof(msg), "%s: invalid argument", who);
     mpet_set_error(msg);
     return ERR_INVALID;
   }
-  if (opt->mode != 0) {
-    snprintf(msg, sizeof(msg), "%s: only FP64 MINRES (mode 0) runs on a CSR operator alone", who);
+  const int want = prec32 ? 1 : 0;
+  if (opt->mode != want) {
+    snprintf(msg, sizeof(msg), "%s: mode %d does not match the entry point (mode %d)", who,
+             opt->mode, want);
     mpet_set_error(msg);
     return ERR_INVALID;
   }
@@ -770,14 +793,51 @@ static int create_csr(const char* who, const MpetCsrDesc* csr64, const uint8_t* 
   S->csr_ci = csr64->ci;
   S->csr_v = csr64->v;
   S->csr_lpr = csr64->lpr ? csr64->lpr : 16;
-  return finish_create(S, prec64, nullptr, handle);
+  S->csr_only = true;
+  if (prec32) {
+    // FP32 copy of the CSR values for the inner iterations
+    long long nnz = 0;
+    if (hipMemcpy(&nnz, csr64->rp + csr64->nrows, sizeof(long long), hipMemcpyDeviceToHost) !=
+            hipSuccess || nnz < 1) {
+      snprintf(msg, sizeof(msg), "%s: cannot read the CSR row pointers", who);
+      mpet_set_error(msg);
+      delete S;
+      return ERR_INVALID;
+    }
+    int rc = alloc_dev(S->allocs, &S->csr_v32, (size_t)nnz);
+    if (rc) { mpet_solver_destroy(S); return rc; }
+    const long long blocks = (nnz + NT - 1) / NT;
+    hipLaunchKernelGGL(f64_to_f32_kernel, dim3((unsigned)blocks), dim3(NT), 0, 0, nnz,
+                       csr64->v, S->csr_v32);
+    if (hipDeviceSynchronize() != hipSuccess) {
+      mpet_set_error("mpet_solver_create_csr_mixed: conversion of the CSR values failed");
+      mpet_solver_destroy(S);
+      return ERR_HIP;
+    }
+  }
+  return finish_create(S, prec64, prec32, handle);
+}
+
+extern "C" int mpet_solver_create_csr_mixed(const MpetCsrDesc* csr64, const uint8_t* dmask,
+                                            int nP2, int nP1, int F, const MpetPrecDesc* prec32,
+                                            const MpetSolverOptions* opt, void** handle) {
+  if (!prec32) {
+    mpet_set_error("mpet_solver_create_csr_mixed: invalid argument");
+    return ERR_INVALID;
+  }
+  return create_csr("mpet_solver_create_csr_mixed", csr64, dmask, nP2, nP1, F, nullptr, prec32,
+                    nullptr, 0, 0.0, opt, handle);
 }
 
 extern "C" int mpet_solver_create_csr(const MpetCsrDesc* csr64, const uint8_t* dmask, int nP2,
                                       int nP1, int F, const MpetPrecDesc* prec64,
                                       const MpetSolverOptions* opt, void** handle) {
-  return create_csr("mpet_solver_create_csr", csr64, dmask, nP2, nP1, F, prec64, nullptr, 0,
-                    0.0, opt, handle);
+  if (!prec64) {
+    mpet_set_error("mpet_solver_create_csr: invalid argument");
+    return ERR_INVALID;
+  }
+  return create_csr("mpet_solver_create_csr", csr64, dmask, nP2, nP1, F, prec64, nullptr,
+                    nullptr, 0, 0.0, opt, handle);
 }
 
 extern "C" int mpet_solver_create_bordered(const MpetCsrDesc* csr64, const uint8_t* dmask,
@@ -788,8 +848,12 @@ extern "C" int mpet_solver_create_bordered(const MpetCsrDesc* csr64, const uint8
     mpet_set_error("mpet_solver_create_bordered: needs at least one constraint row");
     return ERR_INVALID;
   }
-  return create_csr("mpet_solver_create_bordered", csr64, dmask, nP2, nP1, F, prec64, B, m,
-                    omega, opt, handle);
+  if (!prec64) {
+    mpet_set_error("mpet_solver_create_bordered: invalid argument");
+    return ERR_INVALID;
+  }
+  return create_csr("mpet_solver_create_bordered", csr64, dmask, nP2, nP1, F, prec64, nullptr,
+                    B, m, omega, opt, handle);
 }
 
 extern "C" int mpet_solver_create(const MpetOpDesc* op64, const MpetOpDesc* op32,
@@ -970,7 +1034,7 @@ extern "C" int mpet_solver_solve(void* h, const double* b, double* x, MpetSolveS
         }
       }
       int it = 0, conv = 0;
-      rc = minres_continue(S, S->m32, S->pc32, false, S->opt.inner_rtol, S->opt.maxit,
+      rc = minres_continue(S, S->m32, S->pc32, S->csr_only, S->opt.inner_rtol, S->opt.maxit,
                            S->opt.check_every, S->opt.use_graphs ? &S->graph32 : nullptr, s,
                            &it, &conv);
       total += it;

@@ -72,6 +72,9 @@ def _bind():
     L.mpet_solver_create_bordered.argtypes = [P, P, C.c_int, C.c_int, C.c_int, P, P, C.c_int,
                                               C.c_double, P, C.POINTER(C.c_void_p)]
     L.mpet_solver_create_bordered.restype = C.c_int
+    L.mpet_solver_create_csr_mixed.argtypes = [P, P, C.c_int, C.c_int, C.c_int, P, P,
+                                               C.POINTER(C.c_void_p)]
+    L.mpet_solver_create_csr_mixed.restype = C.c_int
     L.mpet_solver_destroy.argtypes = [P]
     L.mpet_solver_destroy.restype = C.c_int
     L.mpet_solver_solve.argtypes = [P, P, P, P, P]
@@ -204,8 +207,8 @@ class KrylovSolver:
         self.pmask = torch.as_tensor(pmask).to(dev)
         self.Pd = np.array(system.Pd, dtype=np.float64)
         two_field = getattr(system, "two_field", False)
-        if two_field and cfg.mode != "fp64":
-            raise ValueError("the two-field system is solved in FP64 (mode 'fp64') only")
+        if two_field and cfg.mode not in ("fp64", "mixed"):
+            raise ValueError("the two-field system is solved in mode 'fp64' or 'mixed'")
         if cfg.p0_weight not in ("robust", "reference"):
             raise ValueError("p0_weight must be 'robust' or 'reference'")
         if cfg.p0_weight == "robust" and not two_field:
@@ -232,7 +235,7 @@ class KrylovSolver:
         if two_field or constraints is not None:
             use_csr = True
         self.csr_desc = None
-        if use_csr and cfg.mode == "fp64":
+        if use_csr and (cfg.mode == "fp64" or two_field):
             if system.csr is None:
                 system.csr = system.expand_csr(0)
             rp, ci, v = system.csr
@@ -259,10 +262,16 @@ class KrylovSolver:
             return
         if two_field:
             self.op64 = self.op32 = None
-            check(L.mpet_solver_create_csr(C.byref(self.csr_desc), ptr(system.dmask),
-                                           system.nP2, system.nP1, system.F,
-                                           C.byref(self.prec64), C.byref(self.opt),
-                                           C.byref(h)))
+            if mixed:
+                check(L.mpet_solver_create_csr_mixed(C.byref(self.csr_desc), ptr(system.dmask),
+                                                     system.nP2, system.nP1, system.F,
+                                                     C.byref(self.prec32), C.byref(self.opt),
+                                                     C.byref(h)))
+            else:
+                check(L.mpet_solver_create_csr(C.byref(self.csr_desc), ptr(system.dmask),
+                                               system.nP2, system.nP1, system.F,
+                                               C.byref(self.prec64), C.byref(self.opt),
+                                               C.byref(h)))
             self.handle = h
             return
         self.op64 = system.op_desc(0)

@@ -50,7 +50,10 @@ class MPETSolver(object):
         """mpet/mpetsolver.py:86-98 plus backend keys: "krylov_rtol" (relative
         preconditioned-residual tolerance of the MINRES solve standing in for
         LU), "maxit", "u_smoother_degree", "u_level_degrees",
-        "p_smoother_degree", "extrapolate" / "extrapolate_order" (initial
+        "p_smoother_degree", "precision" ("fp64", or "mixed": FP32 MINRES
+        and preconditioner on an FP32 copy of the CSR system inside FP64
+        refinement to "outer_tol", measured like krylov_rtol; "inner_rtol"
+        of the inner solves), "extrapolate" / "extrapolate_order" (initial
         guess of a step extrapolated in time from the previous states; the
         tolerance is relative to the right-hand side, so only the iteration
         count depends on it), "device"."""
@@ -68,6 +71,9 @@ class MPETSolver(object):
         params.add("u_smoother_degree", 3)
         params.add("u_level_degrees", (1,))
         params.add("p_smoother_degree", 2)
+        params.add("precision", "fp64")
+        params.add("outer_tol", 1e-12)
+        params.add("inner_rtol", 1e-6)
         params.add("extrapolate", True)
         params.add("extrapolate_order", 3)
         params.add("device", "cuda")
@@ -115,7 +121,13 @@ class MPETSolver(object):
 
     def _config(self):
         p = self.params
-        return SolverConfig(mode="fp64", rtol=float(p["krylov_rtol"]), maxit=int(p["maxit"]),
+        if p["precision"] not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        if p["precision"] == "mixed" and self.up.function_space().nmult:
+            raise ValueError("systems with nullspace multipliers are solved in FP64 only")
+        return SolverConfig(mode=p["precision"], rtol=float(p["krylov_rtol"]),
+                            outer_tol=float(p["outer_tol"]), inner_rtol=float(p["inner_rtol"]),
+                            maxit=int(p["maxit"]),
                             amg=AmgParams(smoother_degree=int(p["u_smoother_degree"]),
                                           level_degrees=tuple(p["u_level_degrees"])),
                             pamg=AmgParams(smoother_degree=int(p["p_smoother_degree"])))
