@@ -218,3 +218,29 @@ def test_new_entry_points_reject_invalid_arguments():
     assert "invalid argument" in err(
         L.mpet_error_indicators(None, None, None, None, 0, None, None, None, None))
     assert h.value is None
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_extrapolated_guess_is_exact_for_polynomials(order):
+    """Host logic of the initial guess (StepEngine, MPETBatch): with the
+    history filled, the guess reproduces states that are polynomials of
+    degree `order` in time exactly; before that it uses the states it has."""
+    import torch
+    from waterscapes_amd.solver import extrapolated_guess, push_history
+    coef = torch.tensor([[0.3, -1.0, 2.0, 0.5, -0.25][: order + 1]], dtype=torch.float64)
+
+    def state(t):
+        return (coef * torch.tensor([t ** k for k in range(order + 1)],
+                                    dtype=torch.float64)).sum(1).repeat(3)
+
+    hist = []
+    x = torch.empty(3, dtype=torch.float64)
+    for n in range(order + 4):
+        extrapolated_guess(hist, order, state(n), x)
+        if len(hist) >= order:
+            assert torch.allclose(x, state(n + 1), rtol=1e-12, atol=1e-12)
+        elif not hist:
+            assert torch.equal(x, state(n))
+        push_history(hist, order, state(n))
+        assert len(hist) == min(n + 1, order)
+        assert torch.equal(hist[0], state(n))
