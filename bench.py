@@ -736,12 +736,21 @@ def main():
         ref_states = fp64_res[2]
         print(f"[bench] fp64 {K} steps {fp64_res[0]:.2f}s iters {fp64_res[3]}", file=sys.stderr,
               flush=True)
+    fp64m_res = None
+    if not args.skip_fp64:
+        # the plain FP64 path at equal footing: the mixed path's initial guess
+        # and a tolerance that meets the 1e-8 error bar
+        fp64m = make_solver(args.n, args.A, "fp64", krylov_rtol=1e-11, extrapolate=True)
+        fp64m_res = run_steps(fp64m, W, K, keep=True)
+        print(f"[bench] fp64 matched {K} steps {fp64m_res[0]:.2f}s iters {fp64m_res[3]}",
+              file=sys.stderr, flush=True)
+        del fp64m
     mix_res = run_steps(mixed, W, K, keep=True)
     print(f"[bench] mixed {K} steps {mix_res[0]:.2f}s inner {mix_res[3]} outer {mix_res[4]}",
           file=sys.stderr, flush=True)
     eng = mixed.engine()
     space = eng.space
-    errs, errs64 = [], []
+    errs, errs64, errs64m = [], [], []
     if not args.skip_reference:
         import gc
         ref = make_solver(args.n, args.A, "fp64", krylov_rtol=1e-13)
@@ -759,6 +768,8 @@ def main():
         errs = [step_err(a, b) for a, b in zip(mix_res[2], ref_res[2])]
         if ref_states is not None:
             errs64 = [step_err(a, b) for a, b in zip(ref_states, ref_res[2])]
+        errs64m = [step_err(a, b) for a, b in zip(fp64m_res[2], ref_res[2])] \
+            if fp64m_res is not None else []
         del ref, ref_res
         gc.collect()
     # live roofline of the dominant operator kernel (FP32 structured apply)
@@ -803,6 +814,14 @@ def main():
                        "iterations": fp64_res[3], "rtol": 1e-10,
                        "max_rel_l2_err_vs_fp64_reference": max(errs64) if errs64 else None}
         out["speedup_mixed_vs_fp64"] = round(mix_sps / fp_sps, 3)
+        if fp64m_res is not None:
+            fm_sps = K / fp64m_res[0]
+            out["fp64_matched"] = {
+                "steps_per_s": round(fm_sps, 4), "ms_per_step": round(1e3 * fp64m_res[0] / K, 3),
+                "iterations": fp64m_res[3], "rtol": 1e-11,
+                "initial_guess": "cubic extrapolation (as the mixed path)",
+                "max_rel_l2_err_vs_fp64_reference": max(errs64m) if errs64m else None}
+            out["speedup_mixed_vs_fp64_matched"] = round(mix_sps / fm_sps, 3)
         fsys = fp64.engine().system
         if fsys.csr is not None:
             rp, ci, v = fsys.csr
