@@ -138,11 +138,14 @@ def test_dilation_with_pressure_nullspaces():
         t_prev = t
 
 
-@pytest.mark.parametrize("direct", [True, False])
-def test_loaded_total_pressure_with_p0_constraint(direct):
+@pytest.mark.parametrize("direct,pnull", [(True, (True, False)), (False, (True, False)),
+                                          (True, (False, True))])
+def test_loaded_total_pressure_with_p0_constraint(direct, pnull):
     """Body force with a net force and torque, traction-free boundary, u
-    nullspace, p_has_nullspace = (True, False): the reference's p[0] is
-    p_0; network 1 Dirichlet on x = 0, network 2 Neumann."""
+    nullspace; network 1 Dirichlet on x = 0, network 2 Neumann.
+    p_has_nullspace = (True, False): the reference's p[0] is p_0;
+    (False, True): p[1] = p_1, whose Dirichlet DOFs then also enter its
+    constraint row (the lifting of the multiplier row)."""
     from waterscapes_amd import MPETTotalPressureSolver, Expression, CompiledSubDomain
     A = 2
     problem, params = _problem(A, (0.5, 0.3), (1e-2, 1e-3), c=(1e-3, 2e-3), E=1e3, nu=0.3,
@@ -150,7 +153,7 @@ def test_loaded_total_pressure_with_p0_constraint(direct):
     problem.f = Expression(("1.0 + x[1]", "0.5*x[0]*t", "-0.2"), t=problem.time, degree=1)
     problem.g = [Expression("x[2]", degree=1), Expression("0.1*t", t=problem.time, degree=0)]
     problem.u_has_nullspace = True
-    problem.p_has_nullspace = (True, False)
+    problem.p_has_nullspace = pnull
     CompiledSubDomain("on_boundary").mark(problem.momentum_boundary_markers, 1)
     CompiledSubDomain("on_boundary && near(x[0], 0.0)").mark(
         problem.continuity_boundary_markers[0], 0)
@@ -171,7 +174,7 @@ def test_loaded_total_pressure_with_p0_constraint(direct):
         g=[lambda X, t: X[:, 2], lambda X, t: np.full(X.shape[0], 0.1 * t)], g_degree=1,
         u_bar=lambda X, t: np.zeros((X.shape[0], 3)),
         p_bar=[lambda X, t: np.full(X.shape[0], 0.3 * t), lambda X, t: np.zeros(X.shape[0])],
-        momentum_markers=mm, continuity_markers=cm, u_nullspace=True, p_nullspace=(True, False))
+        momentum_markers=mm, continuity_markers=cm, u_nullspace=True, p_nullspace=pnull)
     ref_steps = list(ref.solve_direct(np.zeros(space.dim())))
     nd = space.ndofs
     for (x, t), (xr, _) in zip(steps, ref_steps):
