@@ -75,7 +75,10 @@ __device__ __forceinline__ void igeom(const double* __restrict__ X, const int* _
 }
 
 // P2 nodes: vertices 0..3, edges (2,3), (1,3), (1,2), (0,3), (0,2), (0,1).
-__constant__ int c_edge[6][2] = {{2, 3}, {1, 3}, {1, 2}, {0, 3}, {0, 2}, {0, 1}};
+// The node loops below are unrolled, so every index into the geometry and
+// the barycentric point is a compile-time constant (no private-memory
+// arrays).
+__device__ constexpr int kEdge[6][2] = {{2, 3}, {1, 3}, {1, 2}, {0, 3}, {0, 2}, {0, 1}};
 
 // grad phi_a at barycentric point l: sum_k dphi_a/dlambda_k grad lambda_k.
 __device__ __forceinline__ void p2_grad(const IGeom& G, const double l[4], int a, double out[3]) {
@@ -83,7 +86,7 @@ __device__ __forceinline__ void p2_grad(const IGeom& G, const double l[4], int a
     const double s = 4.0 * l[a] - 1.0;
     for (int m = 0; m < 3; ++m) out[m] = s * G.g[a][m];
   } else {
-    const int i = c_edge[a - 4][0], j = c_edge[a - 4][1];
+    const int i = kEdge[a - 4][0], j = kEdge[a - 4][1];
     for (int m = 0; m < 3; ++m) out[m] = 4.0 * (l[j] * G.g[i][m] + l[i] * G.g[j][m]);
   }
 }
@@ -95,7 +98,7 @@ __device__ __forceinline__ void p2_hess(const IGeom& G, int a, double H[3][3]) {
       if (a < 4) {
         H[m][q] = 4.0 * G.g[a][m] * G.g[a][q];
       } else {
-        const int i = c_edge[a - 4][0], j = c_edge[a - 4][1];
+        const int i = kEdge[a - 4][0], j = kEdge[a - 4][1];
         H[m][q] = 4.0 * (G.g[i][m] * G.g[j][q] + G.g[j][m] * G.g[i][q]);
       }
     }
@@ -107,6 +110,7 @@ __device__ __forceinline__ void grad_u(const IGeom& Gm, const double l[4],
                                        double Gu[3][3]) {
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) Gu[i][j] = 0.0;
+#pragma unroll
   for (int a = 0; a < 10; ++a) {
     double gp[3];
     p2_grad(Gm, l, a, gp);
@@ -122,6 +126,7 @@ __device__ __forceinline__ void grad_u(const IGeom& Gm, const double l[4],
 __device__ __forceinline__ void div_sigma(const IGeom& Gm, const double* __restrict__ x,
                                           const int* p2, double mu, double lm, double out[3]) {
   out[0] = out[1] = out[2] = 0.0;
+#pragma unroll
   for (int a = 0; a < 10; ++a) {
     double H[3][3];
     p2_hess(Gm, a, H);
@@ -152,7 +157,9 @@ ind_cell_kernel(MpetIndDesc d, const double* __restrict__ x, const double* __res
   div_sigma(G, x, p2, d.mu, d.lmbda, cu);
   div_sigma(G, xp, p2, d.mu, d.lmbda, cup);
   double r4 = 0.0;
-  for (int j = 0; j < A; ++j) {
+#pragma unroll
+  for (int j = 0; j < MPET_MAXF; ++j) {
+    if (j >= A) continue;
     double gp[3] = {0, 0, 0}, gpp[3] = {0, 0, 0};
     for (int k = 0; k < 4; ++k) {
       const long long q = nu + (long long)A * v[k] + j;
@@ -200,24 +207,31 @@ ind_cell_kernel(MpetIndDesc d, const double* __restrict__ x, const double* __res
       for (int i = 0; i < 3; ++i) du += (x[b + i] - xp[b + i]) * gphi[i];
     }
     double pj[MPET_MAXF], ppj[MPET_MAXF];
-    for (int j = 0; j < A; ++j) {
+#pragma unroll
+    for (int j = 0; j < MPET_MAXF; ++j) {
       pj[j] = ppj[j] = 0.0;
+      if (j >= A) continue;
       for (int k = 0; k < 4; ++k) {
         const long long qq = nu + (long long)A * v[k] + j;
         pj[j] += l[k] * x[qq];
         ppj[j] += l[k] * xp[qq];
       }
     }
-    for (int j = 0; j < A; ++j) {
+#pragma unroll
+    for (int j = 0; j < MPET_MAXF; ++j) {
+      if (j >= A) continue;
       double gj = 0.0;
       if (d.g)
         for (int m = 0; m < d.nkg; ++m)
           gj += d.Lg[q * d.nkg + m] * d.g[((long long)e * d.nkg + m) * A + j];
       double r = -gj - d.c[j] * (pj[j] - ppj[j]) * idt - d.alpha[j] * du * idt;
-      for (int i = 0; i < A; ++i) r -= d.S[i * MPET_MAXF + j] * (pj[j] - pj[i]);
+#pragma unroll
+      for (int i = 0; i < MPET_MAXF; ++i)
+        if (i < A) r -= d.S[i * MPET_MAXF + j] * (pj[j] - pj[i]);
       zp += w * r * r;
-      for (int i = 0; i < A; ++i)
-        if (d.S[i * MPET_MAXF + j] > 3.0e-16)
+#pragma unroll
+      for (int i = 0; i < MPET_MAXF; ++i)
+        if (i < A && d.S[i * MPET_MAXF + j] > 3.0e-16)
           r4s += w * d.S[i * MPET_MAXF + j] * ((pj[j] - ppj[j]) - (pj[i] - ppj[i])) *
                  (pj[j] - ppj[j]);
     }
@@ -242,8 +256,40 @@ __device__ __forceinline__ void bary_of(const IGeom& G, const double P[3], doubl
 
 __constant__ int c_face[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
 
+// sigma(u) n at barycentric point l of a cell (geometry G, P2 nodes p2).
+__device__ __forceinline__ void traction(const IGeom& G, const double l[4],
+                                         const double* __restrict__ x,
+                                         const int* __restrict__ p2, double mu, double lm,
+                                         const double n[3], double t[3]) {
+  double Gu[3][3];
+  grad_u(G, l, x, p2, Gu);
+  const double tr = Gu[0][0] + Gu[1][1] + Gu[2][2];
+  for (int i = 0; i < 3; ++i) {
+    double s = lm * tr * n[i];
+    for (int j = 0; j < 3; ++j) s += mu * (Gu[i][j] + Gu[j][i]) * n[j];
+    t[i] = s;
+  }
+}
+
+// sum_j alpha_j p_j at barycentric point l of a cell with vertices v.
+__device__ __forceinline__ double alpha_p(const MpetIndDesc& d, const double l[4],
+                                          const double* __restrict__ x,
+                                          const int* __restrict__ v, long long nu) {
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < MPET_MAXF; ++j) {
+    if (j >= d.A) continue;
+    double p = 0.0;
+    for (int k = 0; k < 4; ++k) p += l[k] * x[nu + (long long)d.A * v[k] + j];
+    s += d.alpha[j] * p;
+  }
+  return s;
+}
+
 // One interior facet: fac = (cell a, local face a, cell b, local face b).
 // out_f[0..2][t] = avg(h) int_F |RE_p|^2 (sum over j), |RE_u|^2, |RE_u,dt|^2.
+// The tractions of the two cells are formed one after the other (one
+// displacement gradient live at a time).
 __global__ void __launch_bounds__(128)
 ind_facet_kernel(MpetIndDesc d, const double* __restrict__ x, const double* __restrict__ xp,
                  const int* __restrict__ fac, int nF, double* __restrict__ out_f) {
@@ -255,27 +301,25 @@ ind_facet_kernel(MpetIndDesc d, const double* __restrict__ x, const double* __re
   IGeom Ga, Gb;
   igeom(d.coords, d.cells, ca, Ga);
   igeom(d.coords, d.cells, cb, Gb);
-  int pa[10], pb[10], va[4], vb[4];
-  for (int a = 0; a < 10; ++a) {
-    pa[a] = d.cell_p2[10 * ca + a];
-    pb[a] = d.cell_p2[10 * cb + a];
-  }
-  for (int k = 0; k < 4; ++k) {
-    va[k] = d.cells[4 * ca + k];
-    vb[k] = d.cells[4 * cb + k];
-  }
+  const int* pa = d.cell_p2 + 10ll * ca;
+  const int* pb = d.cell_p2 + 10ll * cb;
+  const int* va = d.cells + 4ll * ca;
+  const int* vb = d.cells + 4ll * cb;
   // facet vertices (those of cell a's face la), outward normal of cell a
   double P[3][3];
   for (int k = 0; k < 3; ++k)
     for (int m = 0; m < 3; ++m) P[k][m] = d.coords[3ll * va[c_face[la][k]] + m];
-  double e1[3], e2[3], nrm[3];
-  for (int m = 0; m < 3; ++m) {
-    e1[m] = P[1][m] - P[0][m];
-    e2[m] = P[2][m] - P[0][m];
+  double nrm[3];
+  {
+    double e1[3], e2[3];
+    for (int m = 0; m < 3; ++m) {
+      e1[m] = P[1][m] - P[0][m];
+      e2[m] = P[2][m] - P[0][m];
+    }
+    nrm[0] = e1[1] * e2[2] - e1[2] * e2[1];
+    nrm[1] = e1[2] * e2[0] - e1[0] * e2[2];
+    nrm[2] = e1[0] * e2[1] - e1[1] * e2[0];
   }
-  nrm[0] = e1[1] * e2[2] - e1[2] * e2[1];
-  nrm[1] = e1[2] * e2[0] - e1[0] * e2[2];
-  nrm[2] = e1[0] * e2[1] - e1[1] * e2[0];
   const double len = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
   const double area = 0.5 * len;
   double opp = 0.0;   // (opposite vertex of a - P0) . n  < 0 for the outward normal
@@ -286,7 +330,9 @@ ind_facet_kernel(MpetIndDesc d, const double* __restrict__ x, const double* __re
   const double idt = 1.0 / d.dt;
   // RE_p,j: constant on the facet
   double fp = 0.0;
-  for (int j = 0; j < A; ++j) {
+#pragma unroll
+  for (int j = 0; j < MPET_MAXF; ++j) {
+    if (j >= A) continue;
     double jn = 0.0;
     for (int k = 0; k < 4; ++k) {
       const double xa = x[nu + (long long)A * va[k] + j], xb = x[nu + (long long)A * vb[k] + j];
@@ -296,45 +342,38 @@ ind_facet_kernel(MpetIndDesc d, const double* __restrict__ x, const double* __re
     fp += r * r;
   }
   fp *= area;
-  // RE_u, RE_u^- at the three edge midpoints
+  // RE_u = sum_j alpha_j (p_j^a - p_j^b) n - (sigma_a - sigma_b) n, and the
+  // same for the previous state, at the three edge midpoints.  The four
+  // (cell, state) tractions are formed one at a time, each cell's geometry
+  // recomputed, so that only one displacement gradient is live.
   double fu = 0.0, fud = 0.0;
+#pragma unroll 1
   for (int q = 0; q < 3; ++q) {
     double X[3];
-    for (int m = 0; m < 3; ++m) X[m] = 0.5 * (P[q][m] + P[(q + 1) % 3][m]);
-    double la_[4], lb_[4];
-    bary_of(Ga, X, la_);
-    bary_of(Gb, X, lb_);
-    double re[2][3];
-    for (int s = 0; s < 2; ++s) {
-      const double* xs = s == 0 ? x : xp;
-      double Gua[3][3], Gub[3][3];
-      grad_u(Ga, la_, xs, pa, Gua);
-      grad_u(Gb, lb_, xs, pb, Gub);
-      const double tra = Gua[0][0] + Gua[1][1] + Gua[2][2];
-      const double trb = Gub[0][0] + Gub[1][1] + Gub[2][2];
-      double pjump = 0.0;
-      for (int j = 0; j < A; ++j) {
-        double pva = 0.0, pvb = 0.0;
-        for (int k = 0; k < 4; ++k) {
-          pva += la_[k] * xs[nu + (long long)A * va[k] + j];
-          pvb += lb_[k] * xs[nu + (long long)A * vb[k] + j];
-        }
-        pjump += d.alpha[j] * (pva - pvb);
-      }
+    for (int m = 0; m < 3; ++m)   // midpoint of edge q (static indices into P)
+      X[m] = 0.5 * (q == 0 ? P[0][m] + P[1][m] : (q == 1 ? P[1][m] + P[2][m] : P[2][m] + P[0][m]));
+    double re[3] = {0.0, 0.0, 0.0}, rp[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int c4 = 0; c4 < 4; ++c4) {
+      const int cell = (c4 & 1) ? cb : ca;
+      const double* xs = (c4 & 2) ? xp : x;
+      IGeom G;
+      igeom(d.coords, d.cells, cell, G);
+      double l[4];
+      bary_of(G, X, l);
+      double tt[3];
+      traction(G, l, xs, d.cell_p2 + 10ll * cell, d.mu, d.lmbda, nrm, tt);
+      const double ap = alpha_p(d, l, xs, d.cells + 4ll * cell, nu);
+      const double sg = (c4 & 1) ? 1.0 : -1.0;     // cell a: + p_a n - t_a; cell b: - p_b n + t_b
       for (int i = 0; i < 3; ++i) {
-        double sn = 0.0;
-        for (int j = 0; j < 3; ++j) {
-          const double sa = d.mu * (Gua[i][j] + Gua[j][i]) + (i == j ? d.lmbda * tra : 0.0);
-          const double sb = d.mu * (Gub[i][j] + Gub[j][i]) + (i == j ? d.lmbda * trb : 0.0);
-          sn += (sa - sb) * nrm[j];
-        }
-        re[s][i] = pjump * nrm[i] - sn;
+        const double c = sg * (tt[i] - ap * nrm[i]);
+        if (c4 & 2) rp[i] += c; else re[i] += c;
       }
     }
     double a2 = 0.0, d2 = 0.0;
     for (int i = 0; i < 3; ++i) {
-      a2 += re[0][i] * re[0][i];
-      const double r = (re[0][i] - re[1][i]) * idt;
+      a2 += re[i] * re[i];
+      const double r = (re[i] - rp[i]) * idt;
       d2 += r * r;
     }
     fu += a2 * (area / 3.0);
