@@ -175,6 +175,22 @@ ind_cell_kernel(MpetIndDesc d, const double* __restrict__ x, const double* __res
     }
   }
   const double idt = 1.0 / d.dt;
+  // div(u - u^-) is linear on the cell: its values at the four vertices
+  double dv[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double lk[4] = {k == 0 ? 1.0 : 0.0, k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0,
+                          k == 3 ? 1.0 : 0.0};
+    double sdv = 0.0;
+#pragma unroll
+    for (int a = 0; a < 10; ++a) {
+      double gphi[3];
+      p2_grad(G, lk, a, gphi);
+      const long long b = 3ll * p2[a];
+      for (int i = 0; i < 3; ++i) sdv += (x[b + i] - xp[b + i]) * gphi[i];
+    }
+    dv[k] = sdv;
+  }
   double zu = 0.0, zudt = 0.0, zp = 0.0, r4s = 0.0;
   for (int q = 0; q < d.nq; ++q) {
     const double* l = d.qbary + 4 * q;
@@ -198,14 +214,8 @@ ind_cell_kernel(MpetIndDesc d, const double* __restrict__ x, const double* __res
     }
     zu += w * ru2;
     zudt += w * rd2;
-    // div(u - u^-) at the point
-    double du = 0.0;
-    for (int a = 0; a < 10; ++a) {
-      double gphi[3];
-      p2_grad(G, l, a, gphi);
-      const long long b = 3ll * p2[a];
-      for (int i = 0; i < 3; ++i) du += (x[b + i] - xp[b + i]) * gphi[i];
-    }
+    // div(u - u^-) at the point (linear on the cell)
+    const double du = l[0] * dv[0] + l[1] * dv[1] + l[2] * dv[2] + l[3] * dv[3];
     double pj[MPET_MAXF], ppj[MPET_MAXF];
 #pragma unroll
     for (int j = 0; j < MPET_MAXF; ++j) {
