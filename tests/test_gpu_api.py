@@ -24,12 +24,12 @@ def cstr(e):
 
 
 def build_problem(n, params=CFG1, precision="fp64", direct=False, T=0.1, dt=0.05, theta=1.0,
-                  **solver_params):
+                  mesh=None, **solver_params):
     from waterscapes_amd import (MPETProblem, MPETTotalPressureSolver, Constant, Expression,
                                  UnitCubeMesh, CompiledSubDomain, interpolate, assign)
     A = params["J"]
     u, p, f, g = mms.exact_solutions(params)
-    mesh = UnitCubeMesh(n, n, n)
+    mesh = UnitCubeMesh(n, n, n) if mesh is None else mesh
     time = Constant(0.0)
     problem = MPETProblem(mesh, time, params=params)
     problem.f = Expression(tuple(cstr(e) for e in f), t=time, degree=3)

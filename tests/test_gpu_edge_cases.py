@@ -97,3 +97,35 @@ def test_out_of_range_inputs_are_rejected():
     y = torch.empty_like(x)
     rc = _lib.lib().mpet_op_apply(C.byref(d), _lib.ptr(x), None, _lib.ptr(y), _lib.stream_ptr())
     assert rc == 1
+
+
+@pytest.mark.parametrize("precision,direct,extra", [("fp64", True, {}),
+                                                    ("mixed", False, dict(outer_tol=1e-11))])
+def test_unstructured_numbering_matches_oracle(precision, direct, extra):
+    """A cube mesh with randomly permuted vertex and cell numbering and
+    randomly moved interior vertices (generic Mesh: no box fast path,
+    scattered DOF maps and colouring): the cfg1 manufactured-solution steps
+    match the oracle's LU to 1e-8 per field."""
+    from conftest import CFG1
+    from test_gpu_api import build_problem, oracle_steps, field_errors
+    from waterscapes_amd.mesh import Mesh, UnitCubeMesh
+    base = UnitCubeMesh(3, 3, 3)
+    rng = np.random.default_rng(3)
+    X = base.coordinates().copy()
+    inner = np.all((X > 1e-12) & (X < 1.0 - 1e-12), axis=1)
+    X[inner] += (0.2 / 3) * rng.uniform(-1.0, 1.0, (int(inner.sum()), 3))
+    perm = rng.permutation(X.shape[0])
+    inv = np.argsort(perm)
+    cells = inv[base.cells()][rng.permutation(base.num_cells())]
+    mesh = Mesh(X[perm], cells)
+    assert mesh.structured is None
+    problem, solver, sym = build_problem(3, CFG1, precision, direct, mesh=mesh, **extra)
+    ref = oracle_steps(solver, CFG1, sym, 0.05, 1.0, 0.1)
+    space = solver.up_.function_space()
+    k = 0
+    for (up, t) in solver.solve():
+        errs = field_errors(space, up.vector().get_local(), ref[k][0])
+        print("unstructured", precision, k, errs, flush=True)
+        assert max(errs) <= 1e-8, (precision, k, errs)
+        k += 1
+    assert k == len(ref) == 2
