@@ -446,7 +446,7 @@ def make_nullspace_solver(n, A, seed=2024):
     multipliers of u_has_nullspace (mpettotalpressuresolver.py:179-202,
     285-293); network pressures oscillating on z = 1.  FP64 MINRES to rtol
     1e-10 on the bordered system (iterative path), initial guess
-    extrapolated in time (cubic)."""
+    extrapolated in time (quartic)."""
     from waterscapes_amd import (MPETProblem, MPETTotalPressureSolver, Constant, Expression,
                                  UnitCubeMesh, CompiledSubDomain)
     mesh = UnitCubeMesh(n, n, n)
@@ -675,7 +675,7 @@ def main():
     # 48^3 against 197.8 ms at 1e-5, ~200 ms at 1e-4, 201-206 ms at 1e-7 /
     # 3e-7; the 96-instance batch 62.5 instance-steps/s against 60.3 at 1e-4)
     ap.add_argument("--inner-rtol", type=float, default=None)
-    ap.add_argument("--ex-order", type=int, default=3)
+    ap.add_argument("--ex-order", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=8)
     ap.add_argument("--skip-fp64", action="store_true")
@@ -819,7 +819,7 @@ def main():
             out["fp64_matched"] = {
                 "steps_per_s": round(fm_sps, 4), "ms_per_step": round(1e3 * fp64m_res[0] / K, 3),
                 "iterations": fp64m_res[3], "rtol": 1e-11,
-                "initial_guess": "cubic extrapolation (as the mixed path)",
+                "initial_guess": "quartic extrapolation (as the mixed path)",
                 "max_rel_l2_err_vs_fp64_reference": max(errs64m) if errs64m else None}
             out["speedup_mixed_vs_fp64_matched"] = round(mix_sps / fm_sps, 3)
         fsys = fp64.engine().system

@@ -187,7 +187,7 @@ class MPETBatch:
 
     def __init__(self, solvers, precision="mixed", rtol=1e-10, outer_tol=3e-11,
                  inner_rtol=1e-6, maxit=5000, max_outer=30, check_every=10, use_graphs=True,
-                 extrapolate=None, extrapolate_order=3):
+                 extrapolate=None, extrapolate_order=4):
         if not torch.cuda.is_available():
             raise RuntimeError("MPETBatch needs a GPU (no CPU fallback)")
         if precision not in ("mixed", "fp64"):

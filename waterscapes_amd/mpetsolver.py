@@ -75,7 +75,7 @@ class MPETSolver(object):
         params.add("outer_tol", 1e-12)
         params.add("inner_rtol", 1e-6)
         params.add("extrapolate", True)
-        params.add("extrapolate_order", 3)
+        params.add("extrapolate_order", 4)
         params.add("device", "cuda")
         return params
 
