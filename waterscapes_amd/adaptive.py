@@ -158,6 +158,7 @@ class ErrorEstimator:
         self.cell_f = torch.as_tensor(cell_f).to(dev)
         self.tmp = torch.empty(max(3 * self.nF, 1), dtype=torch.float64, device=dev)
         self.out = torch.empty(4 * d.nC, dtype=torch.float64, device=dev)
+        self._cache = {}               # device node values of time-independent sources
         # lists of error estimate terms, one for each time (:155-159)
         self.eta_1s, self.eta_2s, self.eta_3s, self.eta_4s = [], [], [], []
         nC = d.nC
@@ -186,14 +187,13 @@ class ErrorEstimator:
         if expr is None or expr.is_zero():
             return None
         key = (id(expr), k, ncomp)
-        cache = self.__dict__.setdefault("_cache", {})
         dep = getattr(expr, "depends_on_time", lambda: True)()
-        if not dep and key in cache:
-            return cache[key]
+        if not dep and key in self._cache:
+            return self._cache[key]
         v = self._nodal(expr, k, ncomp)
         t = torch.as_tensor(np.array(v.reshape(-1), dtype=np.float64)).to(self.device)
         if not dep:
-            cache[key] = t
+            self._cache[key] = t       # the expression is held by the caller
         return t
 
     def indicators(self):
