@@ -1,0 +1,180 @@
+"""The benchmark workloads other than the 48^3 one, exercised on the GPU.
+
+  * cfg1 exactly: unit cube 8^3 x 6 tets, A = 2 (E 1e4, nu 0.35, alpha 0.5,
+    c 1e-4, K 1e-3 / 1e-4, S_12 = 1e-2), theta = 1, dt = 0.01, 10 steps of
+    the manufactured solution, Dirichlet everywhere: fp64, mixed and direct
+    paths against the oracle's LU at every step (1e-8 relative L2 per field);
+  * cfg4: the seed-11 batch of 96 uneven instances (N 8..32, A 1..4) stepped
+    20 times in mixed precision: operator symmetry per instance (sampled),
+    a sample of the smallest instances against the oracle at every step, and
+    a sample run alone equal to its run in the mix;
+  * cfg5: a real-clock Poisson stream through the server with one 48^3 A = 4
+    request among small ones; every result equals its run served alone.
+Tolerances: 1e-8 relative L2 per field (the workloads' stated accuracy)."""
+import numpy as np
+import pytest
+import torch
+
+import bench
+from conftest import CFG1, oracle_mesh
+from oracle import mpet_oracle as O
+from test_gpu_api import build_problem, field_errors, oracle_steps
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-8
+
+_cfg1_ref = {}
+
+
+def _cfg1_reference():
+    if "ref" not in _cfg1_ref:
+        problem, solver, sym = build_problem(8, CFG1, "fp64", True, T=0.1, dt=0.01)
+        _cfg1_ref["ref"] = oracle_steps(solver, CFG1, sym, 0.01, 1.0, 0.1)
+    return _cfg1_ref["ref"]
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("precision,direct,extra", [
+    ("fp64", False, dict(krylov_rtol=1e-12)), ("mixed", False, dict(outer_tol=1e-11)),
+    ("fp64", True, {})])
+def test_cfg1_exact_workload_matches_oracle(precision, direct, extra):
+    ref = _cfg1_reference()
+    problem, solver, sym = build_problem(8, CFG1, precision, direct, T=0.1, dt=0.01, **extra)
+    space = solver.up_.function_space()
+    k = 0
+    for up, t in solver.solve():
+        assert abs(t - ref[k][1]) < 1e-14
+        errs = field_errors(space, up.vector().get_local(), ref[k][0])
+        assert max(errs) <= RTOL, (precision, direct, k, errs)
+        k += 1
+    assert k == len(ref) == 10
+
+
+# ---------------------------------------------------------------------------
+# cfg4: 96 uneven instances
+# ---------------------------------------------------------------------------
+def _uneven_oracle(solver, params, steps):
+    """The oracle's direct solve of one uneven-family instance (u = 0 on
+    z = 0, p_j = sin(2 pi t) on z = 1, no sources)."""
+    space = solver.up_.function_space()
+    mesh = space.mesh
+    m = oracle_mesh(space)
+    A = int(params["J"])
+    mid = mesh.facet_midpoints()
+    inv = np.uint64(2**63 - 1)
+    mk_u = np.full(mesh.num_facets(), inv, dtype=np.uint64)
+    mk_u[mesh.exterior & (np.abs(mid[:, 2]) < 1e-12)] = 0
+    mk_p = np.full(mesh.num_facets(), inv, dtype=np.uint64)
+    mk_p[mesh.exterior & (np.abs(mid[:, 2] - 1.0) < 1e-12)] = 0
+    dt = float(solver.params["dt"])
+    pb = [lambda X, t: np.full(X.shape[0], np.sin(2.0 * np.pi * t)) for _ in range(A)]
+    prob = O.OracleProblem(m, params, dict(dt=dt, theta=1.0, T=steps * dt),
+                           u_bar=lambda X, t: np.zeros((X.shape[0], 3)), p_bar=pb,
+                           momentum_markers=mk_u, continuity_markers=[mk_p] * A)
+    up0 = np.zeros(prob.lay.n)
+    return [x.copy() for x, t in prob.solve_direct(up0)][:steps]
+
+
+@pytest.mark.timeout(900)
+def test_cfg4_uneven_batch_of_96():
+    from waterscapes_amd.batch import MPETBatch
+    rng = np.random.default_rng(11)
+    specs = [bench.uneven_instance(rng) for _ in range(96)]
+    assert {N for N, _ in specs} <= set(range(8, 33)) and len({N for N, _ in specs}) > 10
+    solvers = [bench.make_uneven_solver(N, p) for N, p in specs]
+    batch = MPETBatch(solvers, precision="mixed")
+    order = np.argsort([N for N, _ in specs], kind="stable")
+    small = [int(k) for k in order[:2]]                   # oracle sample (N <= 9 here)
+    alone = small[:1] + [int(order[len(order) // 2])]      # isolation sample
+    assert max(specs[k][0] for k in small) <= 12
+    # per-instance operator symmetry on free DOFs (FP64 packed operator)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    x = torch.randn(batch.n, generator=g, device="cuda", dtype=torch.float64)
+    y = torch.randn(batch.n, generator=g, device="cuda", dtype=torch.float64)
+    for k, e in enumerate(batch.engines):
+        a, b = int(batch.n_off[k]), int(batch.n_off[k + 1])
+        dm = e.system.dmask.bool()
+        x[a:b][dm] = 0.0
+        y[a:b][dm] = 0.0
+    Ax = batch.unpack(batch.apply(batch.pack(x)))
+    Ay = batch.unpack(batch.apply(batch.pack(y)))
+    for k in rng.choice(96, 8, replace=False):
+        a, b = int(batch.n_off[k]), int(batch.n_off[k + 1])
+        l, r = torch.dot(y[a:b], Ax[a:b]).item(), torch.dot(x[a:b], Ay[a:b]).item()
+        scale = torch.linalg.norm(Ax[a:b]).item() * torch.linalg.norm(y[a:b]).item()
+        assert abs(l - r) <= 1e-12 * scale, k
+    keep = {k: [] for k in set(small) | set(alone)}
+    for step in range(20):
+        st = batch.step()
+        assert all(st[k].converged for k in range(96)), step
+        batch.commit()
+        for k in keep:
+            keep[k].append(batch.solvers[k].up.vector().get_local().copy())
+    batch.close()
+    for k in small:
+        N, p = specs[k]
+        ref = _uneven_oracle(bench.make_uneven_solver(N, p), p, 20)
+        space = solvers[k].up_.function_space()
+        for s, (got, r) in enumerate(zip(keep[k], ref)):
+            errs = field_errors(space, got, r)
+            assert max(errs) <= RTOL, (k, s, errs)
+    for k in alone:
+        N, p = specs[k]
+        s1 = bench.make_uneven_solver(N, p)
+        b1 = MPETBatch([s1], precision="mixed")
+        space = s1.up_.function_space()
+        for s in range(20):
+            b1.step()
+            b1.commit()
+            errs = field_errors(space, keep[k][s], s1.up.vector().get_local())
+            assert max(errs) <= RTOL, (k, s, errs)
+        b1.close()
+
+
+# ---------------------------------------------------------------------------
+# cfg5: a real-clock stream with one 48^3 A = 4 request
+# ---------------------------------------------------------------------------
+def _stream(seed=3, n_small=5, steps=10):
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    for k in range(n_small + 1):
+        t += rng.exponential(1.0 / 20.0)               # 20 req/s
+        big = k == 2
+        N, p = bench.uneven_instance(rng, big)
+        if not big:
+            N = min(N, 12)                             # keep the small ones small
+        out.append((N, p, t, steps, k))
+    return out
+
+
+def _requests(spec):
+    from waterscapes_amd.serve import Request
+    return [Request(make_solver=lambda N=N, p=p: bench.make_uneven_solver(N, p), steps=st,
+                    arrival=t, rid=k) for N, p, t, st, k in spec]
+
+
+@pytest.mark.timeout(900)
+def test_cfg5_stream_with_48cube_request_isolation():
+    from waterscapes_amd.serve import MPETServer
+    spec = _stream()
+    assert any(N == 48 and p["J"] == 4 for N, p, *_ in spec)
+    srv = MPETServer(hbm_budget=200e9, precision="mixed")
+    for r in _requests(spec):
+        srv.submit(r)
+    srv.run()
+    done = {r.rid: r for r in srv.finished}
+    assert len(done) == len(spec) and not any(r.refused for r in done.values())
+    # the big request overlapped with others on the device
+    big = done[2]
+    assert any(r.admitted < big.done and r.done > big.admitted for k, r in done.items() if k != 2)
+    from waterscapes_amd.fem import MixedSpace
+    from waterscapes_amd.mesh import UnitCubeMesh
+    for N, p, t, st, k in spec:
+        solo = MPETServer(hbm_budget=200e9, precision="mixed")
+        for r in _requests([(N, p, 0.0, st, k)]):
+            solo.submit(r)
+        solo.run()
+        space = MixedSpace(UnitCubeMesh(N, N, N), p["J"])
+        errs = field_errors(space, done[k].result, solo.finished[0].result)
+        assert max(errs) <= RTOL, (k, N, errs)

@@ -181,19 +181,25 @@ class ErrorEstimator:
             v = np.einsum("nm,cmj->cnj", T, v)
         return v
 
-    def _values(self, expr, k, ncomp):
+    def _values(self, slot, expr, k, ncomp):
         """Device P_k nodal values of expr on every cell; kept between calls
         for an expression that does not depend on time."""
         if expr is None or expr.is_zero():
             return None
-        key = (id(expr), k, ncomp)
+        key = (slot, k, ncomp)
         dep = getattr(expr, "depends_on_time", lambda: True)()
-        if not dep and key in self._cache:
-            return self._cache[key]
+        if not dep:
+            hit = self._cache.get(key)
+            # the entry keeps its expression: an identity check, not an id()
+            # that a later object may reuse
+            if hit is not None and hit[0] is expr:
+                return hit[1]
         v = self._nodal(expr, k, ncomp)
         t = torch.as_tensor(np.array(v.reshape(-1), dtype=np.float64)).to(self.device)
         if not dep:
-            self._cache[key] = t       # the expression is held by the caller
+            # one entry per slot (f or f_): replacing problem.f or f_ evicts
+            # the old expression's values
+            self._cache[key] = (expr, t)
         return t
 
     def indicators(self):
@@ -201,8 +207,8 @@ class ErrorEstimator:
         current up and previous up_."""
         d = self.desc
         d.dt = float(self.solver.params["dt"])
-        f = self._values(self.problem.f, self.kf, 3)
-        fp = self._values(self.f_, self.kf, 3)
+        f = self._values("f", self.problem.f, self.kf, 3)
+        fp = self._values("f_", self.f_, self.kf, 3)
         g = None
         if any(not gi.is_zero() for gi in self.problem.g):
             cols = [self._nodal(gi, self.kg, 1)[:, :, 0] for gi in self.problem.g]

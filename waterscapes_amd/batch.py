@@ -481,7 +481,11 @@ class MPETBatch:
         """One implicit step of every instance k with active[k] (default:
         all): the reference's per-step sequence (:461-494) for each, one
         batched solve for all.  Advances each instance's time; returns the
-        per-instance SolveStats (zeros for inactive instances)."""
+        per-instance SolveStats (zeros for inactive instances).  An instance
+        that does not converge (status 3) or breaks down (NaN, status 4)
+        raises MpetError (its .stats the per-instance stats) after every
+        instance's result is stored; with raise_on_fail=False those two
+        statuses return the stats instead."""
         act = np.ones(self.K, dtype=np.int32) if active is None else \
             np.asarray(active, dtype=np.int32).copy()
         t_next = list(self.t)
@@ -499,11 +503,17 @@ class MPETBatch:
         rc = _bind().mpet_batch_solve(self.handle, ptr(self.Bp), ptr(self.Xp),
                                       act.ctypes.data, stats, _lib.stream_ptr())
         self.stats = stats
-        if rc != 0 and (raise_on_fail or rc != 3):
-            check(rc)
+        # the other instances' solutions are valid whatever the status: keep
+        # them, then report (stats[k].converged == 0 names the failed ones)
         self.X_i.index_copy_(0, self.perm, self.Xp)
         for k in np.nonzero(act)[0]:
             self.t[k] = t_next[k]
+        if rc != 0 and (raise_on_fail or rc not in (3, 4)):
+            try:
+                check(rc)
+            except _lib.MpetError as err:
+                err.stats = stats
+                raise
         return stats
 
     def commit(self, active=None):

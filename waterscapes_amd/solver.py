@@ -173,6 +173,7 @@ class StepEngine:
         # gain little: the coefficients (sum 2^(k+1) - 1) amplify the
         # solutions' own error toward the residual left by the guess
         self.hist = []                  # previous states u_{n-1}, u_{n-2}, .. (newest first)
+        self.hist_t = []                # their times
         self.space = space
         self.device = torch.device(device)
         self.dt, self.theta = float(dt), float(theta)
@@ -294,14 +295,35 @@ class StepEngine:
             self.s_term.add_source(b, prob.s, 1.0)
         return t_np1
 
+    def reset_history(self):
+        self.hist = []
+        self.hist_t = []
+
+    def _valid_history(self, t_n):
+        """Drop the stored states that are not u_{n-1}, u_{n-2}, .. at the
+        uniform spacing dt behind t_n (a step repeated from the same state,
+        a changed dt, a jump in time): the extrapolation weights assume
+        that spacing."""
+        dt = self.dt
+        k = 0
+        while k < len(self.hist_t) and abs(self.hist_t[k] - (t_n - (k + 1) * dt)) <= \
+                1e-9 * max(dt, 1e-300):
+            k += 1
+        del self.hist[k:]
+        del self.hist_t[k:]
+
     def step(self, up_prev, x, t_n, testing_guess=False):
         """One implicit step from up_prev (device FP64) into x; returns t_{n+1}."""
         t_np1 = self.assemble_rhs(up_prev, t_n)
         g = torch.as_tensor(self.bc_values(None)).to(self.device)
+        if self.extrapolate:
+            self._valid_history(float(t_n))
         if not testing_guess:
             extrapolated_guess(self.hist if self.extrapolate else [], self.ex_order, up_prev, x)
         if self.extrapolate:
             push_history(self.hist, self.ex_order, up_prev)
+            self.hist_t.insert(0, float(t_n))
+            del self.hist_t[self.ex_order:]
         if self.bc_dofs.numel():
             x.index_copy_(0, self.bc_dofs, g)
         st = self.krylov.solve(self.b, x)
@@ -343,6 +365,7 @@ class MPETTotalPressureSolver(object):
         self.up_ = up_
         self.up = up
         self._engine = None
+        self._engine_key_built = None
 
     @staticmethod
     def default_params():
@@ -414,9 +437,25 @@ class MPETTotalPressureSolver(object):
                             pamg=AmgParams(smoother_degree=int(p["p_smoother_degree"]),
                                            level_degrees=tuple(p["p_level_degrees"])))
 
+    def _engine_key(self, direct):
+        """Everything the device engine is built from that a caller may
+        change between runs: the solve path and precision, tolerances,
+        smoother settings, dt, theta and the initial-guess policy."""
+        p = self.params
+        keys = ("precision", "krylov_rtol", "outer_tol", "inner_rtol", "maxit", "p0_weight",
+                "extrapolate", "extrapolate_order", "u_smoother_degree", "p_smoother_degree",
+                "u_level_degrees", "p_level_degrees", "u_max_coarse", "dt", "theta", "device")
+        return (bool(direct),) + tuple(repr(p[k]) for k in keys)
+
     def engine(self, direct=None):
+        """The device engine of the requested solve path.  It is rebuilt when
+        the path (direct / iterative) or a parameter it depends on changed
+        since the last call, so solve_direct() after solve_iterative() runs
+        the direct path's FP64 solve, not the cached iterative one."""
         direct = self.params["direct_solver"] if direct is None else direct
-        if self._engine is None:
+        key = self._engine_key(direct)
+        if self._engine is None or self._engine_key_built != key:
+            self._engine = None
             cfg = self._config(direct)
             ex = self.params["extrapolate"]
             # default: on for the mixed path and for the direct path (whose
@@ -428,6 +467,7 @@ class MPETTotalPressureSolver(object):
                                       self.params["dt"], self.params["theta"], cfg,
                                       self.params["device"], extrapolate=ex,
                                       extrapolate_order=int(self.params["extrapolate_order"]))
+            self._engine_key_built = key
         return self._engine
 
     def _run(self, direct):
@@ -435,6 +475,7 @@ class MPETTotalPressureSolver(object):
         T = self.params["T"]
         time = self.problem.time
         eng = self.engine(direct)
+        eng.reset_history()
         self.solver_monitor["niter"] = []
         self.solver_monitor["outer"] = []
         self.solver_monitor["residual"] = []
