@@ -202,9 +202,10 @@ class ErrorEstimator:
             self._cache[key] = (expr, t)
         return t
 
-    def indicators(self):
-        """(zeta_p, zeta_u, zeta_u_dt, r4) per cell (numpy) for the solver's
-        current up and previous up_."""
+    def descriptor(self):
+        """The kernel descriptor with the current dt and source values set,
+        and the device tensors it points to (keep them alive while the
+        descriptor is used)."""
         d = self.desc
         d.dt = float(self.solver.params["dt"])
         f = self._values("f", self.problem.f, self.kf, 3)
@@ -216,6 +217,12 @@ class ErrorEstimator:
         d.f = ptr(f) if f is not None else None
         d.fp = ptr(fp) if fp is not None else None
         d.g = ptr(g) if g is not None else None
+        return d, (f, fp, g)
+
+    def indicators(self):
+        """(zeta_p, zeta_u, zeta_u_dt, r4) per cell (numpy) for the solver's
+        current up and previous up_."""
+        d, keep = self.descriptor()
         x = self.solver.up.data
         xp = self.solver.up_.data
         check(_lib.lib().mpet_error_indicators(C.byref(d), ptr(x), ptr(xp), ptr(self.fac),

@@ -241,19 +241,23 @@ bk4f_kernel(const double* __restrict__ partial, const int* __restrict__ r0,
   if (threadIdx.x == 0) {
     State s = st[k];
     s.oldb = s.beta;
+    // fmax(NaN, c) is c: test the raw sum and the rotation for NaN / inf
+    // before clamping, so a breakdown is reported instead of read as 0
+    const bool bad_in = !(tot == tot) || isinf(tot);
     s.beta = sqrt(fmax(tot, 0.0));
     s.oldeps = s.epsln;
     s.delta = s.cs * s.dbar + s.sn * s.alfa;
     const double gbar = s.sn * s.dbar - s.cs * s.alfa;
     s.epsln = s.sn * s.beta;
     s.dbar = -s.cs * s.beta;
-    s.gamma = fmax(hypot(gbar, s.beta), 1e-300);
+    const double hg = hypot(gbar, s.beta);
+    s.gamma = fmax(hg, 1e-300);
     s.cs = gbar / s.gamma;
     s.sn = s.beta / s.gamma;
     s.phi = s.cs * s.phibar;
     s.phibar = s.sn * s.phibar;
     s.it += 1;
-    const bool bad = !(s.phibar == s.phibar) || !(s.gamma == s.gamma) || isinf(s.phibar);
+    const bool bad = bad_in || !(hg == hg) || !(s.phibar == s.phibar) || isinf(s.phibar);
     s.nan = bad ? 1 : 0;
     const int stop = (bad || s.phibar <= s.tol || s.beta == 0.0 || s.it >= s.maxit) ? 1 : 0;
     s.done = stop;
@@ -319,13 +323,14 @@ binit_state_kernel(const double* __restrict__ partial, const int* __restrict__ r
   if (threadIdx.x == 0) {
     State s;
     memset(&s, 0, sizeof(s));
-    s.beta1 = sqrt(fmax(tot, 0.0));
+    const bool bad = !(tot == tot) || isinf(tot);    // before fmax, which drops NaN
+    s.beta1 = bad ? tot : sqrt(fmax(tot, 0.0));
     s.beta = s.beta1;
     s.phibar = s.beta1;
     s.cs = -1.0;
     s.tol = rtol >= 0 ? rtol * s.beta1 : -rtol * ref[k];
     s.maxit = maxit;
-    s.nan = !(s.beta1 == s.beta1) ? 1 : 0;
+    s.nan = bad ? 1 : 0;
     s.done = (s.beta1 == 0.0 || s.beta1 <= s.tol || s.nan || !keep[k]) ? 1 : 0;
     s.inactive[0] = s.done;
     s.inactive[1] = s.done;

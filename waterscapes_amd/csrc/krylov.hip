@@ -201,19 +201,23 @@ k4f_kernel(const double* __restrict__ partial, int np, MinresState* st, int cur,
   if (threadIdx.x == 0) {
     MinresState s = *st;
     s.oldb = s.beta;
+    // fmax(NaN, c) is c: test the raw sum and the rotation for NaN / inf
+    // before clamping, so a breakdown is reported instead of read as 0
+    const bool bad_in = !(tot == tot) || isinf(tot);
     s.beta = sqrt(fmax(tot, 0.0));
     s.oldeps = s.epsln;
     s.delta = s.cs * s.dbar + s.sn * s.alfa;
     double gbar = s.sn * s.dbar - s.cs * s.alfa;
     s.epsln = s.sn * s.beta;
     s.dbar = -s.cs * s.beta;
-    s.gamma = fmax(hypot(gbar, s.beta), 1e-300);
+    const double hg = hypot(gbar, s.beta);
+    s.gamma = fmax(hg, 1e-300);
     s.cs = gbar / s.gamma;
     s.sn = s.beta / s.gamma;
     s.phi = s.cs * s.phibar;
     s.phibar = s.sn * s.phibar;
     s.it += 1;
-    bool bad = !(s.phibar == s.phibar) || !(s.gamma == s.gamma) || isinf(s.phibar);
+    bool bad = bad_in || !(hg == hg) || !(s.phibar == s.phibar) || isinf(s.phibar);
     s.nan = bad ? 1 : 0;
     int conv = (bad || s.phibar <= s.tol || s.beta == 0.0 || s.it >= s.maxit) ? 1 : 0;
     s.done = conv;
@@ -256,14 +260,15 @@ minres_init_state_kernel(const double* __restrict__ partial, int np, MinresState
   if (threadIdx.x == 0) {
     MinresState s;
     memset(&s, 0, sizeof(s));
-    s.beta1 = sqrt(fmax(tot, 0.0));
+    const bool bad = !(tot == tot) || isinf(tot);    // before fmax, which drops NaN
+    s.beta1 = bad ? tot : sqrt(fmax(tot, 0.0));
     s.beta = s.beta1;
     s.phibar = s.beta1;
     s.cs = -1.0;
     s.sn = 0.0;
     s.tol = rtol >= 0 ? rtol * s.beta1 : -rtol;
     s.maxit = maxit;
-    s.nan = !(s.beta1 == s.beta1) ? 1 : 0;
+    s.nan = bad ? 1 : 0;
     s.done = (s.beta1 == 0.0 || s.beta1 <= s.tol || s.nan) ? 1 : 0;
     s.inactive[0] = s.done;
     s.inactive[1] = s.done;
