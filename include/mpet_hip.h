@@ -239,6 +239,19 @@ int mpet_error_indicators(const MpetIndDesc* d, const double* x, const double* x
                           const int32_t* fac, int nF, const int32_t* cell_f, double* tmp,
                           double* out, hipStream_t stream);
 
+/* mpet_error_indicators without the tmp workspace, allocating nothing: no
+ * counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight adaptive loops, harnesses that
+ * forbid hipMalloc).  The facet values live in a library-owned device array
+ * of 3 * MPET_NOALLOC_MAX_FACETS doubles (48 MiB) per GPU, held until the
+ * process ends; nF above the maximum returns MPET_ERR_INVALID.  Calls that
+ * overlap on one GPU share that array and race.  Same outputs as
+ * mpet_error_indicators. */
+#define MPET_NOALLOC_MAX_FACETS 2097152
+int mpet_error_indicators_noalloc(const MpetIndDesc* d, const double* x, const double* xp,
+                                  const int32_t* fac, int nF, const int32_t* cell_f,
+                                  double* out, hipStream_t stream);
+
 /* P2 mass int phi_a phi_b on the P2 node pattern (nnz22 doubles, device),
  * the vector mass of the u block that the reference adds to the
  * preconditioner with a displacement nullspace (inner(u, v) dx,

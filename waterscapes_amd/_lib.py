@@ -108,6 +108,7 @@ def lib():
                                C.c_int),
         "mpet_assemble_p2_mass": ([P, P, P, P], C.c_int),
         "mpet_error_indicators": ([P, P, P, P, C.c_int, P, P, P, P], C.c_int),
+        "mpet_error_indicators_noalloc": ([P, P, P, P, C.c_int, P, P, P], C.c_int),
         "mpet_constraint_rows": ([P, P, P, C.c_int, C.c_int, C.c_int, P, C.c_int, P, C.c_int,
                                   P, P], C.c_int),
         "mpet_rhs_pressure": ([P, P, C.c_int, C.c_int, C.c_int, P, P, P, P], C.c_int),

@@ -22,6 +22,7 @@
 // quadratic |RE_u|^2).  One thread per cell / per interior facet; the facet
 // values are summed into the cells by a gather kernel (deterministic).
 #include "mpet_ops.h"
+#include <mutex>
 #include <stdio.h>
 
 namespace mpet {
@@ -414,16 +415,9 @@ ind_gather_kernel(int nC, const int* __restrict__ cell_f, const double* __restri
 
 using namespace mpet;
 
-extern "C" int mpet_error_indicators(const MpetIndDesc* d, const double* x, const double* xp,
-                                     const int* fac, int nF, const int* cell_f, double* tmp,
-                                     double* out, hipStream_t stream) {
-  if (!d || !x || !xp || !out || !d->coords || !d->cells || !d->cell_p2 || d->nC < 1 ||
-      d->A < 1 || d->A > MPET_MAXF || d->nq < 1 || !d->qbary || !d->qw || !(d->dt > 0.0) ||
-      ((d->f || d->fp) && (!d->Lf || d->nkf < 1)) || (d->g && (!d->Lg || d->nkg < 1)) ||
-      nF < 0 || (nF > 0 && (!fac || !cell_f || !tmp))) {
-    mpet_set_error("mpet_error_indicators: invalid argument");
-    return ERR_INVALID;
-  }
+static int launch_indicators(const MpetIndDesc* d, const double* x, const double* xp,
+                             const int* fac, int nF, const int* cell_f, double* tmp,
+                             double* out, hipStream_t stream, const char* who) {
   hipLaunchKernelGGL(ind_cell_kernel, dim3((d->nC + 127) / 128), dim3(128), 0, stream, *d, x, xp,
                      out);
   if (nF > 0) {
@@ -435,9 +429,96 @@ extern "C" int mpet_error_indicators(const MpetIndDesc* d, const double* x, cons
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     char m[300];
-    snprintf(m, sizeof(m), "mpet_error_indicators: %s", hipGetErrorString(e));
+    snprintf(m, sizeof(m), "%s: %s", who, hipGetErrorString(e));
     mpet_set_error(m);
     return ERR_HIP;
   }
   return OK;
+}
+
+static bool indicators_args_ok(const MpetIndDesc* d, const double* x, const double* xp,
+                               const double* out, const int* fac, int nF, const int* cell_f) {
+  return d && x && xp && out && d->coords && d->cells && d->cell_p2 && d->nC >= 1 &&
+         d->A >= 1 && d->A <= MPET_MAXF && d->nq >= 1 && d->qbary && d->qw && d->dt > 0.0 &&
+         !((d->f || d->fp) && (!d->Lf || d->nkf < 1)) && !(d->g && (!d->Lg || d->nkg < 1)) &&
+         nF >= 0 && !(nF > 0 && (!fac || !cell_f));
+}
+
+extern "C" int mpet_error_indicators(const MpetIndDesc* d, const double* x, const double* xp,
+                                     const int* fac, int nF, const int* cell_f, double* tmp,
+                                     double* out, hipStream_t stream) {
+  if (!indicators_args_ok(d, x, xp, out, fac, nF, cell_f) || (nF > 0 && !tmp)) {
+    mpet_set_error("mpet_error_indicators: invalid argument");
+    return ERR_INVALID;
+  }
+  return launch_indicators(d, x, xp, fac, nF, cell_f, tmp, out, stream,
+                           "mpet_error_indicators");
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant: the facet values (3 doubles per interior facet,
+// the tmp workspace above) live in a file-scope device array, one copy per
+// GPU for the life of the process.  Every facet value is written by the facet
+// kernel before the gather reads it, so nothing needs clearing between
+// calls; calls that overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[3ll * MPET_NOALLOC_MAX_FACETS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+}  // namespace
+
+// Device address of this GPU's copy of scratch, looked up once per device
+// (with that device current) and kept.  The array's name is a host-side
+// stand-in; only this address may be passed to kernels.
+static int scratch_address(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    mpet_set_error("mpet_error_indicators_noalloc: no current device");
+    return ERR_HIP;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      char m[300];
+      snprintf(m, sizeof(m), "mpet_error_indicators_noalloc: hipGetSymbolAddress: %s",
+               hipGetErrorString(e));
+      mpet_set_error(m);
+      return ERR_HIP;
+    }
+    g_scratch[dev] = (double*)p;
+  }
+  *out = g_scratch[dev];
+  return OK;
+}
+
+extern "C" int mpet_error_indicators_noalloc(const MpetIndDesc* d, const double* x,
+                                             const double* xp, const int* fac, int nF,
+                                             const int* cell_f, double* out,
+                                             hipStream_t stream) {
+  if (!indicators_args_ok(d, x, xp, out, fac, nF, cell_f)) {
+    mpet_set_error("mpet_error_indicators_noalloc: invalid argument");
+    return ERR_INVALID;
+  }
+  if (nF > MPET_NOALLOC_MAX_FACETS) {
+    char m[200];
+    snprintf(m, sizeof(m),
+             "mpet_error_indicators_noalloc: %d interior facets exceed the maximum %d "
+             "(use mpet_error_indicators with a workspace)",
+             nF, (int)MPET_NOALLOC_MAX_FACETS);
+    mpet_set_error(m);
+    return ERR_INVALID;
+  }
+  double* tmp = nullptr;
+  if (nF > 0) {
+    const int rc = scratch_address(&tmp);
+    if (rc) return rc;
+  }
+  return launch_indicators(d, x, xp, fac, nF, cell_f, tmp, out, stream,
+                           "mpet_error_indicators_noalloc");
 }
