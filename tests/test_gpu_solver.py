@@ -13,6 +13,7 @@ import scipy.sparse.linalg as spla
 
 from conftest import cfg2_params, lognormal_K, oracle_mesh
 from oracle import mpet_oracle as O
+from oracle.vcycle import block_preconditioner, vcycle
 
 pytestmark = pytest.mark.gpu
 
@@ -42,69 +43,9 @@ def _setup(n=4, A=4, mode="fp64", seed=0, **kw):
     return params, space, dev, solver, dofs, K
 
 
-def _ref_vcycle(levels, cinv, A0, mask0, prm, b):
-    """numpy V-cycle with Chebyshev smoothing (columns of b solved at once)."""
-    from waterscapes_amd import amg
-    degs = [prm.degree(l) for l in range(len(levels))]
-    ops = [A0] + [sp.csr_matrix((lev["A_v"], lev["A_ci"], lev["A_rp"]), shape=(lev["n"],) * 2)
-                  for lev in levels[1:]]
-    Ps = [sp.csr_matrix((lev["P_v"], lev["P_ci"], lev["P_rp"]), shape=(lev["n"], lev["nc"]))
-          if lev["nc"] else None for lev in levels]
-
-    def smooth(l, b, x=None):
-        lev = levels[l]
-        deg = degs[l]
-        D = lev["dinv"][:, None]
-        lmax = lev["rho"] * prm.eig_safety
-        cd, cr = amg.chebyshev_coefficients(lmax / prm.eig_ratio, lmax, deg)
-        m = mask0 if (l == 0 and mask0 is not None) else np.zeros(lev["n"], bool)
-        if x is None:
-            d = cr[0] * D * b
-            d[m] = 0
-            x = d.copy()
-            start = 1
-        else:
-            d = np.zeros_like(b)
-            start = 0
-        for i in range(start, deg):
-            d = cd[i] * d + cr[i] * D * (b - ops[l] @ x)
-            d[m] = 0
-            x = x + d
-            x[m] = 0
-        return x
-
-    def vc(l, b):
-        if l == len(levels) - 1:
-            return cinv @ b
-        m = mask0 if (l == 0 and mask0 is not None) else np.zeros(levels[l]["n"], bool)
-        x = smooth(l, b)
-        r_ = b - ops[l] @ x
-        r_[m] = 0
-        x = x + Ps[l] @ vc(l + 1, Ps[l].T @ r_)
-        return smooth(l, b, x)
-
-    return vc(0, b)
-
-
-def _ref_precond(solver, r, compact=False):
-    """numpy restatement of the block preconditioner on the same data."""
-    s = solver.sys
-    cfg = solver.cfg
-    rp, ci = s.pat_host["22"]
-    lap = s.prim_t["Lap"].cpu().numpy()
-    c = getattr(solver, "compact", None) if compact else None
-    if c is not None:          # the FP32 hierarchy's half-precision finest level
-        lap = c["A_v16"].float().cpu().numpy().astype(np.float64) * c["vscale"]
-    A0 = sp.csr_matrix((lap, ci, rp), shape=(s.nP2, s.nP2))
-    mask0 = solver.mask0.cpu().numpy().astype(bool)
-    nu = s.nu
-    zu = _ref_vcycle(solver.levels, solver.cinv, A0, mask0, cfg.amg,
-                     r[:nu].reshape(-1, 3) / s.mu).reshape(-1)
-    F = s.F
-    rp_fm = r[nu:].reshape(s.nP1, F).T.reshape(-1, 1)          # field-major
-    zp = _ref_vcycle(solver.plevels, solver.pcinv, solver.Pp_host, None, cfg.pamg, rp_fm)
-    zp = zp.reshape(F, s.nP1).T.reshape(-1)
-    return np.concatenate([zu, zp])
+# the numpy restatement of the device preconditioner lives with the oracle
+_ref_vcycle = vcycle
+_ref_precond = block_preconditioner
 
 
 def test_precond_matches_numpy_restatement():

@@ -15,10 +15,12 @@ from .solver import MPETTotalPressureSolver, DirichletBC, Parameters
 from .mpetsolver import MPETSolver
 from .krylov import SolverConfig
 from .amg import AmgParams
+from .io import File, XDMFFile, load_mesh, save_mesh
 
 __all__ = ["Mesh", "BoxMesh", "UnitCubeMesh", "MeshFunction", "FacetFunction", "SubDomain",
            "CompiledSubDomain", "DOLFIN_EPS", "INVALID", "Constant", "Expression", "Function",
            "FacetNormal", "MixedSpace", "interpolate", "assign", "errornorm", "MPETProblem",
            "convert_to_mu_lmbda", "convert_to_E_nu", "MPETTotalPressureSolver", "MPETSolver",
            "DirichletBC",
-           "Parameters", "SolverConfig", "AmgParams"]
+           "Parameters", "SolverConfig", "AmgParams", "File", "XDMFFile", "load_mesh",
+           "save_mesh"]

@@ -392,6 +392,14 @@ class MPETTotalPressureSolver(object):
         params.add("u_level_degrees", (1,))
         params.add("p_level_degrees", ())
         params.add("u_max_coarse", 2500)
+        # plain FP64 path: "csr" = the scalar CSR matrix of A (a PETSc AIJ
+        # analogue, default), "structured" = the node-pattern operator the
+        # mixed path uses in FP64; host checks of the MINRES state every
+        # check_every iterations (0: 1 for FP64, 10 for mixed) and chunks
+        # replayed from a captured hipGraph (None: mixed only)
+        params.add("fp64_operator", "csr")
+        params.add("check_every", 0)
+        params.add("use_graphs", None)
         params.add("device", "cuda")
         return params
 
@@ -428,7 +436,12 @@ class MPETTotalPressureSolver(object):
             # the bordered systems of nullspace multipliers run in FP64 only
             return SolverConfig(mode="fp64", rtol=1e-12, maxit=int(p["maxit"]),
                                 p0_weight=p["p0_weight"])
+        if p["fp64_operator"] not in ("csr", "structured"):
+            raise ValueError("fp64_operator must be 'csr' or 'structured'")
         return SolverConfig(mode=p["precision"], rtol=float(p["krylov_rtol"]),
+                            use_csr=(p["fp64_operator"] == "csr") if p["precision"] == "fp64"
+                            else None,
+                            check_every=int(p["check_every"]), use_graphs=p["use_graphs"],
                             outer_tol=float(p["outer_tol"]), inner_rtol=float(p["inner_rtol"]),
                             maxit=int(p["maxit"]), p0_weight=p["p0_weight"],
                             amg=AmgParams(smoother_degree=int(p["u_smoother_degree"]),
@@ -444,7 +457,8 @@ class MPETTotalPressureSolver(object):
         p = self.params
         keys = ("precision", "krylov_rtol", "outer_tol", "inner_rtol", "maxit", "p0_weight",
                 "extrapolate", "extrapolate_order", "u_smoother_degree", "p_smoother_degree",
-                "u_level_degrees", "p_level_degrees", "u_max_coarse", "dt", "theta", "device")
+                "u_level_degrees", "p_level_degrees", "u_max_coarse", "fp64_operator",
+                "check_every", "use_graphs", "dt", "theta", "device")
         return (bool(direct),) + tuple(repr(p[k]) for k in keys)
 
     def engine(self, direct=None):
