@@ -446,7 +446,7 @@ def main_batch(args):
     t0 = time.perf_counter()
     solvers = [make_uneven_solver(N, p) for N, p in specs]
     batch = MPETBatch(solvers, precision="mixed", outer_tol=args.outer_tol,
-                      inner_rtol=args.inner_rtol)
+                      inner_rtol=args.inner_rtol, autotune=True)
     torch.cuda.synchronize()
     setup = time.perf_counter() - t0
     ndofs = sum(s.up.data.numel() for s in solvers)
@@ -466,6 +466,7 @@ def main_batch(args):
     wall = time.perf_counter() - t1
     inst_steps = batch.K * args.steps
     mem = torch.cuda.max_memory_allocated()
+    tuning = batch.tuning
     batch.close()
     del batch, solvers
     import gc
@@ -498,6 +499,7 @@ def main_batch(args):
         "dof_steps_per_s": round(rate * ndofs / len(specs), 1),
         "max_inner_iterations_per_step": iters,
         "setup_s": round(setup, 2), "max_hbm_bytes": int(mem),
+        "autotune": tuning,
         "fp64_one_at_a_time": {"instance_steps_per_s": round(seq_rate, 4),
                                "sample": [int(i) for i in idx]},
         "speedup_vs_fp64_one_at_a_time": round(rate / seq_rate, 3),
@@ -505,60 +507,119 @@ def main_batch(args):
     print(json.dumps(out))
 
 
-def main_serve(args):
-    """Serving: Poisson arrivals at `--rate` req/s for `--duration` s from the
-    uneven family (1 in 20 at N = 48, A = 4), 10..100 steps each (scaled by
-    `--steps-scale`), HBM budget `--budget` GB; p50/p99 latency and
-    aggregate instance-steps/s."""
-    import torch
-    from waterscapes_amd.serve import MPETServer, Request
-    rng = np.random.default_rng(args.seed)
-    t = 0.0
-    reqs = []
-    k = 0
-    while True:
-        t += rng.exponential(1.0 / args.rate)
-        if t > args.duration:
-            break
+def serve_specs(seed, nreq, steps_scale):
+    """Request specs of the serving stream: the uneven family with one
+    request in 20 at N = 48, A = 4, 10..100 steps each (scaled), and the
+    unit-rate exponential gaps of the Poisson arrivals."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(nreq):
         big = (k % 20) == 19
         N, p = uneven_instance(rng, big)
-        steps = max(1, int(round(rng.integers(10, 101) * args.steps_scale)))
-        reqs.append(Request(make_solver=lambda N=N, p=p: make_uneven_solver(N, p),
-                            steps=steps, arrival=t, rid=k))
-        k += 1
-    srv = MPETServer(hbm_budget=args.budget * 1e9, precision="mixed", outer_tol=args.outer_tol,
-                     inner_rtol=args.inner_rtol)
-    for r in reqs:
-        srv.submit(r)
-    t0 = time.perf_counter()
-    last = t0
-    while srv.waiting or srv.active:
-        if not srv.tick():
-            time.sleep(1e-3)
-        if time.perf_counter() - last > 10:
-            last = time.perf_counter()
-            print(f"[serve] t={last - t0:.0f}s active {len(srv.active)} waiting "
-                  f"{len(srv.waiting)} finished {len(srv.finished)}", file=sys.stderr, flush=True)
-    wall = time.perf_counter() - t0
-    lat = srv.latencies()
-    done = [r for r in srv.finished if not r.refused]
-    steps = sum(r.steps for r in done)
-    out = {
-        "metric": "served MPET requests: p50/p99 latency, aggregate instance-steps/s",
-        "value": round(steps / wall, 4), "unit": "instance-steps/s", "n_gpus": 1,
-        "steps": int(steps), "warmup": 0, "ms_per_step": round(1e3 * wall / max(srv.ticks, 1), 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32-in-fp64", "data": "synthetic",
-        "config": {"workload": f"Poisson arrivals {args.rate} req/s for {args.duration}s, "
-                               f"uneven family + 1 in 20 at 48^3 A=4, budget {args.budget} GB",
-                   "requests": len(reqs), "seed": args.seed},
-        "latency_s": {"p50": round(float(np.percentile(lat, 50)), 3),
-                      "p99": round(float(np.percentile(lat, 99)), 3)} if lat.size else None,
-        "refused": sum(r.refused for r in srv.finished),
-        "ticks": srv.ticks, "wall_s": round(wall, 2),
-        "max_hbm_bytes": int(torch.cuda.max_memory_allocated()),
-    }
-    print(json.dumps(out))
+        steps = max(1, int(round(rng.integers(10, 101) * steps_scale)))
+        specs.append((N, p, steps))
+    gaps = rng.exponential(1.0, nreq)
+    return specs, np.cumsum(gaps)
+
+
+def fp64_service_times(specs):
+    """Seconds each request takes alone on the plain FP64 GPU path (set-up
+    plus its steps; scalar CSR operator, guess up_, rtol 1e-10)."""
+    import gc
+    import torch
+    out = []
+    for N, p, steps in specs:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s = make_uneven_solver(N, p, precision="fp64")
+        gen = s.solve()
+        for _ in range(steps):
+            next(gen)
+        torch.cuda.synchronize()
+        out.append(time.perf_counter() - t0)
+        del gen, s
+        gc.collect()
+    return out
+
+
+def main_serve(args):
+    """Serving sweep: for each rate in `--rates`, the same `--requests`
+    requests (uneven family, 1 in 20 at 48^3 A = 4, 10..100 steps) arrive as
+    a Poisson stream at that rate and are served by continuous batching
+    (admission by HBM footprint, mixed precision); p50/p99 latency (arrival
+    to last step) and instance-steps/s.  Beside each: the plain FP64 path
+    serving the same arrivals one request at a time (FIFO; each request's
+    service time measured alone once, the queue computed from the
+    arrivals).  One JSON line per rate."""
+    import torch
+    from waterscapes_amd.serve import MPETServer, Request
+    specs, base = serve_specs(args.seed, args.requests, args.steps_scale)
+    rates = [float(r) for r in str(args.rates).split(",")]
+    print(f"[serve] {len(specs)} requests, {sum(s[2] for s in specs)} instance-steps, "
+          f"{sum(1 for s in specs if s[0] == 48)} at 48^3", file=sys.stderr, flush=True)
+    service = None
+    if not args.skip_fp64:
+        service = fp64_service_times(specs)
+        print(f"[serve] FP64 one at a time: {sum(service):.1f} s of service", file=sys.stderr,
+              flush=True)
+    for rate in rates:
+        arrivals = base / rate
+        reqs = [Request(make_solver=lambda N=N, p=p: make_uneven_solver(N, p), steps=st,
+                        arrival=float(a), rid=k)
+                for k, ((N, p, st), a) in enumerate(zip(specs, arrivals))]
+        torch.cuda.reset_peak_memory_stats()
+        srv = MPETServer(hbm_budget=args.budget * 1e9, precision="mixed",
+                         outer_tol=args.outer_tol, inner_rtol=args.inner_rtol)
+        for r in reqs:
+            srv.submit(r)
+        t0 = time.perf_counter()
+        last = t0
+        while srv.waiting or srv.active:
+            if not srv.tick():
+                time.sleep(1e-3)
+            if time.perf_counter() - last > 10:
+                last = time.perf_counter()
+                print(f"[serve] rate {rate}: t={last - t0:.0f}s active {len(srv.active)} "
+                      f"waiting {len(srv.waiting)} finished {len(srv.finished)}",
+                      file=sys.stderr, flush=True)
+        srv._drop_batch()
+        wall = time.perf_counter() - t0
+        lat = srv.latencies()
+        done = [r for r in srv.finished if not r.refused]
+        steps = sum(r.steps for r in done)
+        out = {
+            "metric": "served MPET requests: p50/p99 latency, aggregate instance-steps/s",
+            "value": round(steps / wall, 4), "unit": "instance-steps/s", "n_gpus": 1,
+            "steps": int(steps), "warmup": 0,
+            "ms_per_step": round(1e3 * wall / max(srv.ticks, 1), 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "fp32-in-fp64", "data": "synthetic",
+            "config": {"workload": f"Poisson arrivals at {rate} req/s of {len(specs)} requests, "
+                                   f"uneven family + 1 in 20 at 48^3 A=4, budget "
+                                   f"{args.budget} GB", "rate": rate, "requests": len(specs),
+                       "seed": args.seed},
+            "latency_s": {"p50": round(float(np.percentile(lat, 50)), 3),
+                          "p99": round(float(np.percentile(lat, 99)), 3)} if lat.size else None,
+            "refused": sum(r.refused for r in srv.finished),
+            "ticks": srv.ticks, "wall_s": round(wall, 2),
+            "max_hbm_bytes": int(torch.cuda.max_memory_allocated()),
+        }
+        if service is not None:
+            fin, lat64 = 0.0, []
+            for a, sv in zip(arrivals, service):
+                fin = max(fin, a) + sv
+                lat64.append(fin - a)
+            lat64 = np.array(lat64)
+            span = fin - float(arrivals[0])
+            out["fp64_one_at_a_time"] = {
+                "instance_steps_per_s": round(steps / max(fin, 1e-9), 4),
+                "latency_s": {"p50": round(float(np.percentile(lat64, 50)), 3),
+                              "p99": round(float(np.percentile(lat64, 99)), 3)},
+                "makespan_s": round(fin, 2),
+                "note": "FIFO over the same arrivals; per-request service times (set-up + "
+                        "steps, plain FP64 path) measured alone once"}
+            out["speedup_vs_fp64_one_at_a_time"] = round((steps / wall) / (steps / fin), 3)
+        print(json.dumps(out), flush=True)
 
 
 # two-field MPETSolver workload: the reference's own CPU-runnable material
@@ -925,8 +986,8 @@ def main():
     ap.add_argument("--instances", type=int, default=96)
     ap.add_argument("--sample", type=int, default=8)
     ap.add_argument("--seed", type=int, default=11)
-    ap.add_argument("--rate", type=float, default=1.0)
-    ap.add_argument("--duration", type=float, default=20.0)
+    ap.add_argument("--rates", default="10,50,200,500")
+    ap.add_argument("--requests", type=int, default=40)
     ap.add_argument("--steps-scale", type=float, default=1.0)
     ap.add_argument("--budget", type=float, default=200.0)
     args = ap.parse_args()
@@ -1065,6 +1126,9 @@ def main():
         "outer_iterations": mix_res[4],
         "max_rel_l2_err_vs_fp64_reference": max(errs) if errs else None,
         "hbm": bytes_report(eng, mix_res, True),
+        "autotune": {"picks": {str(k): v for k, v in eng.krylov.tuning.items()},
+                     "legend": "operator: [lanes per P2 row, lanes per P1 row, xcd_chunk]; "
+                               "levels: [A, P, R] lanes per row per level (0: unused)"},
     }
     if fp64_res is not None:
         fp_sps = K / fp64_res[0]

@@ -377,6 +377,21 @@ int mpet_amg_level_copy(void* h, int l, int64_t* A_rp, int* A_ci, double* A_v, d
 int mpet_amg_coarse_inverse(void* h, double* out);
 void mpet_amg_destroy(void* h);
 
+/* Launch shapes of a solver or batch handle, as picked by the package's
+ * autotuner (waterscapes_amd/autotune.py): lanes per node row of the
+ * structured operator's u and p rows (8, 16 or 32; used by the fused MINRES
+ * step and by mpet_op_apply), the run length of row tiles per XCD
+ * (xcd_chunk >= 0; < 0 keeps the current one), and the lanes per row of
+ * each hierarchy level's three SpMMs (level operator, prolongator,
+ * restriction; 1, 2, 4, .., 64).  Zero lanes keep the current value.  No
+ * counterpart in the reference (hypre and PETSc pick their own launch
+ * shapes, mpet/mpettotalpressuresolver.py:448). */
+typedef struct {
+  int32_t lanes_u, lanes_p, xcd_chunk;
+  int32_t u_lpr[MPET_MAXLEV][3];
+  int32_t p_lpr[MPET_MAXLEV][3];
+} MpetTuning;
+
 /* Krylov solver handle: MINRES + block preconditioner on the structured
  * (or CSR) operator — PETScKrylovSolver("minres","hypre_amg") (:448).
  * Borrows every device array of the descriptors (caller keeps them alive);
@@ -443,6 +458,23 @@ int mpet_solver_destroy(void* h);
  * Returns MPET_ERR_NOT_CONVERGED (stats filled) when maxit is reached. */
 int mpet_solver_solve(void* h, const double* b, double* x, MpetSolveStats* stats,
                       hipStream_t stream);
+
+/* One SpMM of an AMG level as the V-cycle launches it (the autotuner's
+ * timing probe): NC = 1 or 3 node-interleaved columns, mode 0 y = A x,
+ * 1 y = b - A x, 3 one Chebyshev step (d = cd d + cr dinv (b - A x),
+ * y = x + d); lpr lanes per row; v16 / dci / vscale the compact FP32 copy
+ * (dtype 1 only; v may then be NULL).  No counterpart in the reference. */
+int mpet_level_spmm(int dtype, int nc, int mode, int nrows, const int32_t* rp,
+                    const int32_t* ci, const void* v, const void* x, const void* b, void* y,
+                    const void* dinv, void* d, double cd, double cr, const uint8_t* mask,
+                    int lpr, const void* v16, const int16_t* dci, double vscale,
+                    hipStream_t stream);
+
+/* Apply launch shapes (MpetTuning) to the operator and preconditioner of
+ * precision dtype (0 FP64, 1 FP32) of a solver handle; captured graphs are
+ * dropped and captured again by the next solve.  Results change only by
+ * the summation order of the reductions inside a row. */
+int mpet_solver_set_tuning(void* h, int dtype, const MpetTuning* t);
 
 /* z = P^-1 r with the FP64 (dtype 0) or FP32 (1) preconditioner. */
 int mpet_precond_apply(void* h, int dtype, const void* r, void* z, hipStream_t stream);
@@ -515,6 +547,10 @@ int mpet_batch_solve(void* h, const double* b, double* x, const int32_t* active,
  * handle (dtype 0 FP64, 1 FP32), Dirichlet rows 0 (as mpet_op_apply). */
 int mpet_batch_apply(void* h, int dtype, const void* x, const void* b, void* y,
                      hipStream_t stream);
+
+/* Lanes per row of the batch's hierarchy levels (the u_lpr / p_lpr part of
+ * MpetTuning; the packed operator keeps its shapes); drops captured graphs. */
+int mpet_batch_set_tuning(void* h, int dtype, const MpetTuning* t);
 
 /* z = P^-1 r with the batch's block preconditioner (as mpet_precond_apply). */
 int mpet_batch_precond_apply(void* h, int dtype, const void* r, void* z, hipStream_t stream);

@@ -187,7 +187,7 @@ class MPETBatch:
 
     def __init__(self, solvers, precision="mixed", rtol=1e-10, outer_tol=3e-11,
                  inner_rtol=1e-6, maxit=5000, max_outer=30, check_every=10, use_graphs=True,
-                 extrapolate=None, extrapolate_order=4):
+                 extrapolate=None, extrapolate_order=4, autotune=False):
         if not torch.cuda.is_available():
             raise RuntimeError("MPETBatch needs a GPU (no CPU fallback)")
         if precision not in ("mixed", "fp64"):
@@ -227,6 +227,33 @@ class MPETBatch:
         self.handle = h
         self._state_views()
         self.stats = None
+        self.tuning = None
+        if autotune:
+            self.autotune()
+
+    def autotune(self, cache=None):
+        """Lanes per row of the packed hierarchies' level SpMMs picked by
+        the autotuner (waterscapes_amd/autotune.py) and applied to the
+        handle; the packed operator keeps its fixed shapes."""
+        from . import autotune as at
+        dt = 0 if self.precision == "fp64" else 1
+        out = {}
+        for name, nc in (("u", 3), ("p", 1)):
+            levels, devs = self.hier_levels[name]
+
+            def rule(l, which, levels=levels, devs=devs, nc=nc):
+                lev, ld = levels[l], devs[l]
+                if which == "A":
+                    return _pick_lpr(ld["A_ci"].numel() / max(lev["n"], 1), nc, lev["n"])
+                nnzp = ld["P_ci"].numel()
+                if which == "P":
+                    return _pick_lpr(nnzp / max(lev["n"], 1), nc, lev["n"])
+                return _pick_lpr(nnzp / max(lev["nc"], 1), nc, lev["nc"])
+            out[name] = at.tune_hierarchy(levels, devs, nc, dt, rule,
+                                          self.mask0 if name == "u" else None, cache)
+        at.apply_tuning(self.handle, dt, u=out["u"], p=out["p"], batch=True)
+        self.tuning = {"u_levels": out["u"], "p_levels": out["p"]}
+        return self.tuning
 
     # -- layout and packing ----------------------------------------------------
     def _layout(self):
@@ -341,11 +368,14 @@ class MPETBatch:
         dev = self.device
         tdt = torch.float64 if self.precision == "fp64" else torch.float32
         keep = []
+        devs = []                   # per level: the device arrays (autotuner)
         h.nlev = len(levels)
         if h.nlev > MAXLEV:
             raise ValueError("hierarchy too deep")
         for l, lev in enumerate(levels):
             L = h.lev[l]
+            ld = {}
+            devs.append(ld)
             L.n = lev["n"]
             L.nc = lev["nc"]
             if l == 0 and level0 is not None:
@@ -357,10 +387,12 @@ class MPETBatch:
             dinv = torch.as_tensor(lev["dinv"]).to(dev, tdt)
             grp = torch.as_tensor(_i32(lev["grp"])).to(dev)
             keep += [rp, ci, v, dinv, grp]
+            ld.update(A_rp=rp, A_ci=ci, A_v=v, dinv=dinv)
             L.A_rp, L.A_ci, L.A_v, L.dinv = ptr(rp), ptr(ci), ptr(v), ptr(dinv)
             L.grp = ptr(grp)
             if "A_v16" in lev:
                 keep += [lev["A_v16"], lev["A_dci"]]
+                ld.update(A_v16=lev["A_v16"], A_dci=lev["A_dci"], vscale=lev["vscale"])
                 L.A_v16, L.A_dci, L.A_vscale = ptr(lev["A_v16"]), ptr(lev["A_dci"]), lev["vscale"]
             L.rho = lev["rho"]
             L.lpr = _pick_lpr(ci.numel() / max(lev["n"], 1), nc, lev["n"])
@@ -374,6 +406,8 @@ class MPETBatch:
                     keep += list(a)
                 (L.P_rp, L.P_ci, L.P_v), (L.R_rp, L.R_ci, L.R_v) = \
                     [tuple(ptr(t) for t in a) for a in arrs]
+                ld.update(P_rp=arrs[0][0], P_ci=arrs[0][1], P_v=arrs[0][2], R_rp=arrs[1][0],
+                          R_ci=arrs[1][1], R_v=arrs[1][2])
                 nnzp = arrs[0][1].numel()
                 L.lprP = _pick_lpr(nnzp / max(lev["n"], 1), nc, lev["n"])
                 L.lprR = _pick_lpr(nnzp / max(lev["nc"], 1), nc, lev["nc"])
@@ -392,6 +426,7 @@ class MPETBatch:
         h.scale = 1.0
         h.mask0 = ptr(mask0) if mask0 is not None else None
         self.keep.append(keep)
+        return devs
 
     def _pack_preconditioner(self):
         dev = self.device
@@ -426,8 +461,9 @@ class MPETBatch:
         self.pmap = torch.as_tensor(_i32(pmap)).to(dev)
         d = PrecDesc()
         cfg = kry[0].cfg
-        self._upload_hier(d.u, ulev, ucoarse, cfg.amg, self.mask0, 3, level0=level0)
-        self._upload_hier(d.p, plev, pcoarse, cfg.pamg, None, 1)
+        udev = self._upload_hier(d.u, ulev, ucoarse, cfg.amg, self.mask0, 3, level0=level0)
+        pdev = self._upload_hier(d.p, plev, pcoarse, cfg.pamg, None, 1)
+        self.hier_levels = {"u": (ulev, udev), "p": (plev, pdev)}
         d.nP1 = int(self.pd_off[-1])
         d.F = 1
         d.pmap = ptr(self.pmap)

@@ -567,6 +567,31 @@ static void precond_apply(const Precond<T>& P, const T* r, T* z, const int* skip
   precond_apply(P, r, z, sk, s);
 }
 
+// Lanes per row of the levels' SpMMs from an MpetTuning table (0 keeps).
+static bool lpr_ok(int v) {
+  return v == 0 || v == 1 || v == 2 || v == 4 || v == 8 || v == 16 || v == 32 || v == 64;
+}
+static bool tuning_ok(const MpetTuning* t) {
+  if (!t) return false;
+  for (int l = 0; l < MPET_MAXLEV; ++l)
+    for (int k = 0; k < 3; ++k)
+      if (!lpr_ok(t->u_lpr[l][k]) || !lpr_ok(t->p_lpr[l][k])) return false;
+  return true;
+}
+template <typename T>
+static void set_hier_lpr(Hier<T>& H, const int32_t (*lpr)[3]) {
+  for (int l = 0; l < H.nlev && l < MPET_MAXLEV; ++l) {
+    if (lpr[l][0] > 0) H.lev[l].lpr = lpr[l][0];
+    if (lpr[l][1] > 0) H.lev[l].lprP = lpr[l][1];
+    if (lpr[l][2] > 0) H.lev[l].lprR = lpr[l][2];
+  }
+}
+template <typename T>
+static void set_precond_tuning(Precond<T>& P, const MpetTuning* t) {
+  set_hier_lpr(P.u, t->u_lpr);
+  set_hier_lpr(P.p, t->p_lpr);
+}
+
 template <typename T>
 static int alloc_dev(std::vector<void*>& allocs, T** p, size_t count) {
   void* q = nullptr;

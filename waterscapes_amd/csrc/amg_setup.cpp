@@ -26,6 +26,8 @@
 #include <vector>
 #include <algorithm>
 #include <omp.h>
+#include <chrono>
+#include <stdlib.h>
 #include "../../include/mpet_hip.h"
 
 extern "C" void mpet_set_error(const char* msg);
@@ -124,9 +126,31 @@ std::vector<double> diag_inv(const Csr& A) {
 
 // Largest eigenvalue of D^-1 A (similar to the SPD D^-1/2 A D^-1/2) by
 // power iteration from a fixed pseudo-random start.
+// Sum of f(i) over 0 <= i < n in a fixed order whatever the thread count:
+// fixed blocks of rows summed in parallel, then the block sums in order.
+template <typename Fn>
+double fixed_sum(int n, Fn f) {
+  constexpr int B = 4096;
+  const int nb = (n + B - 1) / B;
+  std::vector<double> part(nb, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int b = 0; b < nb; ++b) {
+    double t = 0.0;
+    const int hi = std::min(n, (b + 1) * B);
+    for (int i = b * B; i < hi; ++i) t += f(i);
+    part[b] = t;
+  }
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += part[b];
+  return s;
+}
+
+// rho(D^-1 A) by power iteration in the D-inner product from a fixed start
+// vector; every reduction in a fixed order, so the estimate (and the
+// smoothed prolongators built from it) does not depend on the thread count.
 double spectral_radius(const Csr& A, const std::vector<double>& dinv, int iters) {
   int n = A.n;
-  std::vector<double> x(n), y(n);
+  std::vector<double> x(n), y(n), t(n);
   uint64_t s = 0x9E3779B97F4A7C15ull;
   for (int i = 0; i < n; ++i) {
     s ^= s << 13; s ^= s >> 7; s ^= s << 17;
@@ -134,23 +158,17 @@ double spectral_radius(const Csr& A, const std::vector<double>& dinv, int iters)
   }
   double lam = 0;
   for (int it = 0; it < iters; ++it) {
-    double nrm = 0;
-#pragma omp parallel for reduction(+ : nrm)
-    for (int i = 0; i < n; ++i) nrm += x[i] * x[i] / dinv[i];
-    nrm = sqrt(nrm);
-    double num = 0;
-#pragma omp parallel for reduction(+ : num)
+    const double nrm = sqrt(fixed_sum(n, [&](int i) { return x[i] * x[i] / dinv[i]; }));
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) {
-      double t = 0;
-      for (int64_t k = A.rp[i]; k < A.rp[i + 1]; ++k) t += A.v[k] * x[A.ci[k]];
-      y[i] = dinv[i] * t / nrm;
-      num += x[i] * t / nrm;
+      double a = 0;
+      for (int64_t k = A.rp[i]; k < A.rp[i + 1]; ++k) a += A.v[k] * x[A.ci[k]];
+      t[i] = a / nrm;
+      y[i] = dinv[i] * t[i];
     }
-    lam = num / nrm;   // Rayleigh quotient in the D-inner product
-    double ny = 0;
-#pragma omp parallel for reduction(+ : ny)
-    for (int i = 0; i < n; ++i) ny += y[i] * y[i] / dinv[i];
-    ny = sqrt(ny);
+    const double num = fixed_sum(n, [&](int i) { return x[i] * t[i]; }) / nrm;
+    lam = num;         // Rayleigh quotient in the D-inner product
+    const double ny = sqrt(fixed_sum(n, [&](int i) { return y[i] * y[i] / dinv[i]; }));
     for (int i = 0; i < n; ++i) x[i] = y[i] / ny;
   }
   return lam;
@@ -217,40 +235,55 @@ int aggregate(const Csr& A, double eps, std::vector<int>& agg) {
   return na;
 }
 
+// Dense inverse of the SPD coarsest operator: Cholesky A = L L^T, W = L^-1
+// (stored transposed: row c of Wt is column c of W), A^-1 = W^T W.  Every
+// entry is one fixed-order dot product of contiguous rows, so the result
+// does not depend on the thread count; rows / columns are shared over the
+// OpenMP threads (n^3 / 3 multiply-adds for each of the three parts).
 bool cholesky_inverse(const Csr& A, std::vector<double>& inv) {
-  int n = A.n;
+  const int n = A.n;
   std::vector<double> L((size_t)n * n, 0.0);
   for (int i = 0; i < n; ++i)
     for (int64_t k = A.rp[i]; k < A.rp[i + 1]; ++k) L[(size_t)i * n + A.ci[k]] = A.v[k];
-  // in-place Cholesky (lower)
   for (int j = 0; j < n; ++j) {
-    double s = L[(size_t)j * n + j];
-    for (int k = 0; k < j; ++k) s -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
+    const double* Lj = &L[(size_t)j * n];
+    double s = Lj[j];
+    for (int k = 0; k < j; ++k) s -= Lj[k] * Lj[k];
     if (!(s > 0)) return false;
-    double ljj = sqrt(s);
+    const double ljj = sqrt(s);
     L[(size_t)j * n + j] = ljj;
+#pragma omp parallel for schedule(static)
     for (int i = j + 1; i < n; ++i) {
-      double t = L[(size_t)i * n + j];
-      for (int k = 0; k < j; ++k) t -= L[(size_t)i * n + k] * L[(size_t)j * n + k];
-      L[(size_t)i * n + j] = t / ljj;
+      double* Li = &L[(size_t)i * n];
+      double t = Li[j];
+      for (int k = 0; k < j; ++k) t -= Li[k] * Lj[k];
+      Li[j] = t / ljj;
     }
   }
-  inv.assign((size_t)n * n, 0.0);
-  std::vector<double> e(n), y(n);
+  // Wt[c][i] = (L^-1)[i][c], i >= c
+  std::vector<double> Wt((size_t)n * n, 0.0);
+#pragma omp parallel for schedule(dynamic, 8)
   for (int c = 0; c < n; ++c) {
-    std::fill(e.begin(), e.end(), 0.0);
-    e[c] = 1.0;
-    for (int i = 0; i < n; ++i) {
-      double t = e[i];
-      for (int k = 0; k < i; ++k) t -= L[(size_t)i * n + k] * y[k];
-      y[i] = t / L[(size_t)i * n + i];
+    double* w = &Wt[(size_t)c * n];
+    for (int i = c; i < n; ++i) {
+      const double* Li = &L[(size_t)i * n];
+      double t = (i == c) ? 1.0 : 0.0;
+      for (int k = c; k < i; ++k) t -= Li[k] * w[k];
+      w[i] = t / Li[i];
     }
-    for (int i = n - 1; i >= 0; --i) {
-      double t = y[i];
-      for (int k = i + 1; k < n; ++k) t -= L[(size_t)k * n + i] * e[k];
-      e[i] = t / L[(size_t)i * n + i];
+  }
+  // A^-1 [i][j] = sum_{k >= max(i, j)} W[k][i] W[k][j]
+  inv.assign((size_t)n * n, 0.0);
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int i = 0; i < n; ++i) {
+    const double* wi = &Wt[(size_t)i * n];
+    for (int j = i; j < n; ++j) {
+      const double* wj = &Wt[(size_t)j * n];
+      double t = 0.0;
+      for (int k = j; k < n; ++k) t += wi[k] * wj[k];
+      inv[(size_t)i * n + j] = t;
+      inv[(size_t)j * n + i] = t;
     }
-    for (int i = 0; i < n; ++i) inv[(size_t)i * n + c] = e[i];
   }
   return true;
 }
@@ -356,7 +389,12 @@ int mpet_amg_setup(int n, const int64_t* rp, const int* ci, const double* v,
   }
   Level& C = H->lev.back();
   H->nc = C.A.n;
-  if (!cholesky_inverse(C.A, H->coarse_inv)) {
+  auto tc0 = std::chrono::steady_clock::now();
+  const bool ok = cholesky_inverse(C.A, H->coarse_inv);
+  if (getenv("MPET_DEBUG"))
+    fprintf(stderr, "[amg] coarse inverse n=%d: %.3f s\n", C.A.n,
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - tc0).count());
+  if (!ok) {
     delete H;
     mpet_set_error("mpet_amg_setup: coarsest operator not SPD");
     return 3;

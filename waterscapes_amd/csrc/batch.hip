@@ -771,6 +771,23 @@ extern "C" int mpet_batch_precond_apply(void* h, int dtype, const void* r, void*
   return hipGetLastError() == hipSuccess ? OK : ERR_HIP;
 }
 
+// Lanes per row of the packed hierarchies' level SpMMs (the operator keeps
+// its shapes); the captured iteration graphs are dropped.
+extern "C" int mpet_batch_set_tuning(void* h, int dtype, const MpetTuning* t) {
+  Batch* B = (Batch*)h;
+  if (!B || (dtype != 0 && dtype != 1) || !tuning_ok(t)) {
+    mpet_set_error("mpet_batch_set_tuning: invalid argument");
+    return ERR_INVALID;
+  }
+  HIPCHK(hipDeviceSynchronize());
+  if (dtype == 1 && B->has_pc32) set_precond_tuning(B->pc32, t);
+  if (dtype == 0 && B->has_pc64) set_precond_tuning(B->pc64, t);
+  hipGraphExec_t& g = B->graph[dtype == 0 ? 0 : 1];
+  if (g) HIPCHK(hipGraphExecDestroy(g));
+  g = nullptr;
+  return OK;
+}
+
 // One time step of every instance with active[k] != 0 (NULL: all), same
 // contract per instance as mpet_solver_solve.  stats[k] is filled for every
 // instance (zeros for the inactive ones).  Returns MPET_ERR_NAN on a

@@ -70,52 +70,54 @@ __device__ __forceinline__ void k1_scalars(const MinresState* st, T& s, T& c1) {
   c1 = st->it > 0 ? (T)(st->beta / st->oldb) : (T)0;
 }
 
-constexpr int K1_G = 16;                 // lanes per node row
-constexpr int K1_ROWS = NT / K1_G;       // node rows per tile
-
-// K1 on the structured operator, grid-stride over 16-row tiles.
-template <typename T, int F>
+// K1 on the structured operator, grid-stride over row tiles: GU lanes per
+// P2 node row (tiles of NT / GU rows), GP lanes per P1 node row (tiles of
+// NT / GP rows); the u tiles come first.  The lane counts are the
+// operator's lanes_u / lanes_p (8, 16 or 32; picked by the autotuner).
+template <typename T, int F, int GU, int GP>
 __global__ void __launch_bounds__(NT)
 k1_op_kernel(MpetOpT<T> op, const T* __restrict__ z, EpiK1<T> epi, const MinresState* st,
              const int* __restrict__ skip, double* __restrict__ partial) {
   if (*skip) return;
   T s, c1;
   k1_scalars(st, s, c1);
-  const int lane = threadIdx.x % K1_G;
+  constexpr int RU = NT / GU, RP = NT / GP;
   const long long nu = 3ll * op.nP2;
-  const int ntu = (op.nP2 + K1_ROWS - 1) / K1_ROWS, ntp = (op.nP1 + K1_ROWS - 1) / K1_ROWS;
+  const int ntu = (op.nP2 + RU - 1) / RU, ntp = (op.nP1 + RP - 1) / RP;
   const long long n22 = op.nnz22, n21 = op.nnz21, n12 = op.nnz12, n11 = op.nnz11;
   double acc = 0.0;
   for (int t = blockIdx.x; t < ntu + ntp; t += gridDim.x) {
     const int tile = xcd_tile(t, ntu + ntp, op.xcd_chunk);
     if (tile < ntu) {
-      const int a = tile * K1_ROWS + threadIdx.x / K1_G;
+      const int lane = threadIdx.x % GU;
+      const int a = tile * RU + threadIdx.x / GU;
       if (a < op.nP2) {
         T y0 = 0, y1 = 0, y2 = 0;
-        urow_apply<T, K1_G>(op.rp22, op.ci22, op.Kuu, n22, op.rp21, op.ci21, op.Dv, n21, z, nu,
-                            F, a, lane, y0, y1, y2);
-        y0 = group_allsum<K1_G>(y0);
-        y1 = group_allsum<K1_G>(y1);
-        y2 = group_allsum<K1_G>(y2);
+        urow_apply<T, GU>(op.rp22, op.ci22, op.Kuu, n22, op.rp21, op.ci21, op.Dv, n21, z, nu,
+                          F, a, lane, y0, y1, y2);
+        y0 = group_allsum<GU>(y0);
+        y1 = group_allsum<GU>(y1);
+        y2 = group_allsum<GU>(y2);
         if (lane < 3) {
           const T yc = lane == 0 ? y0 : (lane == 1 ? y1 : y2);
           acc += epi(3ll * a + lane, yc, s, c1);
         }
       }
     } else {
-      const int c = (tile - ntu) * K1_ROWS + threadIdx.x / K1_G;
+      const int lane = threadIdx.x % GP;
+      const int c = (tile - ntu) * RP + threadIdx.x / GP;
       if (c < op.nP1) {
         T yd = 0, m[F], sv[F];
 #pragma unroll
         for (int g = 0; g < F; ++g) { m[g] = 0; sv[g] = 0; }
-        prow_apply<T, K1_G, F>(op.rp12, op.ci12, op.Dt, n12, op.rp11, op.ci11, op.M1, op.St, n11,
-                               0, z, nu, F, F, c, lane, yd, m, sv);
-        yd = group_allsum<K1_G>(yd);
+        prow_apply<T, GP, F>(op.rp12, op.ci12, op.Dt, n12, op.rp11, op.ci11, op.M1, op.St, n11,
+                             0, z, nu, F, F, c, lane, yd, m, sv);
+        yd = group_allsum<GP>(yd);
         T mine = 0;
 #pragma unroll
         for (int g = 0; g < F; ++g) {
-          m[g] = group_allsum<K1_G>(m[g]);
-          T svg = group_allsum<K1_G>(sv[g]);
+          m[g] = group_allsum<GP>(m[g]);
+          T svg = group_allsum<GP>(sv[g]);
           if (lane == g) mine = (g == 0) ? yd : (T)op.stiff[g] * svg;
         }
         if (lane < F) {
@@ -128,6 +130,13 @@ k1_op_kernel(MpetOpT<T> op, const T* __restrict__ z, EpiK1<T> epi, const MinresS
   }
   double bs = block_sum(acc);
   if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+}
+
+// Lane counts of the fused K1 (8, 16 or 32; anything else the default 16,
+// and GP at least the number of fields).
+static int k1_lanes(int g, int lo) {
+  const int v = (g == 8 || g == 16 || g == 32) ? g : 16;
+  return v < lo ? 16 : v;
 }
 
 // K1 on a scalar CSR matrix (plain FP64 path), LPR lanes per row.
@@ -565,6 +574,42 @@ static int build_minres(MpetSolver* S, Minres<T>& M) {
 
 static int dot_grid(long long n) { return grid_for(n, NT * 8, 2048); }
 
+template <typename T, int F, int GU>
+static void launch_k1_gp(int gp, dim3 grid, hipStream_t s, const MpetOpT<T>& op, const T* z,
+                         EpiK1<T> epi, const MinresState* st, const int* skip,
+                         double* partial) {
+  switch (gp) {
+    case 8: hipLaunchKernelGGL((k1_op_kernel<T, F, GU, 8>), grid, dim3(NT), 0, s, op, z, epi, st, skip, partial); break;
+    case 32: hipLaunchKernelGGL((k1_op_kernel<T, F, GU, 32>), grid, dim3(NT), 0, s, op, z, epi, st, skip, partial); break;
+    default: hipLaunchKernelGGL((k1_op_kernel<T, F, GU, 16>), grid, dim3(NT), 0, s, op, z, epi, st, skip, partial); break;
+  }
+}
+template <typename T, int F>
+static void launch_k1_f(int gu, int gp, dim3 grid, hipStream_t s, const MpetOpT<T>& op,
+                        const T* z, EpiK1<T> epi, const MinresState* st, const int* skip,
+                        double* partial) {
+  switch (gu) {
+    case 8: launch_k1_gp<T, F, 8>(gp, grid, s, op, z, epi, st, skip, partial); break;
+    case 32: launch_k1_gp<T, F, 32>(gp, grid, s, op, z, epi, st, skip, partial); break;
+    default: launch_k1_gp<T, F, 16>(gp, grid, s, op, z, epi, st, skip, partial); break;
+  }
+}
+template <typename T>
+static void launch_k1_op(int F, int gu, int gp, dim3 grid, hipStream_t s, const MpetOpT<T>& op,
+                         const T* z, EpiK1<T> epi, const MinresState* st, const int* skip,
+                         double* partial) {
+  switch (F) {
+    case 2: launch_k1_f<T, 2>(gu, gp, grid, s, op, z, epi, st, skip, partial); break;
+    case 3: launch_k1_f<T, 3>(gu, gp, grid, s, op, z, epi, st, skip, partial); break;
+    case 4: launch_k1_f<T, 4>(gu, gp, grid, s, op, z, epi, st, skip, partial); break;
+    case 5: launch_k1_f<T, 5>(gu, gp, grid, s, op, z, epi, st, skip, partial); break;
+    case 6: launch_k1_f<T, 6>(gu, gp, grid, s, op, z, epi, st, skip, partial); break;
+    case 7: launch_k1_f<T, 7>(gu, gp, grid, s, op, z, epi, st, skip, partial); break;
+    case 8: launch_k1_f<T, 8>(gu, gp, grid, s, op, z, epi, st, skip, partial); break;
+    default: break;
+  }
+}
+
 // One MINRES iteration i (i selects the buffer rotation and skip flag).
 template <typename T>
 static void minres_iter(MpetSolver* S, Minres<T>& M, const Precond<T>& P, bool csr, int i,
@@ -625,15 +670,10 @@ static void minres_iter(MpetSolver* S, Minres<T>& M, const Precond<T>& P, bool c
     const MpetOpT<T>* op;
     if constexpr (sizeof(T) == 8) op = &S->op64; else op = &S->op32;
     epi.mask = op->dmask;
-    long long ntiles = (op->nP2 + K1_ROWS - 1) / K1_ROWS + (op->nP1 + K1_ROWS - 1) / K1_ROWS;
+    const int gu = k1_lanes(op->lanes_u, 3), gp = k1_lanes(op->lanes_p, S->F);
+    long long ntiles = (op->nP2 + NT / gu - 1) / (NT / gu) + (op->nP1 + NT / gp - 1) / (NT / gp);
     np = grid_for(ntiles, 1, MAX_PARTIALS);
-#define K1OP(FF)                                                                                \
-  case FF:                                                                                      \
-    hipLaunchKernelGGL((k1_op_kernel<T, FF>), dim3(np), dim3(NT), 0, s, *op, M.z, epi, M.st, skip, \
-                       M.partial);                                                              \
-    break;
-    switch (S->F) { K1OP(2) K1OP(3) K1OP(4) K1OP(5) K1OP(6) K1OP(7) K1OP(8) default: break; }
-#undef K1OP
+    launch_k1_op<T>(S->F, gu, gp, dim3(np), s, *op, M.z, epi, M.st, skip, M.partial);
   }
   if (!(csr && S->mb))
     hipLaunchKernelGGL(k1f_kernel, dim3(1), dim3(NT), 0, s, M.partial, np, M.st, skip);
@@ -1060,6 +1100,94 @@ extern "C" int mpet_solver_solve(void* h, const double* b, double* x, MpetSolveS
     mpet_set_error("MINRES did not converge within maxit");
     return ERR_NOT_CONVERGED;
   }
+  return OK;
+}
+
+// One level SpMM as the V-cycle launches it (the autotuner's probe).
+template <typename T, int NC>
+static int level_spmm_t(int mode, int nrows, const int* rp, const int* ci, const void* v,
+                        const void* x, const void* b, void* y, const void* dinv, void* d,
+                        double cd, double cr, const uint8_t* mask, int lpr, const void* v16,
+                        const int16_t* dci, double vscale, hipStream_t s) {
+  LevelDev<T> L;
+  L.n = nrows;
+  L.rp = rp;
+  L.ci = ci;
+  L.v = (const T*)v;
+  L.dinv = (const T*)dinv;
+  L.mask = mask;
+  L.lpr = lpr;
+  L.v16 = (const __half*)v16;
+  L.dci = dci;
+  L.vscale = vscale;
+  Skip sk;
+  switch (mode) {
+    case 0: spmm<T, NC, 0>(L, 0, (const T*)x, nullptr, (T*)y, nullptr, 0, 0, sk, s); break;
+    case 1: spmm<T, NC, 1>(L, 0, (const T*)x, (const T*)b, (T*)y, nullptr, 0, 0, sk, s); break;
+    case 3: spmm<T, NC, 3>(L, 0, (const T*)x, (const T*)b, (T*)y, (T*)d, (T)cd, (T)cr, sk, s);
+            break;
+    default: return ERR_INVALID;
+  }
+  return OK;
+}
+
+extern "C" int mpet_level_spmm(int dtype, int nc, int mode, int nrows, const int* rp,
+                               const int* ci, const void* v, const void* x, const void* b,
+                               void* y, const void* dinv, void* d, double cd, double cr,
+                               const uint8_t* mask, int lpr, const void* v16,
+                               const int16_t* dci, double vscale, hipStream_t s) {
+  if ((dtype != 0 && dtype != 1) || (nc != 1 && nc != 3) || nrows < 0 || !rp || !ci ||
+      !(v || v16) || !x || !y || !lpr_ok(lpr) || lpr == 0 || (v16 && (dtype != 1 || !dci)) ||
+      ((mode == 1 || mode == 3) && !b) || (mode == 3 && (!d || !dinv))) {
+    mpet_set_error("mpet_level_spmm: invalid argument");
+    return ERR_INVALID;
+  }
+  int rc;
+  if (dtype == 0)
+    rc = nc == 3 ? level_spmm_t<double, 3>(mode, nrows, rp, ci, v, x, b, y, dinv, d, cd, cr,
+                                           mask, lpr, nullptr, nullptr, 1.0, s)
+                 : level_spmm_t<double, 1>(mode, nrows, rp, ci, v, x, b, y, dinv, d, cd, cr,
+                                           mask, lpr, nullptr, nullptr, 1.0, s);
+  else
+    rc = nc == 3 ? level_spmm_t<float, 3>(mode, nrows, rp, ci, v, x, b, y, dinv, d, cd, cr,
+                                          mask, lpr, v16, dci, vscale, s)
+                 : level_spmm_t<float, 1>(mode, nrows, rp, ci, v, x, b, y, dinv, d, cd, cr,
+                                          mask, lpr, v16, dci, vscale, s);
+  if (rc) {
+    mpet_set_error("mpet_level_spmm: invalid mode");
+    return rc;
+  }
+  return hipGetLastError() == hipSuccess ? OK : ERR_HIP;
+}
+
+// Launch shapes from the autotuner; the captured iteration graphs are
+// dropped (they hold the old launch configurations).
+extern "C" int mpet_solver_set_tuning(void* h, int dtype, const MpetTuning* t) {
+  MpetSolver* S = (MpetSolver*)h;
+  const bool lanes_ok = t && (t->lanes_u == 0 || t->lanes_u == 8 || t->lanes_u == 16 ||
+                              t->lanes_u == 32) &&
+                        (t->lanes_p == 0 || t->lanes_p == 8 || t->lanes_p == 16 ||
+                         t->lanes_p == 32);
+  if (!S || (dtype != 0 && dtype != 1) || !tuning_ok(t) || !lanes_ok) {
+    mpet_set_error("mpet_solver_set_tuning: invalid argument");
+    return ERR_INVALID;
+  }
+  HIPCHK(hipDeviceSynchronize());
+  if (dtype == 1) {
+    if (t->lanes_u) S->op32.lanes_u = t->lanes_u;
+    if (t->lanes_p) S->op32.lanes_p = t->lanes_p;
+    if (t->xcd_chunk >= 0) S->op32.xcd_chunk = t->xcd_chunk;
+    if (S->has_pc32) set_precond_tuning(S->pc32, t);
+  } else {
+    if (t->lanes_u) S->op64.lanes_u = t->lanes_u;
+    if (t->lanes_p) S->op64.lanes_p = t->lanes_p;
+    if (t->xcd_chunk >= 0) S->op64.xcd_chunk = t->xcd_chunk;
+    if (S->has_pc64) set_precond_tuning(S->pc64, t);
+  }
+  // both graphs: the FP64 one of mode 2 holds the FP32 preconditioner
+  if (S->graph32) HIPCHK(hipGraphExecDestroy(S->graph32));
+  if (S->graph64) HIPCHK(hipGraphExecDestroy(S->graph64));
+  S->graph32 = S->graph64 = nullptr;
   return OK;
 }
 

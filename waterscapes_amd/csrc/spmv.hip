@@ -1,7 +1,7 @@
 // Sparse operator application for the MPET system.
 //
 //  * csr_spmv: scalar CSR (the assembled AIJ matrix of the plain FP64 path),
-//    LPR lanes per row (8/16/32/64 chosen per matrix by the autotuner).
+//    LPR lanes per row (4..64, from the average row length of the matrix).
 //  * op_apply: the structured total-pressure operator on the node patterns,
 //    values stored structure-of-arrays so that every value load of a wave is
 //    one contiguous 256-byte (FP32) or 512-byte (FP64) segment:

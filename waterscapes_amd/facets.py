@@ -82,6 +82,8 @@ class FacetSet:
         self.normals = nrm * sgn[:, None]
         self.area_d = torch.as_tensor(self.area).to(device)
         self._inc = {}
+        self._lat_dev = {}
+        self._nrm_dev = {}
 
     def lattice(self, k):
         """(nF, nk, 3) coordinates of the P_k nodes of every facet."""
@@ -106,21 +108,31 @@ class FacetSet:
         return self._inc[which]
 
     def eval(self, expr, t=None):
-        """Nodal values (nF, nk, ncomp) of an Expression interpolated into
-        P_degree on every facet (normals passed to traction expressions)."""
+        """Nodal values (nF, nk, ncomp), a device tensor, of an Expression
+        interpolated into P_degree on every facet (normals passed to
+        traction expressions).  Evaluated where the facet lattices live
+        (Expression.eval_device); host-only coefficients are copied."""
         k = int(getattr(expr, "degree", 0))
-        Xf = self.lattice(k)
-        nF, nk = Xf.shape[:2]
-        pts = Xf.reshape(-1, 3)
+        if k not in self._lat_dev:
+            Xf = self.lattice(k)
+            self._lat_dev[k] = (torch.as_tensor(Xf.reshape(-1, 3)).to(self.device), Xf.shape[1])
+        Xd, nk = self._lat_dev[k]
+        nF = self.n
         if hasattr(expr, "eval_normal"):
-            v = expr.eval_normal(pts, np.repeat(self.normals, nk, axis=0), t)
+            if self._nrm_dev.get(k) is None:
+                self._nrm_dev[k] = torch.as_tensor(np.repeat(self.normals, nk, axis=0)).to(
+                    self.device)
+            v = expr.eval_normal_device(Xd, self._nrm_dev[k], t)
+        elif hasattr(expr, "eval_device"):
+            v = expr.eval_device(Xd, t)
         else:
-            v = expr.eval(pts, t)
-        v = np.asarray(v, dtype=np.float64)
-        if v.ndim == 0:
-            v = np.full(nF * nk, float(v))
+            v = torch.as_tensor(np.asarray(expr.eval(Xd.cpu().numpy(), t),
+                                           dtype=np.float64)).to(self.device)
+        v = v.to(torch.float64)
+        if v.dim() == 0:
+            v = v.expand(nF * nk)
         v = v.reshape(nF * nk, -1)
-        return np.ascontiguousarray(v).reshape(nF, nk, v.shape[1]), k
+        return v.contiguous().reshape(nF, nk, v.shape[1]), k
 
 
 class FacetTerm:
@@ -143,6 +155,7 @@ class FacetTerm:
         self.row_dof = torch.as_tensor(row_dof.astype(np.int64)).to(fs.device)
         self.loc = torch.empty(max(fs.n, 1) * ntest * ncomp, dtype=torch.float64,
                                device=fs.device)
+        self._tables = {}
 
     def _scatter(self, b):
         check(_bind().mpet_facet_scatter_vec(int(self.rows.size), self.ntest, self.ncomp,
@@ -159,7 +172,7 @@ class FacetTerm:
             raise ValueError(f"facet coefficient has {u.shape[2]} components, "
                              f"expected {self.ncomp}")
         T = np.asarray(refel.facet_source_weights(self.test_degree, k))[:, :, None]
-        self._local(T, torch.as_tensor(u).to(fs.device), None, None, scale)
+        self._local(T, u, None, None, scale)
         self._scatter(b)
 
     def add_product(self, b, expr_u, expr_w, scale, t=None):
@@ -169,8 +182,7 @@ class FacetTerm:
         u, ku = fs.eval(expr_u, t)
         w, kw = fs.eval(expr_w, t)
         T = facet_product_weights(ku, kw, self.test_degree)
-        self._local(T, torch.as_tensor(u).to(fs.device),
-                    torch.as_tensor(w[:, :, 0]).to(fs.device), None, scale)
+        self._local(T, u, w[:, :, 0].contiguous(), None, scale)
         self._scatter(b)
 
     def add_robin_prev(self, b, expr_beta, x, gdof, scale, t=None):
@@ -181,13 +193,17 @@ class FacetTerm:
             return
         w, kw = fs.eval(expr_beta, t)
         T = facet_product_weights(1, kw, self.test_degree)
-        self._local(T, None, torch.as_tensor(w[:, :, 0]).to(fs.device), (x, gdof), scale)
+        self._local(T, None, w[:, :, 0].contiguous(), (x, gdof), scale)
         self._scatter(b)
 
     def _local(self, T, u, w, gather, scale):
         fs = self.fs
         nm, nn = T.shape[1], T.shape[2]
-        Td = torch.as_tensor(np.ascontiguousarray(T, dtype=np.float64)).to(fs.device)
+        Th = np.ascontiguousarray(T, dtype=np.float64)
+        key = (Th.shape, Th.tobytes())           # reference tables: uploaded once
+        Td = self._tables.get(key)
+        if Td is None:
+            Td = self._tables[key] = torch.as_tensor(Th).to(fs.device)
         self._keep = (Td, u, w)
         if gather is not None:
             x, gdof = gather
@@ -213,7 +229,7 @@ def robin_matrix(fs: FacetSet, expr_beta, t, pat_dev, vals, scale=1.0):
     Td = torch.as_tensor(np.ascontiguousarray(T)).to(dev)
     eye = torch.eye(3, dtype=torch.float64, device=dev).reshape(-1)   # u[m][c] = delta
     loc = torch.empty(fs.n * 9, dtype=torch.float64, device=dev)
-    wd = torch.as_tensor(np.ascontiguousarray(w[:, :, 0])).to(dev)
+    wd = w[:, :, 0].contiguous()
     L = _bind()
     check(L.mpet_facet_local(fs.n, 3, 3, T.shape[2], 3, ptr(Td), ptr(fs.area_d), ptr(eye), 0,
                              None, None, ptr(wd), float(scale), ptr(loc), _lib.stream_ptr()))

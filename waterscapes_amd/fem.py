@@ -206,6 +206,17 @@ class Constant:
 _CFUNCS = {"pow": "Pow", "fabs": "Abs", "abs": "Abs"}
 
 
+def _torch_functions():
+    """The functions sympy prints for C expressions, as torch operations."""
+    import torch
+    return {"sin": torch.sin, "cos": torch.cos, "tan": torch.tan, "asin": torch.asin,
+            "acos": torch.acos, "atan": torch.atan, "atan2": torch.atan2, "sinh": torch.sinh,
+            "cosh": torch.cosh, "tanh": torch.tanh, "exp": torch.exp, "log": torch.log,
+            "sqrt": torch.sqrt, "Abs": torch.abs, "floor": torch.floor,
+            "ceiling": torch.ceil, "Min": torch.minimum, "Max": torch.maximum,
+            "sign": torch.sign, "erf": torch.erf}
+
+
 def _cpp_to_sympy(src):
     s = re.sub(r"x\[(\d)\]", r"x\1", src)
     s = s.replace("M_PI", "pi")
@@ -271,6 +282,36 @@ class Expression:
                 vals.append(float(v))
         return vals
 
+    def eval_device(self, X, t=None):
+        """eval() on a device tensor X (n, 3) float64, computed with torch
+        operations where X lives (the per-step source update stays on the
+        GPU).  A Python-callable Expression is evaluated on the host and
+        copied, since its code is opaque."""
+        import torch
+        n = X.shape[0]
+        if self._callable is not None:
+            v = self.eval(X.detach().cpu().numpy(), t)
+            return torch.as_tensor(np.ascontiguousarray(v, dtype=np.float64)).to(X.device)
+        if getattr(self, "_torch", None) is None:
+            import sympy
+            xs = sympy.symbols("x0 x1 x2")
+            args = list(xs) + [self._syms[k] for k in self.params]
+            self._torch = [sympy.lambdify(args, e, modules=[_torch_functions(), "math"])
+                           for e in self._exprs]
+        # parameters as 0-d device tensors: functions of t alone then run in
+        # torch too (a float argument is not accepted by torch.sin etc.)
+        pv = [torch.tensor(v, dtype=torch.float64, device=X.device)
+              for v in self._param_values(t)]
+        out = []
+        for f in self._torch:
+            v = f(X[:, 0], X[:, 1], X[:, 2], *pv)
+            if not isinstance(v, torch.Tensor):
+                v = torch.full((n,), float(v), dtype=torch.float64, device=X.device)
+            out.append(v.to(torch.float64).expand(n))
+        if self.shape == ():
+            return out[0].contiguous()
+        return torch.stack(out, dim=1).reshape((n,) + tuple(self.shape))
+
     def eval(self, X, t=None):
         """Values at points X (n, 3) -> (n,) or (n,)+shape (float64).
         ``t`` overrides the value of a Constant parameter named t."""
@@ -310,6 +351,14 @@ class FacetNormalTraction:
         if s.ndim == 1:                                # scalar * n (test_donut.py:45)
             return s[:, None] * normals
         return np.einsum("nij,nj->ni", s, normals)     # n,3,3
+
+    def eval_normal_device(self, X, normals, t=None):
+        """eval_normal on device tensors (sigma evaluated with torch)."""
+        import torch
+        s = self.sigma.eval_device(X, t)
+        if s.dim() == 1:
+            return s[:, None] * normals
+        return torch.einsum("nij,nj->ni", s, normals)
 
 
 class FacetNormal:

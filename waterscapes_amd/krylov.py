@@ -117,6 +117,11 @@ class SolverConfig:
     # fixed symmetric operator, so MINRES is unaffected beyond the slightly
     # perturbed preconditioner
     compact_level0: bool = True
+    # launch shapes picked by the autotuner (waterscapes_amd/autotune.py):
+    # None = on for systems of at least autotune_min_dofs unknowns (below,
+    # launches are latency-bound and the fixed heuristic is kept)
+    autotune: bool = None
+    autotune_min_dofs: int = 200000
 
 
 def set_level_degrees(h, params: AmgParams):
@@ -284,6 +289,54 @@ class KrylovSolver:
                                    C.byref(self.opt), C.byref(h)))
         self.handle = h
         self.last_stats = None
+        self.tuning = {}
+        tune = cfg.autotune if cfg.autotune is not None else system.n >= cfg.autotune_min_dofs
+        if tune:
+            self.autotune()
+
+    def level_heuristic(self, levels, levs, nc):
+        """The fixed lanes-per-row rule (_pick_lpr) of level l's operator,
+        prolongator or restriction: the autotuner's incumbent."""
+        def rule(l, which):
+            lev, ld = levels[l], levs[l]
+            if which == "A":
+                return _pick_lpr(ld["A_ci"].numel() / max(lev["n"], 1), nc, lev["n"])
+            nnzp = ld["P_ci"].numel()
+            if which == "P":
+                return _pick_lpr(nnzp / max(lev["n"], 1), nc, lev["n"])
+            return _pick_lpr(nnzp / max(lev["nc"], 1), nc, lev["nc"])
+        return rule
+
+    def autotune(self, cache=None):
+        """Pick and apply the launch shapes of the operator(s) and level
+        SpMMs this solver runs (waterscapes_amd/autotune.py); the picks are
+        kept in self.tuning."""
+        from . import autotune as at
+        s = self.sys
+        dtypes = []
+        if self.prec32 is not None:
+            dtypes.append(1)
+        if self.prec64 is not None:
+            dtypes.append(0)
+        for dt in dtypes:
+            op = None
+            structured = self.csr_desc is None or dt == 1
+            if structured:
+                op = at.tune_operator(s, dt, cache)
+            (ulevs, _), (plevs, _) = self._level_arrays(dt)
+            u = at.tune_hierarchy(self.levels, ulevs, 3, dt,
+                                  self.level_heuristic(self.levels, ulevs, 3), self.mask0, cache)
+            p = at.tune_hierarchy(self.plevels, plevs, 1, dt,
+                                  self.level_heuristic(self.plevels, plevs, 1), None, cache)
+            at.apply_tuning(self.handle, dt, op=op, u=u, p=p)
+            self.tuning[dt] = {"operator": list(op) if op else None, "u_levels": u,
+                               "p_levels": p}
+        if self.prec32 is not None and 1 in self.tuning and self.tuning[1]["operator"]:
+            # the FP64 residual of the refinement runs the FP64 operator
+            op64 = at.tune_operator(s, 0, cache)
+            at.apply_tuning(self.handle, 0, op=op64)
+            self.tuning["op64"] = list(op64)
+        return self.tuning
 
     # -- setup ----------------------------------------------------------------
     def _build_hierarchy(self):

@@ -94,6 +94,22 @@ class DirichletBC:
             return np.broadcast_to(v, (self.nodes.size, 3)).reshape(-1)
         return np.broadcast_to(v, (self.nodes.size,)).astype(np.float64)
 
+    def values_device(self, t, device):
+        """values() as a device tensor, the nodal evaluation done where the
+        node coordinates are kept (Expression.eval_device)."""
+        if self.dofs.size == 0:
+            return torch.zeros(0, dtype=torch.float64, device=device)
+        if not hasattr(self.value, "eval_device"):
+            return torch.as_tensor(np.ascontiguousarray(self.values(t))).to(device)
+        if getattr(self, "_coords_dev", None) is None:
+            self._coords_dev = torch.as_tensor(np.ascontiguousarray(self.coords)).to(device)
+        v = self.value.eval_device(self._coords_dev, t).to(torch.float64)
+        n = self.nodes.size
+        if self.field == 0:
+            return v.expand(n, 3).reshape(-1) if v.dim() > 1 else \
+                v[:, None].expand(n, 3).reshape(-1)
+        return v.expand(n).contiguous()
+
     def get_boundary_values(self, t=None):
         return dict(zip(self.dofs.tolist(), self.values(t).tolist()))
 
@@ -111,6 +127,9 @@ class _Source:
         c = mesh.cells()
         bary = refel.pk_nodes_bary(self.k)                           # nk,4
         self.X = np.einsum("nk,ckd->cnd", bary, x[c])                # nC,nk,3
+        # the P_k nodes of every cell stay in HBM: time-dependent sources are
+        # evaluated there each step (Expression.eval_device), not on the host
+        self.Xd = torch.as_tensor(self.X.reshape(-1, 3)).to(engine.device)
         self.Q = torch.as_tensor(refel.source_weights(test_degree, self.k)).to(engine.device)
         self.test_degree = test_degree
         self.ncomp = ncomp
@@ -120,10 +139,14 @@ class _Source:
 
     def add(self, b, t):
         if self._cache is None or self.expr.depends_on_time():
-            v = self.expr.eval(self.X.reshape(-1, 3), t)
-            v = np.broadcast_to(np.asarray(v, dtype=np.float64),
-                                (self.X.shape[0] * self.X.shape[1],) + ((self.ncomp,) if self.ncomp > 1 else ()))
-            self._cache = torch.as_tensor(np.array(v, dtype=np.float64)).to(self.e.device)
+            npts = self.Xd.shape[0]
+            if hasattr(self.expr, "eval_device"):
+                v = self.expr.eval_device(self.Xd, t)
+            else:                      # Constant and other host-only coefficients
+                v = torch.as_tensor(np.asarray(self.expr.eval(self.X.reshape(-1, 3), t),
+                                               dtype=np.float64)).to(self.e.device)
+            shape = (npts,) + ((self.ncomp,) if self.ncomp > 1 else ())
+            self._cache = v.to(torch.float64).expand(shape).contiguous()
         self.e.system.cell_source(self.test_degree, self.Q, self._cache, self.ncomp, self.scale,
                                   *self.dof, b)
 
@@ -265,7 +288,10 @@ class StepEngine:
         self.stats = []
 
     def bc_values(self, t=None):
-        return np.concatenate([bc.values(t) for bc in self.bcs]) if self.bcs else np.zeros(0)
+        """Dirichlet values of all BC DOFs (device tensor, in bc_dofs order)."""
+        if not self.bcs:
+            return torch.zeros(0, dtype=torch.float64, device=self.device)
+        return torch.cat([bc.values_device(t, self.device) for bc in self.bcs])
 
     def assemble_rhs(self, up_prev, t_n):
         """b at the reference's time levels (:461-483), moving the problem's
@@ -315,7 +341,7 @@ class StepEngine:
     def step(self, up_prev, x, t_n, testing_guess=False):
         """One implicit step from up_prev (device FP64) into x; returns t_{n+1}."""
         t_np1 = self.assemble_rhs(up_prev, t_n)
-        g = torch.as_tensor(self.bc_values(None)).to(self.device)
+        g = self.bc_values(None)
         if self.extrapolate:
             self._valid_history(float(t_n))
         if not testing_guess:
@@ -400,6 +426,9 @@ class MPETTotalPressureSolver(object):
         params.add("fp64_operator", "csr")
         params.add("check_every", 0)
         params.add("use_graphs", None)
+        # launch shapes from the autotuner (None: for systems of at least
+        # 200,000 unknowns)
+        params.add("autotune", None)
         params.add("device", "cuda")
         return params
 
@@ -442,6 +471,7 @@ class MPETTotalPressureSolver(object):
                             use_csr=(p["fp64_operator"] == "csr") if p["precision"] == "fp64"
                             else None,
                             check_every=int(p["check_every"]), use_graphs=p["use_graphs"],
+                            autotune=p["autotune"],
                             outer_tol=float(p["outer_tol"]), inner_rtol=float(p["inner_rtol"]),
                             maxit=int(p["maxit"]), p0_weight=p["p0_weight"],
                             amg=AmgParams(smoother_degree=int(p["u_smoother_degree"]),
@@ -458,7 +488,7 @@ class MPETTotalPressureSolver(object):
         keys = ("precision", "krylov_rtol", "outer_tol", "inner_rtol", "maxit", "p0_weight",
                 "extrapolate", "extrapolate_order", "u_smoother_degree", "p_smoother_degree",
                 "u_level_degrees", "p_level_degrees", "u_max_coarse", "fp64_operator",
-                "check_every", "use_graphs", "dt", "theta", "device")
+                "check_every", "use_graphs", "autotune", "dt", "theta", "device")
         return (bool(direct),) + tuple(repr(p[k]) for k in keys)
 
     def engine(self, direct=None):
