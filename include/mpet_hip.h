@@ -369,6 +369,10 @@ typedef struct {
 int mpet_amg_setup(int n, const int64_t* rp, const int* ci, const double* v,
                    const MpetAmgParams* prm, int n1, const int64_t* p0_rp,
                    const int* p0_ci, const double* p0_v, void** handle);
+/* OpenMP threads of the setups the calling host thread runs next (no
+ * counterpart in the reference): instances set up concurrently from a
+ * thread pool split the host cores among them. */
+int mpet_amg_set_threads(int n);
 int mpet_amg_num_levels(void* h);
 int mpet_amg_level_sizes(void* h, int l, int64_t* out);
 int mpet_amg_level_copy(void* h, int l, int64_t* A_rp, int* A_ci, double* A_v, double* dinv,

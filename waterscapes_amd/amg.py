@@ -51,7 +51,14 @@ def _bind():
     L.mpet_amg_coarse_inverse.restype = C.c_int
     L.mpet_amg_destroy.argtypes = [P]
     L.mpet_amg_destroy.restype = None
+    L.mpet_amg_set_threads.argtypes = [C.c_int]
+    L.mpet_amg_set_threads.restype = C.c_int
     return L
+
+
+def set_host_threads(n):
+    """OpenMP threads of the hierarchy setups run next by this thread."""
+    _lib.check(_bind().mpet_amg_set_threads(int(n)))
 
 
 def setup_host(rowptr, cols, vals, params: AmgParams = None, P0=None):

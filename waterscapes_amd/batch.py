@@ -29,7 +29,7 @@ import time as _time
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, amg
 from ._lib import check, ptr, MAXF
 from .krylov import (MAXLEV, HierDesc, PrecDesc, SolveStats, SolverConfig, SolverOptions,
                      _pick_lpr, compact_level, set_level_degrees)
@@ -175,6 +175,37 @@ def batch_engine(solver, precision="mixed"):
     return eng
 
 
+def build_engines(solvers, precision="mixed", workers=None):
+    """batch_engine of every solver.  Instances that have no engine yet are
+    set up concurrently by a pool of host threads (their hierarchy setups
+    run in native code without the interpreter lock, with the host cores
+    split among the workers), each worker enqueuing its device work on a
+    stream of its own and synchronising it before the engine is used."""
+    import concurrent.futures as cf
+    import os
+    import threading
+    todo = [s for s in solvers if not (getattr(s, "_engine", None) is not None
+                                       and getattr(s._engine, "batched", False)
+                                       and s._engine.config.mode == precision)]
+    ncpu = len(os.sched_getaffinity(0))
+    workers = workers or max(1, min(8, len(todo), ncpu // 2))
+    if workers > 1 and len(todo) > 1:
+        local = threading.local()
+        dev = torch.device(solvers[0].params["device"])
+
+        def work(s):
+            if not hasattr(local, "stream"):
+                local.stream = torch.cuda.Stream(device=dev)
+                amg.set_host_threads(max(1, ncpu // workers))
+            with torch.cuda.stream(local.stream):
+                e = batch_engine(s, precision)
+            local.stream.synchronize()
+            return e
+        with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+            list(ex.map(work, todo))
+    return [batch_engine(s, precision) for s in solvers]
+
+
 class MPETBatch:
     """Advance many MPETTotalPressureSolver instances together.
 
@@ -204,11 +235,10 @@ class MPETBatch:
         self.K = K = len(self.solvers)
         dev = torch.device(self.solvers[0].params["device"])
         self.device = dev
-        self.engines = []
         for s in self.solvers:
             if str(torch.device(s.params["device"])) != str(dev):
                 raise ValueError("all instances of a batch must be on one device")
-            self.engines.append(batch_engine(s, precision))
+        self.engines = build_engines(self.solvers, precision)
         self.keep = []
         self._layout()
         self._pack_operator()

@@ -403,6 +403,14 @@ int mpet_amg_setup(int n, const int64_t* rp, const int* ci, const double* v,
   return 0;
 }
 
+// OpenMP threads of the setups that the calling host thread runs next
+// (several instances set up at once from a thread pool share the cores).
+int mpet_amg_set_threads(int n) {
+  if (n < 1) return 1;
+  omp_set_num_threads(n);
+  return 0;
+}
+
 int mpet_amg_num_levels(void* h) { return (int)((Hier*)h)->lev.size(); }
 
 // sizes: out[0]=n, out[1]=nnz(A), out[2]=nc (cols of P, 0 on coarsest), out[3]=nnz(P)

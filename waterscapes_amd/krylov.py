@@ -9,6 +9,7 @@ of the pressure blocks, and the upload of every array to HBM.
 from __future__ import annotations
 
 import ctypes as C
+import time
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -227,8 +228,11 @@ class KrylovSolver:
             for f in constraints.fields:
                 if not self.Pd[f] > 0.0:
                     self.Pd[f] = 1.0
+        t0 = time.perf_counter()
         self._build_hierarchy()
+        t1 = time.perf_counter()
         self._build_pressure()
+        self.setup_seconds = {"u_hierarchy": t1 - t0, "p_hierarchy": time.perf_counter() - t1}
         self.handle = None
         self.last_stats = None
         if not create_handle:        # host hierarchies only (packed by a batch)

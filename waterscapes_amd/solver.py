@@ -100,7 +100,7 @@ class DirichletBC:
         if self.dofs.size == 0:
             return torch.zeros(0, dtype=torch.float64, device=device)
         if not hasattr(self.value, "eval_device"):
-            return torch.as_tensor(np.ascontiguousarray(self.values(t))).to(device)
+            return torch.as_tensor(np.array(self.values(t), dtype=np.float64)).to(device)
         if getattr(self, "_coords_dev", None) is None:
             self._coords_dev = torch.as_tensor(np.ascontiguousarray(self.coords)).to(device)
         v = self.value.eval_device(self._coords_dev, t).to(torch.float64)
@@ -220,6 +220,7 @@ class StepEngine:
                 robin_lhs.append((i, fs, problem.beta[i], None))
         # batched: the batch packs the operator and hierarchies of all its
         # instances, so no per-instance structured copies or solver handle
+        _t0 = _time.perf_counter()
         if space.total_pressure:
             self.system = DeviceSystem(space, params, dt, theta, K_cells=cellK, device=device,
                                        build_csr=(not batched and config.mode == "fp64"
@@ -248,9 +249,11 @@ class StepEngine:
                                  "problem's nullspace flags")
             self.constraints = Constraints(self.system, bool(problem.u_has_nullspace),
                                            pressure_fields(problem))
+        _t1 = _time.perf_counter()
         self.krylov = KrylovSolver(self.system, [bc.dofs for bc in self.bcs], config,
                                    create_handle=not batched, constraints=self.constraints)
         self.config = config
+        self.setup_seconds = {"system": _t1 - _t0, "krylov": _time.perf_counter() - _t1}
         # sources
         nu, F = space.nu, space.F
         self.f_src = None
