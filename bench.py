@@ -209,108 +209,6 @@ def time_kernel(fn, reps=20):
     return e0.elapsed_time(e1) / reps
 
 
-def _mat_bytes(nnz, rows, s, compact=False):
-    """Values + column indices + row pointers of one CSR level matrix (the
-    compact FP32 P2 level: half values and 16-bit column offsets)."""
-    return nnz * (4 if compact else s + 4) + (rows + 1) * 4
-
-
-def vcycle_bytes(levels, amgp, NC, s, compact0=False, scaled=False):
-    """Algorithmic HBM bytes of one V-cycle (amg_cycle.h: vcycle / smooth /
-    spmm modes), each array read once per launch: matrices, the gathered
-    vector once, the epilogue vectors; levels = the host hierarchy dicts."""
-    nl = len(levels)
-    tot = 0
-
-    def spmm(mode, nnz, rows, cols, compact=False):
-        mat = _mat_bytes(nnz, rows, s, compact) + NC * cols * s
-        epi = {0: 1, 1: 2, 2: 2, 3: 4, 4: 3}[mode] * NC * rows * s
-        if mode in (3, 4):
-            epi += rows * s                      # D^-1
-        return mat + epi
-
-    def level(l, entered):
-        nonlocal tot
-        lev = levels[l]
-        n = lev["n"]
-        if l == nl - 1:
-            tot += n * n * s + 2 * NC * n * s      # dense coarsest solve
-            return
-        nnz = len(lev["A_ci"])
-        cmp = compact0 and l == 0
-        deg = amgp.degree(l)
-        if not entered:                            # first Chebyshev step from 0
-            tot += (3 + (1 if (scaled and l == 0) else 0)) * NC * n * s + n * s
-        tot += (deg - 1) * spmm(3, nnz, n, n, cmp)
-        tot += spmm(1, nnz, n, n, cmp)             # residual
-        nc = lev["nc"]
-        nnzP = len(lev["P_ci"])
-        tot += spmm(4 if l + 1 < nl - 1 else 0, nnzP, nc, n)   # restriction
-        level(l + 1, l + 1 < nl - 1)
-        tot += spmm(2, nnzP, n, nc)                # prolongation
-        tot += deg * spmm(3, nnz, n, n, cmp)       # post-smoothing from x
-    level(0, False)
-    return tot
-
-
-def iteration_bytes(eng, mixed):
-    """Algorithmic HBM bytes of one MINRES iteration of the engine's solve
-    (krylov.hip minres_iter): K1 (operator apply with the fused MINRES
-    step), the block preconditioner (displacement and pressure V-cycles, the
-    pressure gather / scatter) and the BLAS-1 kernels (k2 3n, dot 2n, k5 6n
-    vector accesses).  Returns (bytes, parts)."""
-    sysm = eng.system
-    kr = eng.krylov
-    s = 4 if mixed else 8
-    n = sysm.n
-    if mixed or kr.csr_desc is None:
-        k1 = op_bytes(sysm, 1 if mixed else 0) + 2 * n * s
-    else:
-        k1 = csr_bytes(sysm) + 2 * n * s
-    vu = vcycle_bytes(kr.levels, kr.cfg.amg, 3, s, compact0=mixed and hasattr(kr, "compact"),
-                      scaled=True)
-    vp = vcycle_bytes(kr.plevels, kr.cfg.pamg, 1, s) + 4 * sysm.F * sysm.nP1 * s
-    blas = 11 * n * s
-    parts = {"k1": int(k1), "vcycle_u": int(vu), "vcycle_p": int(vp), "blas1": int(blas)}
-    return int(k1 + vu + vp + blas), parts
-
-
-def step_bytes(eng, mixed, inner, outer):
-    """Algorithmic bytes of one time step: inner iterations, the FP64
-    residuals and MINRES starts of the solve (two for the norm of the
-    lifted right-hand side and the first residual, one more per refinement
-    step) and the right-hand-side update (P1-pattern kernel)."""
-    sysm = eng.system
-    it_b, _ = iteration_bytes(eng, mixed)
-    s = 4 if mixed else 8
-    n = sysm.n
-    kr = eng.krylov
-    op64 = op_bytes(sysm, 0) if (mixed or kr.csr_desc is None) else csr_bytes(sysm)
-    pc = vcycle_bytes(kr.levels, kr.cfg.amg, 3, s, compact0=mixed and hasattr(kr, "compact"),
-                      scaled=True) + vcycle_bytes(kr.plevels, kr.cfg.pamg, 1, s)
-    starts = 2 + (outer if mixed else 0)
-    solve_overhead = starts * (op64 + n * 8 + pc + 2 * n * s)
-    nz11 = sysm.nnz["11"]
-    rhs = nz11 * ((sysm.A + 1) * 8 + 4) + (sysm.nP1 + 1) * 4 + 2 * n * 8
-    return int(inner * it_b + solve_overhead + rhs)
-
-
-def time_kernel(fn, reps=20):
-    """Average duration (ms) of fn() on the current stream via HIP events."""
-    import torch
-    fn()
-    torch.cuda.synchronize()
-    st = torch.cuda.current_stream()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(st)
-    for _ in range(reps):
-        fn()
-    e1.record(st)
-    e1.synchronize()
-    return e0.elapsed_time(e1) / reps
-
-
 def cpu_baseline(n_sample, A, steps, budget_s=30.0):
     """The FP64 oracle (numpy/scipy restatement, direct LU as in the
     reference's default solve_direct) on the same workload family at a

@@ -340,6 +340,43 @@ int mpet_facet_scatter_mat(int nrows, const int* row_node, const int* inc_rp, co
 int mpet_op_apply(const MpetOpDesc* op, const void* x, const void* b, void* y,
                   hipStream_t stream);
 
+/* Cell-wise (matrix-free) FP32 form of the same operator, the MatMult of
+ * the inner FP32 MINRES (:493): the u rows and the divergence part of the
+ * p0 rows (2 mu eps(u):eps(v) + p0 div v + div(u) w0, :267-269) are
+ * integrated cell by cell from the geometry, exactly (P2 gradients are
+ * linear); the P1 x P1 pressure block (C (x) M1 + stiff St, with the Robin
+ * terms in St) is read from the FP32 structured copy.  Summation order is
+ * fixed (incidence lists sorted by slot): results are reproducible.
+ *   geom       10*nC floats from mpet_cell_geometry
+ *   inc2_rp/inc2_slot  P2 node -> slots e*10 + k of the cells containing it
+ *   inc1_rp/inc1_slot  P1 node -> slots e*4 + v
+ *   lanes_u in {4, 8, 16}, lanes_p in {8, 16} (others: 8)
+ * The reference always multiplies by the assembled PETSc matrix (:437). */
+typedef struct {
+  int32_t nC, nP2, nP1, A;
+  const int32_t* cell_p2;        /* nC*10, as MpetMeshDesc */
+  const int32_t* cell_p1;        /* nC*4 */
+  const float* geom;
+  const int32_t *inc2_rp, *inc2_slot;
+  const int32_t *inc1_rp, *inc1_slot;
+  const int32_t *rp11, *ci11;
+  int32_t nnz11;
+  const float *M1, *St;          /* FP32 structured copy (nnz11, A*nnz11) */
+  MpetCoef coef;
+  const uint8_t* dmask;
+  int32_t lanes_u, lanes_p, xcd_chunk;
+} MpetCellOpDesc;
+
+/* Per-cell geometry of the cell-wise operator, FP32 structure-of-arrays:
+ * geom[d*nC + e] = d lambda_{1 + d/3} / d x_{d%3} (d < 9), geom[9*nC + e]
+ * = |T_e|, computed in FP64 from mesh->coords and rounded. */
+int mpet_cell_geometry(const MpetMeshDesc* mesh, float* geom, hipStream_t stream);
+
+/* y = A x / y = b - A x with Dirichlet rows zeroed, as mpet_op_apply (FP32)
+ * through the cell-wise operator; work: 34*nC floats (per-cell results). */
+int mpet_cell_op_apply(const MpetCellOpDesc* op, const float* x, const float* b, float* y,
+                       float* work, hipStream_t stream);
+
 /* Scalar CSR SpMV (PETSc MatMult on AIJ, :493), int64 rowptr, lpr lanes
  * per row in {4,8,16,32,64}; optional residual b and row mask as above. */
 int mpet_csr_spmv(int dtype, long long nrows, const long long* rp, const int* ci,
@@ -479,6 +516,13 @@ int mpet_level_spmm(int dtype, int nc, int mode, int nrows, const int32_t* rp,
  * dropped and captured again by the next solve.  Results change only by
  * the summation order of the reductions inside a row. */
 int mpet_solver_set_tuning(void* h, int dtype, const MpetTuning* t);
+
+/* Inner FP32 MINRES of a mixed-precision solver handle on the cell-wise
+ * operator (op != NULL; the handle allocates its 34*nC work floats and
+ * keeps a copy of the descriptor, whose arrays must outlive it) or back on
+ * the FP32 structured operator (op == NULL).  Captured graphs are dropped
+ * and captured again by the next solve.  No counterpart in the reference. */
+int mpet_solver_set_cell_operator(void* h, const MpetCellOpDesc* op);
 
 /* z = P^-1 r with the FP64 (dtype 0) or FP32 (1) preconditioner. */
 int mpet_precond_apply(void* h, int dtype, const void* r, void* z, hipStream_t stream);

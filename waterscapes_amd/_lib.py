@@ -51,6 +51,16 @@ class OpDesc(C.Structure):
                 ("xcd_chunk", C.c_int32)]
 
 
+class CellOpDesc(C.Structure):
+    _fields_ = [("nC", C.c_int32), ("nP2", C.c_int32), ("nP1", C.c_int32), ("A", C.c_int32),
+                ("cell_p2", C.c_void_p), ("cell_p1", C.c_void_p), ("geom", C.c_void_p),
+                ("inc2_rp", C.c_void_p), ("inc2_slot", C.c_void_p), ("inc1_rp", C.c_void_p),
+                ("inc1_slot", C.c_void_p), ("rp11", C.c_void_p), ("ci11", C.c_void_p),
+                ("nnz11", C.c_int32), ("M1", C.c_void_p), ("St", C.c_void_p), ("coef", Coef),
+                ("dmask", C.c_void_p), ("lanes_u", C.c_int32), ("lanes_p", C.c_int32),
+                ("xcd_chunk", C.c_int32)]
+
+
 class TwoFieldCoef(C.Structure):
     _fields_ = [("C", C.c_double * (MAXF * MAXF)), ("stiff", C.c_double * MAXF),
                 ("alpha", C.c_double * MAXF), ("mu", C.c_double), ("lmbda", C.c_double)]
@@ -115,6 +125,9 @@ def lib():
         "mpet_cell_source": ([P, P, C.c_int, P, C.c_int, P, C.c_int, C.c_double,
                               C.c_longlong, C.c_int, C.c_int, P, P], C.c_int),
         "mpet_op_apply": ([P, P, P, P, P], C.c_int),
+        "mpet_cell_geometry": ([P, P, P], C.c_int),
+        "mpet_cell_op_apply": ([P, P, P, P, P, P], C.c_int),
+        "mpet_solver_set_cell_operator": ([P, P], C.c_int),
         "mpet_csr_spmv": ([C.c_int, C.c_longlong, P, P, P, P, P, P, P, C.c_int, P], C.c_int),
         "mpet_soa_convert": ([C.c_int, P, C.c_int, C.c_longlong, P, P], C.c_int),
     }

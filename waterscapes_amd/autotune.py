@@ -7,6 +7,9 @@ one matrix row:
   * the structured operator (fused MINRES step and mpet_op_apply): lanes
     per P2 node row (u rows) and per P1 node row (p rows) in {8, 16, 32},
     and the run length of row tiles dealt to one XCD (xcd_chunk, 0 or 4);
+  * the cell-wise FP32 operator (mpet_cell_op_apply): lanes of its row
+    kernel per P2 node row in {4, 8, 16} and per P1 node row in {8, 16},
+    and its XCD run length (0 or 4);
   * every AMG level of both hierarchies: lanes per row of the level
     operator (timed as the Chebyshev step), of the prolongator and of the
     restriction (timed as plain products), powers of two up to 64.
@@ -37,6 +40,8 @@ from ._lib import check, ptr
 
 ARCH = "gfx950"
 LANES_OP = (8, 16, 32)
+LANES_CELL_U = (4, 8, 16)
+LANES_CELL_P = (8, 16)
 XCD_CHUNKS = (0, 4)
 LPR_ALL = (1, 2, 4, 8, 16, 32, 64)
 
@@ -164,6 +169,37 @@ def tune_operator(sys_, dtype, cache=None, reps=10, repeats=3, margin=0.02):
         return time_launches(lambda: check(L.mpet_op_apply(C.byref(d), ptr(x), None, ptr(y), s)),
                              reps, repeats)
     cands = [(u, p, xc) for u in LANES_OP for p in LANES_OP for xc in XCD_CHUNKS]
+    best, _ = _pick(cands, default, timer, margin, cache)
+    cache.put(key, list(best))
+    return best
+
+
+def cell_key(sys_):
+    nz = sys_.nnz
+    return (f"cell|{ARCH}|F{sys_.F}|{sys_.cell_op()['nC']}|{sys_.nP2}|{sys_.nP1}|"
+            f"{nz['11']}")
+
+
+def tune_cell_operator(sys_, cache=None, reps=10, repeats=3, margin=0.02):
+    """(lanes_u, lanes_p, xcd_chunk) of the cell-wise operator's row kernel."""
+    cache = cache if cache is not None else default_cache()
+    key = cell_key(sys_)
+    hit = cache.get(key)
+    if hit is not None:
+        return tuple(hit)
+    L = _lib.lib()
+    g = torch.Generator(device=sys_.device).manual_seed(0)
+    x = torch.randn(sys_.n, generator=g, device=sys_.device, dtype=torch.float64).float()
+    y = torch.empty_like(x)
+    work = torch.empty(34 * sys_.cell_op()["nC"], dtype=torch.float32, device=sys_.device)
+    s = _lib.stream_ptr()
+    default = tuple(sys_.cell_lanes) + (sys_.cell_xcd_chunk,)
+
+    def timer(c):
+        d = sys_.cell_op_desc(lanes=(c[0], c[1]), xcd_chunk=c[2])
+        return time_launches(lambda: check(L.mpet_cell_op_apply(C.byref(d), ptr(x), None, ptr(y),
+                                                                ptr(work), s)), reps, repeats)
+    cands = [(u, p, xc) for u in LANES_CELL_U for p in LANES_CELL_P for xc in XCD_CHUNKS]
     best, _ = _pick(cands, default, timer, margin, cache)
     cache.put(key, list(best))
     return best

@@ -1,0 +1,398 @@
+// Cell-wise (matrix-free) FP32 application of the total-pressure operator.
+//
+// The u rows and the divergence part of the p0 rows, i.e. the bilinear form
+//     2 mu eps(u):eps(v) + p0 div v + div(u) w0
+// (mpet/mpettotalpressuresolver.py:267-269), are computed cell by cell from
+// the geometry instead of being read from the assembled Kuu / Dv / Dt
+// values, which are 9 + 3 + 3 FP32 numbers per node pair (1.1 GB per apply
+// at 48^3 x 6 tets).  The P1 x P1 pressure block C (x) M1 + stiff St stays
+// assembled (small, and it carries the per-cell permeabilities and the Robin
+// terms).
+//
+// Exact integration without quadrature: the gradient of a P2 field is
+// linear on a tetrahedron, so with g_n = grad u at vertex n and the linear
+// stress s = mu (g + g^T) + p0 I (exact: p0 is P1),
+//     int_T s : grad phi_i = |T|/20 sum_n (S + s_n) : (grad phi_i)(vertex n),
+// S = sum_n s_n, from int_T lambda_m lambda_n = |T| (1 + delta_mn) / 20.
+// With G_j = grad lambda_j and the P2 basis gradients (4 lambda_j - 1) G_j
+// (vertex j) and 4 (lambda_k G_j + lambda_j G_k) (edge jk):
+//     g_n          = 4 u_n G_n - sum_j u_j G_j + 4 sum_{k != n} u_{nk} G_k
+//     y(vertex j)  = |T|/20 (4 s_j - S) G_j
+//     y(edge jk)   = |T|/5 ((S + s_k) G_j + (S + s_j) G_k)
+//     y_p0(vertex v) = |T|/20 (sum_n tr g_n + tr g_v)
+// (u_j, g_n, s_n in 3-component / 3x3 form; y(.) are the 3 rows of a P2
+// node's u DOFs.)
+//
+// Two launches: cw_cell_kernel writes each cell's 30 + 4 local results
+// (Yu[cell][10][3], Yp[cell][4]); cw_rows_kernel gives every row its sum
+// over the incident cells (incidence lists sorted by cell, so the summation
+// order is fixed and results are reproducible) plus the assembled P1 x P1
+// part, and hands the row value to an epilogue (plain apply / masked
+// residual, or the fused MINRES step).
+#pragma once
+#include "amg_cycle.h"
+
+namespace mpet {
+
+// Device view of MpetCellOpDesc plus the two work arrays.
+struct CellOp {
+  int nC, nP2, nP1, A;
+  const int* cell_p2;     // nC x 10 (UFC order: vertices, then edges)
+  const int* cell_p1;     // nC x 4 (P1 node of each vertex)
+  const float* geom;      // SoA: [d * nC + e] = G_{1 + d/3}[d % 3] (d < 9); |T| at d = 9
+  const int *inc2_rp, *inc2_slot;   // P2 node -> slots e * 10 + k (ascending)
+  const int *inc1_rp, *inc1_slot;   // P1 node -> slots e * 4 + v (ascending)
+  const int *rp11, *ci11;
+  int nnz11;
+  const float *M1, *St;             // St: A x nnz11, network-major
+  float C[MPET_MAXF * MPET_MAXF];
+  float stiff[MPET_MAXF];
+  float mu;
+  const uint8_t* dmask;
+  int lanes_u, lanes_p, xcd_chunk;
+  float* Yu;              // nC x 30
+  float* Yp;              // nC x 4
+};
+
+constexpr int CW_NT = 256;   // cells per block of the cell kernel
+
+// Index of the UFC edge joining local vertices j != k.
+__device__ __forceinline__ int cw_edge(int j, int k) {
+  const int lo = j < k ? j : k, hi = j < k ? k : j;
+  // (2,3)->0 (1,3)->1 (1,2)->2 (0,3)->3 (0,2)->4 (0,1)->5
+  return lo == 0 ? 6 - hi : (lo == 1 ? 4 - hi : 0);
+}
+
+// Phase 1: one thread per cell.  Node ids are staged through LDS so that
+// the index and result arrays move in contiguous block-wide segments.
+// (A template, so that the translation units including this header share
+// one instance.)
+template <int NTB>
+__global__ void __launch_bounds__(NTB)
+cw_cell_kernel(CellOp op, const float* __restrict__ x, const int* __restrict__ skip) {
+  static_assert(NTB == CW_NT, "one cell per thread of a CW_NT block");
+  if (skip && *skip) return;
+  __shared__ float sbuf[CW_NT * 34];
+  int* sidx = reinterpret_cast<int*>(sbuf);
+  const long long e0 = (long long)blockIdx.x * CW_NT;
+  const int ncell = (int)(op.nC - e0 < CW_NT ? op.nC - e0 : CW_NT);
+  const int t = threadIdx.x;
+  for (int i = t; i < ncell * 10; i += CW_NT) sidx[i] = op.cell_p2[10 * e0 + i];
+  for (int i = t; i < ncell * 4; i += CW_NT) sidx[CW_NT * 10 + i] = op.cell_p1[4 * e0 + i];
+  __syncthreads();
+  int n2[10], n1[4];
+  const bool live = t < ncell;
+  if (live) {
+#pragma unroll
+    for (int k = 0; k < 10; ++k) n2[k] = sidx[t * 10 + k];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) n1[v] = sidx[CW_NT * 10 + t * 4 + v];
+  }
+  __syncthreads();                 // sbuf is reused for the results below
+  if (live) {
+    const long long e = e0 + t;
+    const long long nC = op.nC;
+    float G[4][3];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) G[1 + d / 3][d % 3] = op.geom[d * nC + e];
+    const float vol = op.geom[9 * nC + e];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) G[0][m] = -(G[1][m] + G[2][m] + G[3][m]);
+    const int F = op.A + 1;
+    const long long nu = 3ll * op.nP2;
+    float u[10][3], p0[4];
+#pragma unroll
+    for (int k = 0; k < 10; ++k)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) u[k][b] = x[3ll * n2[k] + b];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) p0[v] = x[nu + (long long)F * n1[v]];
+    // U = sum_j u_j (x) G_j
+    float U[3][3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        U[b][d] = u[0][b] * G[0][d] + u[1][b] * G[1][d] + u[2][b] * G[2][d] + u[3][b] * G[3][d];
+    // vertex stresses (symmetric: xx yy zz xy xz yz) and divergences
+    float s[4][6], tr[4];
+    const float mu = op.mu;
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      float g[3][3];
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) g[b][d] = 4.0f * u[n][b] * G[n][d] - U[b][d];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (k == n) continue;
+        const int m = 4 + cw_edge(n, k);
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) g[b][d] += 4.0f * u[m][b] * G[k][d];
+      }
+      tr[n] = g[0][0] + g[1][1] + g[2][2];
+      s[n][0] = 2.0f * mu * g[0][0] + p0[n];
+      s[n][1] = 2.0f * mu * g[1][1] + p0[n];
+      s[n][2] = 2.0f * mu * g[2][2] + p0[n];
+      s[n][3] = mu * (g[0][1] + g[1][0]);
+      s[n][4] = mu * (g[0][2] + g[2][0]);
+      s[n][5] = mu * (g[1][2] + g[2][1]);
+    }
+    float S[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) S[q] = (s[0][q] + s[1][q]) + (s[2][q] + s[3][q]);
+    const float c = vol * (1.0f / 20.0f);
+    float* yu = sbuf + t * 30;
+    // symmetric 3x3 (xx yy zz xy xz yz) times a vector
+    auto symv = [](const float* M, const float* w, float* out) {
+      out[0] = M[0] * w[0] + M[3] * w[1] + M[4] * w[2];
+      out[1] = M[3] * w[0] + M[1] * w[1] + M[5] * w[2];
+      out[2] = M[4] * w[0] + M[5] * w[1] + M[2] * w[2];
+    };
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      float M[6], o[3];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) M[q] = c * (4.0f * s[j][q] - S[q]);
+      symv(M, G[j], o);
+      yu[3 * j + 0] = o[0];
+      yu[3 * j + 1] = o[1];
+      yu[3 * j + 2] = o[2];
+    }
+    constexpr int EA[6] = {2, 1, 1, 0, 0, 0}, EB[6] = {3, 3, 2, 3, 2, 1};
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      const int j = EA[m], k = EB[m];
+      float Mk[6], Mj[6], o1[3], o2[3];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        Mk[q] = 4.0f * c * (S[q] + s[k][q]);
+        Mj[q] = 4.0f * c * (S[q] + s[j][q]);
+      }
+      symv(Mk, G[j], o1);
+      symv(Mj, G[k], o2);
+      yu[3 * (4 + m) + 0] = o1[0] + o2[0];
+      yu[3 * (4 + m) + 1] = o1[1] + o2[1];
+      yu[3 * (4 + m) + 2] = o1[2] + o2[2];
+    }
+    const float D = (tr[0] + tr[1]) + (tr[2] + tr[3]);
+    float* yp = sbuf + CW_NT * 30 + t * 4;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) yp[v] = c * (D + tr[v]);
+  }
+  __syncthreads();
+  for (int i = t; i < ncell * 30; i += CW_NT) op.Yu[30 * e0 + i] = sbuf[i];
+  for (int i = t; i < ncell * 4; i += CW_NT) op.Yp[4 * e0 + i] = sbuf[CW_NT * 30 + i];
+}
+
+// Phase-2 epilogue of a plain application: y = A x, or y = b - A x with b;
+// rows with dmask set give 0.
+struct CwEpiApply {
+  float* y;
+  const float* b;
+  const uint8_t* mask;
+  __device__ __forceinline__ bool skipped() const { return false; }
+  __device__ __forceinline__ void prep() {}
+  __device__ __forceinline__ double operator()(long long i, float ax) const {
+    float v = b ? b[i] - ax : ax;
+    if (mask && mask[i]) v = 0.0f;
+    y[i] = v;
+    return 0.0;
+  }
+};
+
+// Phase 2: GU lanes per P2 node row, GP lanes per P1 node row, grid-stride
+// over row tiles (u tiles first), XCD-aware tile order.  The epilogue's
+// per-row results are summed per block into partial[blockIdx.x] (when given).
+template <int F, int GU, int GP, typename Epi>
+__global__ void __launch_bounds__(256)
+cw_rows_kernel(CellOp op, const float* __restrict__ x, Epi epi, double* __restrict__ partial) {
+  if (epi.skipped()) return;
+  epi.prep();
+  constexpr int RU = 256 / GU, RP = 256 / GP;
+  const long long nu = 3ll * op.nP2;
+  const int ntu = (op.nP2 + RU - 1) / RU, ntp = (op.nP1 + RP - 1) / RP;
+  const float* __restrict__ Yu = op.Yu;
+  const float* __restrict__ Yp = op.Yp;
+  double acc = 0.0;
+  for (int tt = blockIdx.x; tt < ntu + ntp; tt += gridDim.x) {
+    const int tile = xcd_tile(tt, ntu + ntp, op.xcd_chunk);
+    if (tile < ntu) {
+      const int lane = threadIdx.x % GU;
+      const int a = tile * RU + threadIdx.x / GU;
+      if (a < op.nP2) {
+        const int kb = op.inc2_rp[a], ke = op.inc2_rp[a + 1];
+        float y0 = 0.f, y1 = 0.f, y2 = 0.f;
+        // two incident cells per lane per trip, loads issued before use
+        for (int k = kb + lane; k < ke; k += 2 * GU) {
+          const int k2 = k + GU;
+          const int sa = op.inc2_slot[k];
+          const int sb = op.inc2_slot[k2 < ke ? k2 : k];
+          const float a0 = Yu[3ll * sa], a1 = Yu[3ll * sa + 1], a2 = Yu[3ll * sa + 2];
+          float b0 = Yu[3ll * sb], b1 = Yu[3ll * sb + 1], b2 = Yu[3ll * sb + 2];
+          if (!(k2 < ke)) b0 = b1 = b2 = 0.f;
+          y0 += a0;
+          y1 += a1;
+          y2 += a2;
+          y0 += b0;
+          y1 += b1;
+          y2 += b2;
+        }
+#pragma unroll
+        for (int off = GU / 2; off > 0; off >>= 1) {
+          y0 += __shfl_xor(y0, off, 64);
+          y1 += __shfl_xor(y1, off, 64);
+          y2 += __shfl_xor(y2, off, 64);
+        }
+        if (lane < 3) {
+          const float yc = lane == 0 ? y0 : (lane == 1 ? y1 : y2);
+          acc += epi(3ll * a + lane, yc);
+        }
+      }
+    } else {
+      const int lane = threadIdx.x % GP;
+      const int c = (tile - ntu) * RP + threadIdx.x / GP;
+      if (c < op.nP1) {
+        float yd = 0.f;
+        const int kb = op.inc1_rp[c], ke = op.inc1_rp[c + 1];
+        for (int k = kb + lane; k < ke; k += GP) yd += Yp[op.inc1_slot[k]];
+        float m[F], sv[F];
+#pragma unroll
+        for (int g = 0; g < F; ++g) { m[g] = 0.f; sv[g] = 0.f; }
+        const int qb = op.rp11[c], qe = op.rp11[c + 1];
+        const long long n11 = op.nnz11;
+        for (int k = qb + lane; k < qe; k += GP) {
+          const float* xp = x + nu + (long long)F * op.ci11[k];
+          const float mm = op.M1[k];
+          float xv[F], st[F];
+#pragma unroll
+          for (int g = 0; g < F; ++g) {
+            xv[g] = xp[g];
+            st[g] = g >= 1 ? op.St[(g - 1) * n11 + k] : 0.f;
+          }
+#pragma unroll
+          for (int g = 0; g < F; ++g) {
+            m[g] += mm * xv[g];
+            if (g >= 1) sv[g] += st[g] * xv[g];
+          }
+        }
+#pragma unroll
+        for (int off = GP / 2; off > 0; off >>= 1) yd += __shfl_xor(yd, off, 64);
+        float mine = 0.f;
+#pragma unroll
+        for (int g = 0; g < F; ++g) {
+          float mg = m[g], sg = sv[g];
+#pragma unroll
+          for (int off = GP / 2; off > 0; off >>= 1) {
+            mg += __shfl_xor(mg, off, 64);
+            sg += __shfl_xor(sg, off, 64);
+          }
+          m[g] = mg;
+          if (lane == g) mine = (g == 0) ? yd : op.stiff[g] * sg;
+        }
+        if (lane < F) {
+#pragma unroll
+          for (int g = 0; g < F; ++g) mine += op.C[lane * MPET_MAXF + g] * m[g];
+          acc += epi(nu + (long long)F * c + lane, mine);
+        }
+      }
+    }
+  }
+  if (partial) {
+    double bs = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+  }
+}
+
+// Launch of phase 2 for F fields and the lane counts of the operator (GU in
+// {4, 8, 16}, GP in {8, 16}; others fall back to 8).  Returns the grid
+// (number of partials written when partial is given).
+template <typename Epi, int F, int GU>
+static void cw_rows_gp(const CellOp& op, int gp, dim3 grid, hipStream_t s, const float* x,
+                       Epi epi, double* partial) {
+  if (gp == 16)
+    hipLaunchKernelGGL((cw_rows_kernel<F, GU, 16, Epi>), grid, dim3(256), 0, s, op, x, epi,
+                       partial);
+  else
+    hipLaunchKernelGGL((cw_rows_kernel<F, GU, 8, Epi>), grid, dim3(256), 0, s, op, x, epi,
+                       partial);
+}
+template <typename Epi, int F>
+static void cw_rows_f(const CellOp& op, dim3 grid, hipStream_t s, const float* x, Epi epi,
+                      double* partial) {
+  switch (op.lanes_u) {
+    case 4: cw_rows_gp<Epi, F, 4>(op, op.lanes_p, grid, s, x, epi, partial); break;
+    case 16: cw_rows_gp<Epi, F, 16>(op, op.lanes_p, grid, s, x, epi, partial); break;
+    default: cw_rows_gp<Epi, F, 8>(op, op.lanes_p, grid, s, x, epi, partial); break;
+  }
+}
+static inline int cw_lanes_u(int g) { return (g == 4 || g == 8 || g == 16) ? g : 8; }
+static inline int cw_lanes_p(int g) { return (g == 8 || g == 16) ? g : 8; }
+static inline int cw_row_tiles(const CellOp& op) {
+  const int gu = cw_lanes_u(op.lanes_u), gp = cw_lanes_p(op.lanes_p);
+  return (op.nP2 + 256 / gu - 1) / (256 / gu) + (op.nP1 + 256 / gp - 1) / (256 / gp);
+}
+// Both phases: cells (x -> Yu, Yp), then rows through epi.  Returns the
+// number of blocks of phase 2 (one partial each when partial is given).
+template <typename Epi>
+static int cw_launch(const CellOp& op, const float* x, Epi epi, double* partial,
+                     const int* skip, hipStream_t s) {
+  const int cb = (op.nC + CW_NT - 1) / CW_NT;
+  hipLaunchKernelGGL(cw_cell_kernel<CW_NT>, dim3(cb), dim3(CW_NT), 0, s, op, x, skip);
+  const int np = grid_for(cw_row_tiles(op), 1, MAX_PARTIALS);
+  const dim3 grid(np);
+  switch (op.A + 1) {
+    case 2: cw_rows_f<Epi, 2>(op, grid, s, x, epi, partial); break;
+    case 3: cw_rows_f<Epi, 3>(op, grid, s, x, epi, partial); break;
+    case 4: cw_rows_f<Epi, 4>(op, grid, s, x, epi, partial); break;
+    case 5: cw_rows_f<Epi, 5>(op, grid, s, x, epi, partial); break;
+    case 6: cw_rows_f<Epi, 6>(op, grid, s, x, epi, partial); break;
+    case 7: cw_rows_f<Epi, 7>(op, grid, s, x, epi, partial); break;
+    case 8: cw_rows_f<Epi, 8>(op, grid, s, x, epi, partial); break;
+    default: break;
+  }
+  return np;
+}
+
+// Host: device view of a descriptor (coefficients rounded to FP32) with
+// the work arrays Yu = work, Yp = work + 30 nC; checks the descriptor.
+static int cw_from_desc(const MpetCellOpDesc* d, float* work, CellOp& op) {
+  if (!d || d->nC <= 0 || d->nP2 <= 0 || d->nP1 <= 0 || d->A < 1 || d->A + 1 > MPET_MAXF ||
+      d->nnz11 <= 0 || !d->cell_p2 || !d->cell_p1 || !d->geom || !d->inc2_rp ||
+      !d->inc2_slot || !d->inc1_rp || !d->inc1_slot || !d->rp11 || !d->ci11 || !d->M1 ||
+      !d->St || !work) {
+    mpet_set_error("invalid cell-wise operator description");
+    return ERR_INVALID;
+  }
+  op.nC = d->nC;
+  op.nP2 = d->nP2;
+  op.nP1 = d->nP1;
+  op.A = d->A;
+  op.cell_p2 = d->cell_p2;
+  op.cell_p1 = d->cell_p1;
+  op.geom = d->geom;
+  op.inc2_rp = d->inc2_rp;
+  op.inc2_slot = d->inc2_slot;
+  op.inc1_rp = d->inc1_rp;
+  op.inc1_slot = d->inc1_slot;
+  op.rp11 = d->rp11;
+  op.ci11 = d->ci11;
+  op.nnz11 = d->nnz11;
+  op.M1 = d->M1;
+  op.St = d->St;
+  for (int i = 0; i < MPET_MAXF * MPET_MAXF; ++i) op.C[i] = (float)d->coef.C[i];
+  for (int i = 0; i < MPET_MAXF; ++i) op.stiff[i] = (float)d->coef.stiff[i];
+  op.mu = (float)d->coef.mu;
+  op.dmask = d->dmask;
+  op.lanes_u = cw_lanes_u(d->lanes_u);
+  op.lanes_p = cw_lanes_p(d->lanes_p);
+  op.xcd_chunk = d->xcd_chunk;
+  op.Yu = work;
+  op.Yp = work + 30ll * d->nC;
+  return OK;
+}
+
+}  // namespace mpet

@@ -27,6 +27,7 @@
 // Dirichlet invariant: Krylov vectors have zero Dirichlet components and
 // operators zero their Dirichlet rows (see spmv.hip).
 #include "amg_cycle.h"
+#include "cellwise.h"
 #include "op_rows.h"
 #include <vector>
 #include <math.h>
@@ -69,6 +70,19 @@ __device__ __forceinline__ void k1_scalars(const MinresState* st, T& s, T& c1) {
   s = (T)(1.0 / st->beta);
   c1 = st->it > 0 ? (T)(st->beta / st->oldb) : (T)0;
 }
+
+// K1 epilogue for the cell-wise operator's row kernel (cellwise.h).
+struct CwEpiK1 {
+  EpiK1<float> e;
+  const MinresState* st;
+  const int* skip;
+  float s = 0.f, c1 = 0.f;
+  __device__ __forceinline__ bool skipped() const { return *skip != 0; }
+  __device__ __forceinline__ void prep() { k1_scalars(st, s, c1); }
+  __device__ __forceinline__ double operator()(long long i, float ax) const {
+    return e(i, ax, s, c1);
+  }
+};
 
 // K1 on the structured operator, grid-stride over row tiles: GU lanes per
 // P2 node row (tiles of NT / GU rows), GP lanes per P1 node row (tiles of
@@ -489,6 +503,11 @@ struct MpetSolver {
   MpetOpT<double> op64;
   MpetOpT<float> op32;
   bool has_op32 = false;
+  // optional cell-wise FP32 operator of the inner MINRES (cellwise.h)
+  bool has_cw = false;
+  CellOp cw;
+  float* cw_work = nullptr;
+  long long cw_work_n = 0;
   // optional scalar CSR (plain FP64 path)
   long long csr_rows = 0;
   const long long* csr_rp = nullptr;
@@ -666,7 +685,15 @@ static void minres_iter(MpetSolver* S, Minres<T>& M, const Precond<T>& P, bool c
 #undef K1CSR
     }
   }
-  if (!csr) {
+  bool cw_done = false;
+  if constexpr (sizeof(T) == 4) {
+    if (!csr && S->has_cw) {
+      CwEpiK1 cepi{EpiK1<float>{M.y, M.v, M.z, Ra, S->cw.dmask}, M.st, skip};
+      np = cw_launch(S->cw, M.z, cepi, M.partial, skip, s);
+      cw_done = true;
+    }
+  }
+  if (!csr && !cw_done) {
     const MpetOpT<T>* op;
     if constexpr (sizeof(T) == 8) op = &S->op64; else op = &S->op32;
     epi.mask = op->dmask;
@@ -1188,6 +1215,42 @@ extern "C" int mpet_solver_set_tuning(void* h, int dtype, const MpetTuning* t) {
   if (S->graph32) HIPCHK(hipGraphExecDestroy(S->graph32));
   if (S->graph64) HIPCHK(hipGraphExecDestroy(S->graph64));
   S->graph32 = S->graph64 = nullptr;
+  return OK;
+}
+
+// Inner FP32 MINRES on the cell-wise operator (op != NULL) or back on the
+// structured FP32 operator (op == NULL); the captured graphs are dropped.
+extern "C" int mpet_solver_set_cell_operator(void* h, const MpetCellOpDesc* op) {
+  MpetSolver* S = (MpetSolver*)h;
+  if (!S || !S->has_op32 || S->csr_only || S->mb) {
+    mpet_set_error("mpet_solver_set_cell_operator: needs a mixed-precision solver on the "
+                   "structured operator");
+    return ERR_INVALID;
+  }
+  HIPCHK(hipDeviceSynchronize());
+  if (S->graph32) HIPCHK(hipGraphExecDestroy(S->graph32));
+  if (S->graph64) HIPCHK(hipGraphExecDestroy(S->graph64));
+  S->graph32 = S->graph64 = nullptr;
+  if (!op) {
+    S->has_cw = false;
+    return OK;
+  }
+  if (op->nP2 != S->op32.nP2 || op->nP1 != S->op32.nP1 || op->A != S->op32.A ||
+      op->nnz11 != S->op32.nnz11) {
+    mpet_set_error("mpet_solver_set_cell_operator: sizes do not match the solver's operator");
+    return ERR_INVALID;
+  }
+  const long long need = 34ll * (op->nC > 0 ? op->nC : 1);
+  if (S->cw_work_n < need) {
+    int rc = alloc_dev(S->allocs, &S->cw_work, (size_t)need);
+    if (rc) return rc;
+    S->cw_work_n = need;
+  }
+  CellOp c;
+  int rc = cw_from_desc(op, S->cw_work, c);
+  if (rc) return rc;
+  S->cw = c;
+  S->has_cw = true;
   return OK;
 }
 

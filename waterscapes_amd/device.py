@@ -156,6 +156,59 @@ class DeviceSystem:
                            ptr(vals["Dv"]), ptr(vals["Dt"]), ptr(vals["M1"]), ptr(vals["St"]),
                            coef, ptr(self.dmask) if masked else None, lu, lp, xc)
 
+    # lanes per (P2, P1) node row and XCD run length of the cell-wise
+    # operator's row kernel
+    cell_lanes = (8, 8)
+    cell_xcd_chunk = 4
+
+    def cell_op(self):
+        """Arrays of the cell-wise FP32 operator, built once: per-cell
+        geometry (FP32 gradients of the barycentric coordinates and |T|) and
+        the node -> cell-slot incidence lists (slots e*10 + k of P2 nodes,
+        e*4 + v of P1 nodes, ascending within a node)."""
+        if getattr(self, "_cw", None) is None:
+            dev = self.device
+            nC = int(self.cells.shape[0])
+            geom = torch.empty(10 * nC, dtype=torch.float32, device=dev)
+            check(_lib.lib().mpet_cell_geometry(C.byref(self.mesh_desc), ptr(geom),
+                                                _lib.stream_ptr()))
+
+            def incidence(table, nnode):
+                nodes = table.reshape(-1).to(torch.int64)
+                slot = torch.argsort(nodes, stable=True).to(torch.int32)
+                rp = torch.zeros(nnode + 1, dtype=torch.int64, device=dev)
+                rp[1:] = torch.cumsum(torch.bincount(nodes, minlength=nnode), 0)
+                return rp.to(torch.int32), slot
+            self._cw = {"nC": nC, "geom": geom, "inc2": incidence(self.cell_p2, self.nP2),
+                        "inc1": incidence(self.cells, self.nP1)}
+        return self._cw
+
+    def cell_op_desc(self, masked=True, lanes=None, xcd_chunk=None):
+        """MpetCellOpDesc of the cell-wise FP32 operator (the P1 x P1 block
+        from the FP32 structured copy)."""
+        if self.op32 is None:
+            raise ValueError("the cell-wise operator needs the FP32 structured copy")
+        cw = self.cell_op()
+        v = self.op32
+        lu, lp = lanes or self.cell_lanes
+        xc = self.cell_xcd_chunk if xcd_chunk is None else xcd_chunk
+        p11 = self.pat_dev["11"]
+        return _lib.CellOpDesc(cw["nC"], self.nP2, self.nP1, self.A, ptr(self.cell_p2),
+                               ptr(self.cells), ptr(cw["geom"]), ptr(cw["inc2"][0]),
+                               ptr(cw["inc2"][1]), ptr(cw["inc1"][0]), ptr(cw["inc1"][1]),
+                               ptr(p11[0]), ptr(p11[1]), self.nnz["11"], ptr(v["M1"]),
+                               ptr(v["St"]), _lib.make_coef(self.Cm, self.Cs, self.mu),
+                               ptr(self.dmask) if masked else None, lu, lp, xc)
+
+    def cell_apply(self, x32, b=None, masked=True, lanes=None):
+        """y = A x (or b - A x) in FP32 through the cell-wise operator."""
+        y = torch.empty_like(x32)
+        work = torch.empty(34 * self.cell_op()["nC"], dtype=torch.float32, device=self.device)
+        d = self.cell_op_desc(masked, lanes)
+        check(_lib.lib().mpet_cell_op_apply(C.byref(d), ptr(x32), ptr(b), ptr(y), ptr(work),
+                                            _lib.stream_ptr()))
+        return y
+
     def row_lengths(self, which):
         rp = {k: np.diff(v[0]) for k, v in self.pat_host.items()}
         F = self.F

@@ -80,6 +80,8 @@ def _bind():
     L.mpet_solver_destroy.restype = C.c_int
     L.mpet_solver_solve.argtypes = [P, P, P, P, P]
     L.mpet_solver_solve.restype = C.c_int
+    L.mpet_solver_set_cell_operator.argtypes = [P, P]
+    L.mpet_solver_set_cell_operator.restype = C.c_int
     L.mpet_precond_apply.argtypes = [P, C.c_int, P, P, P]
     L.mpet_precond_apply.restype = C.c_int
     return L
@@ -123,6 +125,9 @@ class SolverConfig:
     # launches are latency-bound and the fixed heuristic is kept)
     autotune: bool = None
     autotune_min_dofs: int = 200000
+    # inner FP32 MINRES of the mixed mode on the cell-wise (matrix-free)
+    # operator instead of the FP32 copy of the assembled one
+    cell_operator: bool = True
 
 
 def set_level_degrees(h, params: AmgParams):
@@ -293,6 +298,10 @@ class KrylovSolver:
                                    C.byref(self.opt), C.byref(h)))
         self.handle = h
         self.last_stats = None
+        self.cell_desc = None
+        if mixed and cfg.cell_operator:
+            self.cell_desc = system.cell_op_desc()
+            check(L.mpet_solver_set_cell_operator(h, C.byref(self.cell_desc)))
         self.tuning = {}
         tune = cfg.autotune if cfg.autotune is not None else system.n >= cfg.autotune_min_dofs
         if tune:
@@ -324,7 +333,9 @@ class KrylovSolver:
             dtypes.append(0)
         for dt in dtypes:
             op = None
-            structured = self.csr_desc is None or dt == 1
+            # the FP32 structured operator runs only when the cell-wise one is off
+            structured = (self.csr_desc is None or dt == 1) and not (
+                dt == 1 and self.cell_desc is not None)
             if structured:
                 op = at.tune_operator(s, dt, cache)
             (ulevs, _), (plevs, _) = self._level_arrays(dt)
@@ -335,7 +346,12 @@ class KrylovSolver:
             at.apply_tuning(self.handle, dt, op=op, u=u, p=p)
             self.tuning[dt] = {"operator": list(op) if op else None, "u_levels": u,
                                "p_levels": p}
-        if self.prec32 is not None and 1 in self.tuning and self.tuning[1]["operator"]:
+        if self.cell_desc is not None:
+            cell = at.tune_cell_operator(s, cache)
+            self.cell_desc = s.cell_op_desc(lanes=(cell[0], cell[1]), xcd_chunk=cell[2])
+            check(_bind().mpet_solver_set_cell_operator(self.handle, C.byref(self.cell_desc)))
+            self.tuning["cell"] = list(cell)
+        if self.prec32 is not None and 1 in self.tuning and self.opt.mode in (1, 2):
             # the FP64 residual of the refinement runs the FP64 operator
             op64 = at.tune_operator(s, 0, cache)
             at.apply_tuning(self.handle, 0, op=op64)

@@ -427,6 +427,10 @@ class MPETTotalPressureSolver(object):
         # check_every iterations (0: 1 for FP64, 10 for mixed) and chunks
         # replayed from a captured hipGraph (None: mixed only)
         params.add("fp64_operator", "csr")
+        # operator of the inner FP32 MINRES of the mixed path: "cellwise" =
+        # integrated cell by cell from the geometry (matrix-free u block),
+        # "structured" = the FP32 copy of the assembled node-pattern operator
+        params.add("fp32_operator", "cellwise")
         params.add("check_every", 0)
         params.add("use_graphs", None)
         # launch shapes from the autotuner (None: for systems of at least
@@ -470,11 +474,14 @@ class MPETTotalPressureSolver(object):
                                 p0_weight=p["p0_weight"])
         if p["fp64_operator"] not in ("csr", "structured"):
             raise ValueError("fp64_operator must be 'csr' or 'structured'")
+        if p["fp32_operator"] not in ("cellwise", "structured"):
+            raise ValueError("fp32_operator must be 'cellwise' or 'structured'")
         return SolverConfig(mode=p["precision"], rtol=float(p["krylov_rtol"]),
                             use_csr=(p["fp64_operator"] == "csr") if p["precision"] == "fp64"
                             else None,
                             check_every=int(p["check_every"]), use_graphs=p["use_graphs"],
                             autotune=p["autotune"],
+                            cell_operator=(p["fp32_operator"] == "cellwise"),
                             outer_tol=float(p["outer_tol"]), inner_rtol=float(p["inner_rtol"]),
                             maxit=int(p["maxit"]), p0_weight=p["p0_weight"],
                             amg=AmgParams(smoother_degree=int(p["u_smoother_degree"]),
@@ -491,6 +498,7 @@ class MPETTotalPressureSolver(object):
         keys = ("precision", "krylov_rtol", "outer_tol", "inner_rtol", "maxit", "p0_weight",
                 "extrapolate", "extrapolate_order", "u_smoother_degree", "p_smoother_degree",
                 "u_level_degrees", "p_level_degrees", "u_max_coarse", "fp64_operator",
+                "fp32_operator",
                 "check_every", "use_graphs", "autotune", "dt", "theta", "device")
         return (bool(direct),) + tuple(repr(p[k]) for k in keys)
 
