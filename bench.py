@@ -101,6 +101,22 @@ def op_bytes(sys_, dtype):
     return vals + idx + vec
 
 
+def cell_op_bytes(sys_):
+    """Algorithmic HBM bytes of one cell-wise FP32 operator apply
+    (csrc/cellwise.h), each array once: the cell kernel reads the four cell
+    tables (14 node and 14 position ints per cell), the geometry (10 floats)
+    and x, and writes the 34 per-cell results; the row kernel reads the
+    node offsets, the 34 per-cell results, the P1 x P1 block (indices, M1,
+    A stiffness arrays), the pressure part of x, and writes y."""
+    nC = sys_.cell_op()["nC"]
+    n, F, A = sys_.n, sys_.F, sys_.A
+    nz11 = sys_.nnz["11"]
+    cells = nC * (28 * 4 + 10 * 4) + n * 4 + nC * 34 * 4
+    rows = ((sys_.nP2 + 1) + 2 * (sys_.nP1 + 1)) * 4 + nC * 34 * 4 + nz11 * (4 + 4 + 4 * A) \
+        + F * sys_.nP1 * 4 + n * 4
+    return cells + rows
+
+
 def csr_bytes(sys_):
     rp, ci, v = sys_.csr
     nnz = v.numel()
@@ -161,7 +177,9 @@ def iteration_bytes(eng, mixed):
     kr = eng.krylov
     s = 4 if mixed else 8
     n = sysm.n
-    if mixed or kr.csr_desc is None:
+    if mixed and getattr(kr, "cell_desc", None) is not None:
+        k1 = cell_op_bytes(sysm) + 2 * n * s
+    elif mixed or kr.csr_desc is None:
         k1 = op_bytes(sysm, 1 if mixed else 0) + 2 * n * s
     else:
         k1 = csr_bytes(sysm) + 2 * n * s
@@ -832,10 +850,12 @@ def main_indicators(args):
 
 
 def main_probe(args):
-    """Short run for hardware-counter passes: 5 launches of the FP32
-    structured operator apply of the 48^3 A = 4 workload and 5 launches of a
-    plain device copy kernel of 2^27 doubles (copy_convert_kernel<double>:
-    1 GiB read, 1 GiB written) that calibrates the counters."""
+    """Short run for hardware-counter passes: 5 launches each of the FP32
+    structured operator apply and of the cell-wise FP32 apply (cell kernel +
+    row kernel, the solver's tuned lanes) of the 48^3 A = 4 workload, and 5
+    launches of a plain device copy kernel of 2^27 doubles
+    (copy_convert_kernel<double>: 1 GiB read, 1 GiB written) that calibrates
+    the counters."""
     import torch
     from waterscapes_amd import _lib
     s = make_solver(args.n, args.A, "mixed")
@@ -850,11 +870,20 @@ def main_probe(args):
     torch.cuda.synchronize()
     for _ in range(5):
         _lib.check(L.mpet_op_apply(C.byref(d32), _lib.ptr(x), None, _lib.ptr(y), sp))
+    cd = s.engine().krylov.cell_desc
+    work = torch.empty(34 * sysm.cell_op()["nC"], dtype=torch.float32, device="cuda")
+    for _ in range(5):
+        _lib.check(L.mpet_cell_op_apply(C.byref(cd), _lib.ptr(x), None, _lib.ptr(y),
+                                        _lib.ptr(work), sp))
     for _ in range(5):
         _lib.check(L.mpet_soa_convert(0, _lib.ptr(a), 1, a.numel(), _lib.ptr(b), sp))
     torch.cuda.synchronize()
-    print(json.dumps({"probe": "op_apply_fp32 x5, copy_convert<double> 1 GiB x5",
-                      "op_bytes": int(op_bytes(sysm, 1)), "copy_read_bytes": a.numel() * 8,
+    print(json.dumps({"probe": "op_apply_fp32 x5, cell_op_apply x5, copy_convert<double> "
+                               "1 GiB x5",
+                      "op_bytes": int(op_bytes(sysm, 1)),
+                      "cell_op_bytes": int(cell_op_bytes(sysm)),
+                      "cell_lanes": [cd.lanes_u, cd.lanes_p, cd.xcd_chunk],
+                      "copy_read_bytes": a.numel() * 8,
                       "copy_write_bytes": a.numel() * 8}))
 
 
@@ -974,15 +1003,29 @@ def main():
             if best_res is not None else []
         del ref, ref_res
         gc.collect()
-    # live roofline of the dominant operator kernel (FP32 structured apply)
+    # live roofline of the dominant operator: the cell-wise FP32 apply of
+    # the inner MINRES (cell kernel + row kernel, at the solver's tuned
+    # lanes), and the FP32 structured apply it replaced
     sysm = eng.system
     x32 = (mix_res[2][-1] if mix_res[2] else mixed.up.data).float()
     y32 = torch.empty_like(x32)
-    d32 = sysm.op_desc(1)
     L = _lib.lib()
     sp = _lib.stream_ptr()
-    ms = time_kernel(lambda: L.mpet_op_apply(C.byref(d32), _lib.ptr(x32), None, _lib.ptr(y32), sp))
-    abytes = op_bytes(sysm, 1)
+    cdesc = eng.krylov.cell_desc
+    if cdesc is not None:
+        work = torch.empty(34 * sysm.cell_op()["nC"], dtype=torch.float32, device=x32.device)
+        ms = time_kernel(lambda: L.mpet_cell_op_apply(C.byref(cdesc), _lib.ptr(x32), None,
+                                                      _lib.ptr(y32), _lib.ptr(work), sp))
+        abytes = cell_op_bytes(sysm)
+        rkernel = (f"cw_cell_kernel + cw_rows_kernel<{sysm.F},{cdesc.lanes_u},{cdesc.lanes_p}> "
+                   f"(cell-wise A, FP32)")
+    d32 = sysm.op_desc(1)
+    ms_s = time_kernel(lambda: L.mpet_op_apply(C.byref(d32), _lib.ptr(x32), None, _lib.ptr(y32),
+                                               sp))
+    bytes_s = op_bytes(sysm, 1)
+    if cdesc is None:
+        ms, abytes = ms_s, bytes_s
+        rkernel = f"op_apply_kernel<float,{sysm.F},16,16> (structured A, FP32)"
     achieved = abytes / (ms * 1e-3) / 1e9
     mix_sps = K / mix_res[0]
     lat = np.array(mix_res[1]) * 1e3
@@ -1057,16 +1100,27 @@ def main():
                                                        _lib.ptr(v), _lib.ptr(x64), None, None,
                                                        _lib.ptr(y64), lpr, sp))
             out["fp64"]["csr_spmv"] = {"ms": round(ms64, 4), "GB_s": round(csr_bytes(fsys) / (ms64 * 1e-3) / 1e9, 1)}
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "op_apply_48cube_A4_traffic.json")
-    if args.n == 48 and args.A == 4 and os.path.exists(tfile):
+    def traffic_of(name):
         # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter passes
-        # (calibrated on a copy kernel of known size), recorded there
-        traffic = int(json.load(open(tfile))["traffic_bytes_per_launch"])
-    out["roofline"] = {"bound": "hbm", "kernel": "op_apply_kernel<float,5,16,16> (structured A, FP32)",
+        # (calibrated on a copy kernel of known size), recorded in profiles/
+        tfile = os.path.join(ROOT, "profiles", name)
+        if args.n == 48 and args.A == 4 and os.path.exists(tfile):
+            return int(json.load(open(tfile))["traffic_bytes_per_launch"])
+        return None
+    out["roofline"] = {"bound": "hbm", "kernel": rkernel,
                        "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-                       "frac": round(achieved / 8000.0, 4), "traffic": traffic,
+                       "frac": round(achieved / 8000.0, 4),
+                       "traffic": traffic_of("cell_op_48cube_A4_traffic.json" if cdesc is not None
+                                             else "op_apply_48cube_A4_traffic.json"),
                        "bytes_per_launch": int(abytes), "ms_per_launch": round(ms, 4)}
+    if cdesc is not None:
+        ach_s = bytes_s / (ms_s * 1e-3) / 1e9
+        out["roofline_structured_operator"] = {
+            "kernel": f"op_apply_kernel<float,{sysm.F},16,16> (assembled A, FP32; replaced in "
+                      f"the inner MINRES by the cell-wise apply)",
+            "achieved": round(ach_s, 1), "frac": round(ach_s / 8000.0, 4),
+            "traffic": traffic_of("op_apply_48cube_A4_traffic.json"),
+            "bytes_per_launch": int(bytes_s), "ms_per_launch": round(ms_s, 4)}
     if not args.no_cpu_baseline and fp64_res is not None:
         # the plain FP64 path on the host at the full workload, a bounded
         # sample of MINRES iterations scaled by the iterations per step

@@ -346,19 +346,21 @@ int mpet_op_apply(const MpetOpDesc* op, const void* x, const void* b, void* y,
  * integrated cell by cell from the geometry, exactly (P2 gradients are
  * linear); the P1 x P1 pressure block (C (x) M1 + stiff St, with the Robin
  * terms in St) is read from the FP32 structured copy.  Summation order is
- * fixed (incidence lists sorted by slot): results are reproducible.
- *   geom       10*nC floats from mpet_cell_geometry
- *   inc2_rp/inc2_slot  P2 node -> slots e*10 + k of the cells containing it
- *   inc1_rp/inc1_slot  P1 node -> slots e*4 + v
+ * fixed (each node's cell contributions are summed in cell order):
+ * results are reproducible.  Cell tables structure-of-arrays, [k*nC + e]:
+ *   node2 / node1  P2 node of local node k (UFC order) / P1 node of vertex k
+ *   pos2 / pos1    entry of (cell e, local k) in the list of all (cell, k)
+ *                  sorted by node then cell (the entries of a node are
+ *                  contiguous, from inc2_rp[node] / inc1_rp[node])
+ *   geom           10*nC floats from mpet_cell_geometry
  *   lanes_u in {4, 8, 16}, lanes_p in {8, 16} (others: 8)
  * The reference always multiplies by the assembled PETSc matrix (:437). */
 typedef struct {
   int32_t nC, nP2, nP1, A;
-  const int32_t* cell_p2;        /* nC*10, as MpetMeshDesc */
-  const int32_t* cell_p1;        /* nC*4 */
+  const int32_t *node2, *node1;  /* 10*nC, 4*nC */
+  const int32_t *pos2, *pos1;    /* 10*nC, 4*nC */
   const float* geom;
-  const int32_t *inc2_rp, *inc2_slot;
-  const int32_t *inc1_rp, *inc1_slot;
+  const int32_t *inc2_rp, *inc1_rp;   /* nP2 + 1, nP1 + 1 */
   const int32_t *rp11, *ci11;
   int32_t nnz11;
   const float *M1, *St;          /* FP32 structured copy (nnz11, A*nnz11) */

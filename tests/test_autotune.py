@@ -18,9 +18,11 @@ def test_pick_keeps_incumbent_within_margin():
 
 
 def test_lane_candidates():
-    assert at._lpr_cands(3, 8) == (4, 8, 16, 32, 64)
+    assert at._lpr_cands(3, 8) == (1, 4, 8, 16, 32, 64)
     assert at._lpr_cands(1, 4) == (1, 2, 4, 8, 16, 32)
-    assert all(c >= 4 for c in at._lpr_cands(3, 4))
+    # three columns: one thread per row or at least one lane per component
+    assert all(c == 1 or c >= 4 for c in at._lpr_cands(3, 4))
+    assert at._lpr_cands(3, 64) == (8, 16, 32, 64)
 
 
 def test_cache_round_trip(tmp_path):

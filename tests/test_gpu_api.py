@@ -103,3 +103,28 @@ def test_api_mms_matches_oracle(precision, direct, extra):
     # split() gives (u, p0, p1, p2)
     parts = solver.up.split()
     assert len(parts) == params["J"] + 2
+
+
+def test_direct_after_iterative_on_one_solver():
+    """solve_direct() after solve_iterative() on the same solver runs the
+    direct path's FP64 solve (the engine is rebuilt for it, and the initial
+    guess history starts afresh): each step matches the oracle's LU to 1e-8
+    per field; the iterative run before it used the mixed engine."""
+    params = CFG1
+    problem, solver, sym = build_problem(4, params, "mixed", False, outer_tol=1e-11)
+    up0 = solver.up_.vector().get_local().copy()
+    ref = oracle_steps(solver, params, sym, 0.05, 1.0, 0.1)
+    it_states = [up.vector().get_local().copy() for up, t in solver.solve_iterative()]
+    assert solver.engine(direct=False).krylov.cfg.mode == "mixed"
+    # rewind the problem to the initial state and run the direct path
+    problem.time.assign(0.0)
+    solver.up_.vector()[:] = up0
+    space = solver.up_.function_space()
+    k = 0
+    for up, t in solver.solve_direct():
+        errs = field_errors(space, up.vector().get_local(), ref[k][0])
+        assert max(errs) <= 1e-8, (k, errs)
+        k += 1
+    assert k == len(ref) == len(it_states) == 2
+    eng = solver.engine(direct=True)
+    assert eng.krylov.cfg.mode == "fp64" and eng.krylov.cfg.rtol == 1e-12

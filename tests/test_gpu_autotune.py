@@ -32,7 +32,12 @@ def test_solver_picks_are_valid_reused_and_accurate(tmp_path, monkeypatch):
     tuned = _solver(True)
     tun = tuned.engine().krylov.tuning
     assert 1 in tun and cache.timed > 0
-    op = tun[1]["operator"]
+    # the inner FP32 MINRES runs the cell-wise operator (its row kernel's
+    # lanes are tuned), the FP64 residual the structured FP64 operator
+    assert tun[1]["operator"] is None
+    cell = tun["cell"]
+    assert cell[0] in at.LANES_CELL_U and cell[1] in at.LANES_CELL_P and cell[2] in at.XCD_CHUNKS
+    op = tun["op64"]
     assert op[0] in at.LANES_OP and op[1] in at.LANES_OP and op[2] in at.XCD_CHUNKS
     for rows in (tun[1]["u_levels"], tun[1]["p_levels"]):
         for r in rows:

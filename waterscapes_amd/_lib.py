@@ -53,9 +53,9 @@ class OpDesc(C.Structure):
 
 class CellOpDesc(C.Structure):
     _fields_ = [("nC", C.c_int32), ("nP2", C.c_int32), ("nP1", C.c_int32), ("A", C.c_int32),
-                ("cell_p2", C.c_void_p), ("cell_p1", C.c_void_p), ("geom", C.c_void_p),
-                ("inc2_rp", C.c_void_p), ("inc2_slot", C.c_void_p), ("inc1_rp", C.c_void_p),
-                ("inc1_slot", C.c_void_p), ("rp11", C.c_void_p), ("ci11", C.c_void_p),
+                ("node2", C.c_void_p), ("node1", C.c_void_p), ("pos2", C.c_void_p),
+                ("pos1", C.c_void_p), ("geom", C.c_void_p), ("inc2_rp", C.c_void_p),
+                ("inc1_rp", C.c_void_p), ("rp11", C.c_void_p), ("ci11", C.c_void_p),
                 ("nnz11", C.c_int32), ("M1", C.c_void_p), ("St", C.c_void_p), ("coef", Coef),
                 ("dmask", C.c_void_p), ("lanes_u", C.c_int32), ("lanes_p", C.c_int32),
                 ("xcd_chunk", C.c_int32)]

@@ -109,8 +109,18 @@ def default_cache():
 
 
 def time_launches(fn, reps=10, repeats=3):
-    """Mean duration (ms) of fn() over `reps` launches, best of `repeats`."""
+    """Mean duration (ms) of fn() over `reps` launches, best of `repeats`.
+    The launches are captured once in a graph and the graph is replayed (as
+    the solver replays its iterations), so that host launch overhead does
+    not swamp the short kernels of the coarse levels; fn must launch on
+    torch's current stream, looked up at each call."""
     fn()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for _ in range(reps):
+            fn()
+    g.replay()
     torch.cuda.synchronize()
     st = torch.cuda.current_stream()
     best = float("inf")
@@ -118,11 +128,11 @@ def time_launches(fn, reps=10, repeats=3):
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record(st)
-        for _ in range(reps):
-            fn()
+        g.replay()
         e1.record(st)
         e1.synchronize()
         best = min(best, e0.elapsed_time(e1) / reps)
+    del g
     return best
 
 
@@ -161,13 +171,12 @@ def tune_operator(sys_, dtype, cache=None, reps=10, repeats=3, margin=0.02):
     g = torch.Generator(device=sys_.device).manual_seed(0)
     x = torch.randn(sys_.n, generator=g, device=sys_.device, dtype=torch.float64).to(tdt)
     y = torch.empty_like(x)
-    s = _lib.stream_ptr()
     default = (16, 16, sys_.xcd_chunk[dtype])
 
     def timer(c):
         d = sys_.op_desc(dtype, lanes=(c[0], c[1]), xcd_chunk=c[2])
-        return time_launches(lambda: check(L.mpet_op_apply(C.byref(d), ptr(x), None, ptr(y), s)),
-                             reps, repeats)
+        return time_launches(lambda: check(L.mpet_op_apply(C.byref(d), ptr(x), None, ptr(y),
+                                                           _lib.stream_ptr())), reps, repeats)
     cands = [(u, p, xc) for u in LANES_OP for p in LANES_OP for xc in XCD_CHUNKS]
     best, _ = _pick(cands, default, timer, margin, cache)
     cache.put(key, list(best))
@@ -192,13 +201,13 @@ def tune_cell_operator(sys_, cache=None, reps=10, repeats=3, margin=0.02):
     x = torch.randn(sys_.n, generator=g, device=sys_.device, dtype=torch.float64).float()
     y = torch.empty_like(x)
     work = torch.empty(34 * sys_.cell_op()["nC"], dtype=torch.float32, device=sys_.device)
-    s = _lib.stream_ptr()
     default = tuple(sys_.cell_lanes) + (sys_.cell_xcd_chunk,)
 
     def timer(c):
         d = sys_.cell_op_desc(lanes=(c[0], c[1]), xcd_chunk=c[2])
         return time_launches(lambda: check(L.mpet_cell_op_apply(C.byref(d), ptr(x), None, ptr(y),
-                                                                ptr(work), s)), reps, repeats)
+                                                                ptr(work), _lib.stream_ptr())),
+                             reps, repeats)
     cands = [(u, p, xc) for u in LANES_CELL_U for p in LANES_CELL_P for xc in XCD_CHUNKS]
     best, _ = _pick(cands, default, timer, margin, cache)
     cache.put(key, list(best))
@@ -209,8 +218,10 @@ def tune_cell_operator(sys_, cache=None, reps=10, repeats=3, margin=0.02):
 # AMG levels
 # ---------------------------------------------------------------------------
 def _lpr_cands(nc, heuristic):
-    lo = 4 if nc == 3 else 1
-    return tuple(v for v in LPR_ALL if v >= lo and heuristic / 8 <= v <= heuristic * 8)
+    # three-column SpMMs: one thread per row (1) or at least one lane per
+    # component (4 and up)
+    ok = (lambda v: v == 1 or v >= 4) if nc == 3 else (lambda v: True)
+    return tuple(v for v in LPR_ALL if ok(v) and heuristic / 8 <= v <= heuristic * 8)
 
 
 def level_key(dtype, nc, which, rows, cols, nnz, compact):
@@ -223,7 +234,6 @@ def tune_hierarchy(levels_host, levels_dev, nc, dtype, heuristic, mask0=None, ca
     which) gives the fixed pick used as the incumbent."""
     cache = cache if cache is not None else default_cache()
     L = _bind()
-    s = _lib.stream_ptr()
     tdt = torch.float64 if dtype == 0 else torch.float32
     out = []
     for l, (lev, ld) in enumerate(zip(levels_host, levels_dev)):
@@ -263,7 +273,7 @@ def tune_hierarchy(levels_host, levels_dev, nc, dtype, heuristic, mask0=None, ca
                     ptr(dinv) if dinv is not None else None, ptr(d) if d is not None else None,
                     0.5, 0.5, ptr(m0) if m0 is not None else None, lpr,
                     ptr(v16) if v16 is not None else None, ptr(dci) if dci is not None else None,
-                    vs, s)), reps, repeats)
+                    vs, _lib.stream_ptr())), reps, repeats)
             dflt = int(heuristic(l, which))
             cands = _lpr_cands(nc, dflt)
             if dflt not in cands:

@@ -149,9 +149,7 @@ spmm_kernel(int nrows, const int* __restrict__ rp, const int* __restrict__ ci,
     }
     T xv[U][NC];
 #pragma unroll
-    for (int j = 0; j < U; ++j)
-#pragma unroll
-      for (int c = 0; c < NC; ++c) xv[j][c] = x[cc[j] + c];
+    for (int j = 0; j < U; ++j) load_nc<T, NC>(x + cc[j], xv[j]);
 #pragma unroll
     for (int j = 0; j < U; ++j)
 #pragma unroll
@@ -187,6 +185,68 @@ spmm_kernel(int nrows, const int* __restrict__ rp, const int* __restrict__ ci,
   }
 }
 
+// The same SpMM modes with one thread per row doing all NC columns (rows of
+// a few nonzeros, such as the prolongator from the P1 level: no lanes idle
+// on short rows).  Nonzeros are summed in row order.
+template <typename T, int NC, int MODE>
+__global__ void __launch_bounds__(NT)
+spmm_row_kernel(int nrows, const int* __restrict__ rp, const int* __restrict__ ci,
+                const T* __restrict__ val, const T* __restrict__ x, const T* __restrict__ b,
+                T* __restrict__ y, const T* __restrict__ dinv, T* __restrict__ d, T cd, T cr,
+                const uint8_t* __restrict__ mask, Skip sk, const int* __restrict__ grp,
+                T* __restrict__ y2) {
+  if (sk.all && *sk.all) return;
+  const int r = blockIdx.x * NT + threadIdx.x;
+  if (r >= nrows || row_skipped(sk, grp, r)) return;
+  T acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = 0;
+  const int kb = rp[r], ke = rp[r + 1];
+  int k = kb;
+  for (; k + 1 < ke; k += 2) {
+    const T va = val[k], vb = val[k + 1];
+    const long long ca = (long long)NC * ci[k], cb = (long long)NC * ci[k + 1];
+    T xa[NC], xb[NC];
+    load_nc<T, NC>(x + ca, xa);
+    load_nc<T, NC>(x + cb, xb);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      acc[c] += va * xa[c];
+      acc[c] += vb * xb[c];
+    }
+  }
+  if (k < ke) {
+    const T va = val[k];
+    T xa[NC];
+    load_nc<T, NC>(x + (long long)NC * ci[k], xa);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] += va * xa[c];
+  }
+  const bool m = mask ? mask[r] != 0 : false;
+  const T di = (MODE == 3 || MODE == 4) ? dinv[r] : (T)0;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const long long o = (long long)NC * r + c;
+    const T mine = acc[c];
+    if (MODE == 0) {
+      y[o] = m ? (T)0 : mine;
+    } else if (MODE == 1) {
+      y[o] = m ? (T)0 : b[o] - mine;
+    } else if (MODE == 2) {
+      if (!m) y[o] = y[o] + mine;
+    } else if (MODE == 4) {
+      y[o] = mine;
+      const T v = cr * di * mine;
+      d[o] = v;
+      y2[o] = v;
+    } else {
+      const T dd = m ? (T)0 : cd * d[o] + cr * di * (b[o] - mine);
+      d[o] = dd;
+      y[o] = m ? (T)0 : x[o] + dd;
+    }
+  }
+}
+
 // First Chebyshev step from x0 = 0: d = y = cr * Dinv * (scale * b); the
 // scaled right-hand side is kept in bcopy when given.
 template <typename T, int NC>
@@ -206,7 +266,8 @@ cheb0_kernel(int nrows, const T* __restrict__ b, T scale, const T* __restrict__ 
   y[i] = v;
 }
 
-// Dense coarsest solve x = Cinv b (NC columns), one wave per row.
+// Dense coarsest solve x = Cinv b (NC columns), one wave per row; each lane
+// keeps four row entries (and their b values) in flight per trip.
 template <typename T, int NC>
 __global__ void __launch_bounds__(NT)
 coarse_kernel(int n, const T* __restrict__ cinv, const T* __restrict__ b, T* __restrict__ x,
@@ -218,10 +279,20 @@ coarse_kernel(int n, const T* __restrict__ cinv, const T* __restrict__ b, T* __r
   T acc[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = 0;
-  for (int k = lane; k < n; k += 64) {
-    const T a = cinv[(long long)r * n + k];
+  const T* __restrict__ row = cinv + (long long)r * n;
+  for (int k0 = lane; k0 < n; k0 += 256) {
+    T a[4], bv[4][NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) acc[c] += a * b[NC * k + c];
+    for (int j = 0; j < 4; ++j) {
+      const int k = k0 + 64 * j;
+      const int kc = k < n ? k : n - 1;
+      a[j] = k < n ? row[kc] : (T)0;
+      load_nc<T, NC>(b + NC * kc, bv[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[c] += a[j] * bv[j][c];
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
@@ -301,7 +372,6 @@ p_scatter_map_kernel(long long np, const int* __restrict__ map, const T* __restr
   long long i = (long long)blockIdx.x * NT + threadIdx.x;
   if (i < np) dst[i] = src[map[i]];
 }
-
 
 }  // namespace mpet
 
@@ -410,6 +480,15 @@ static void spmm(const LevelDev<T>& L, int which, const T* x, const T* b, T* y, 
   const int* ci = which == 0 ? L.ci : (which == 1 ? L.Pci : L.Rci);
   const T* v = which == 0 ? L.v : (which == 1 ? L.Pv : L.Rv);
   int lpr = which == 0 ? L.lpr : (which == 1 ? L.lprP : L.lprR);
+  const bool compact = which == 0 && L.v16;
+  if (NC > 1 && lpr == 1 && !compact) {     // one thread per row, all columns
+    const int blocks1 = (nrows + NT - 1) / NT;
+    if (blocks1 == 0) return;
+    hipLaunchKernelGGL((spmm_row_kernel<T, NC, MODE>), dim3(blocks1), dim3(NT), 0, s, nrows,
+                       rp, ci, v, x, b, y, dinv ? dinv : L.dinv, d, cd, cr,
+                       which == 2 ? nullptr : L.mask, sk, which == 2 ? L.grpc : L.grp, y2);
+    return;
+  }
   if (lpr < NC) lpr = NC == 3 ? 4 : NC;
   const uint8_t* mask = which == 2 ? nullptr : L.mask;
   const int* grp = which == 2 ? L.grpc : L.grp;

@@ -23,12 +23,14 @@
 // (u_j, g_n, s_n in 3-component / 3x3 form; y(.) are the 3 rows of a P2
 // node's u DOFs.)
 //
-// Two launches: cw_cell_kernel writes each cell's 30 + 4 local results
-// (Yu[cell][10][3], Yp[cell][4]); cw_rows_kernel gives every row its sum
-// over the incident cells (incidence lists sorted by cell, so the summation
-// order is fixed and results are reproducible) plus the assembled P1 x P1
-// part, and hands the row value to an epilogue (plain apply / masked
-// residual, or the fused MINRES step).
+// Two launches: cw_cell_kernel writes each cell's 30 + 4 local results to
+// their places in node-sorted entry lists (the entries of one node
+// contiguous, ordered by cell: Yu[3 pos2(e, k) + b], Yp[pos1(e, v)]);
+// cw_rows_kernel gives every row the sum of its contiguous run (fixed order,
+// so results are reproducible) plus the assembled P1 x P1 part, and hands
+// the row value to an epilogue (plain apply / masked residual, or the fused
+// MINRES step).  Cell tables are structure-of-arrays ([k * nC + e]) so that
+// every load of the cell kernel is one contiguous segment per wave.
 #pragma once
 #include "amg_cycle.h"
 
@@ -37,11 +39,12 @@ namespace mpet {
 // Device view of MpetCellOpDesc plus the two work arrays.
 struct CellOp {
   int nC, nP2, nP1, A;
-  const int* cell_p2;     // nC x 10 (UFC order: vertices, then edges)
-  const int* cell_p1;     // nC x 4 (P1 node of each vertex)
+  const int* node2;       // [k * nC + e]: P2 node of local node k (UFC order)
+  const int* node1;       // [v * nC + e]: P1 node of vertex v
+  const int* pos2;        // [k * nC + e]: entry of (e, k) in the P2-node-sorted list
+  const int* pos1;        // [v * nC + e]: entry of (e, v) in the P1-node-sorted list
   const float* geom;      // SoA: [d * nC + e] = G_{1 + d/3}[d % 3] (d < 9); |T| at d = 9
-  const int *inc2_rp, *inc2_slot;   // P2 node -> slots e * 10 + k (ascending)
-  const int *inc1_rp, *inc1_slot;   // P1 node -> slots e * 4 + v (ascending)
+  const int *inc2_rp, *inc1_rp;     // node -> first entry (nP2 + 1, nP1 + 1)
   const int *rp11, *ci11;
   int nnz11;
   const float *M1, *St;             // St: A x nnz11, network-major
@@ -50,8 +53,8 @@ struct CellOp {
   float mu;
   const uint8_t* dmask;
   int lanes_u, lanes_p, xcd_chunk;
-  float* Yu;              // nC x 30
-  float* Yp;              // nC x 4
+  float* Yu;              // 3 x 10 nC, node-sorted
+  float* Yp;              // 4 nC, node-sorted
 };
 
 constexpr int CW_NT = 256;   // cells per block of the cell kernel
@@ -63,8 +66,7 @@ __device__ __forceinline__ int cw_edge(int j, int k) {
   return lo == 0 ? 6 - hi : (lo == 1 ? 4 - hi : 0);
 }
 
-// Phase 1: one thread per cell.  Node ids are staged through LDS so that
-// the index and result arrays move in contiguous block-wide segments.
+// Phase 1: one thread per cell.
 // (A template, so that the translation units including this header share
 // one instance.)
 template <int NTB>
@@ -72,26 +74,14 @@ __global__ void __launch_bounds__(NTB)
 cw_cell_kernel(CellOp op, const float* __restrict__ x, const int* __restrict__ skip) {
   static_assert(NTB == CW_NT, "one cell per thread of a CW_NT block");
   if (skip && *skip) return;
-  __shared__ float sbuf[CW_NT * 34];
-  int* sidx = reinterpret_cast<int*>(sbuf);
-  const long long e0 = (long long)blockIdx.x * CW_NT;
-  const int ncell = (int)(op.nC - e0 < CW_NT ? op.nC - e0 : CW_NT);
-  const int t = threadIdx.x;
-  for (int i = t; i < ncell * 10; i += CW_NT) sidx[i] = op.cell_p2[10 * e0 + i];
-  for (int i = t; i < ncell * 4; i += CW_NT) sidx[CW_NT * 10 + i] = op.cell_p1[4 * e0 + i];
-  __syncthreads();
-  int n2[10], n1[4];
-  const bool live = t < ncell;
-  if (live) {
-#pragma unroll
-    for (int k = 0; k < 10; ++k) n2[k] = sidx[t * 10 + k];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) n1[v] = sidx[CW_NT * 10 + t * 4 + v];
-  }
-  __syncthreads();                 // sbuf is reused for the results below
-  if (live) {
-    const long long e = e0 + t;
+  const long long e = (long long)blockIdx.x * CW_NT + threadIdx.x;
+  if (e < op.nC) {
     const long long nC = op.nC;
+    int n2[10], n1[4];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) n2[k] = op.node2[k * nC + e];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) n1[v] = op.node1[v * nC + e];
     float G[4][3];
 #pragma unroll
     for (int d = 0; d < 9; ++d) G[1 + d / 3][d % 3] = op.geom[d * nC + e];
@@ -102,9 +92,7 @@ cw_cell_kernel(CellOp op, const float* __restrict__ x, const int* __restrict__ s
     const long long nu = 3ll * op.nP2;
     float u[10][3], p0[4];
 #pragma unroll
-    for (int k = 0; k < 10; ++k)
-#pragma unroll
-      for (int b = 0; b < 3; ++b) u[k][b] = x[3ll * n2[k] + b];
+    for (int k = 0; k < 10; ++k) load_nc<float, 3>(x + 3ll * n2[k], u[k]);
 #pragma unroll
     for (int v = 0; v < 4; ++v) p0[v] = x[nu + (long long)F * n1[v]];
     // U = sum_j u_j (x) G_j
@@ -145,7 +133,10 @@ cw_cell_kernel(CellOp op, const float* __restrict__ x, const int* __restrict__ s
 #pragma unroll
     for (int q = 0; q < 6; ++q) S[q] = (s[0][q] + s[1][q]) + (s[2][q] + s[3][q]);
     const float c = vol * (1.0f / 20.0f);
-    float* yu = sbuf + t * 30;
+    float* __restrict__ Yu = op.Yu;
+    auto put = [&](int k, const float* o) {   // one 12-byte store
+      __builtin_memcpy(Yu + 3ll * op.pos2[k * nC + e], o, 12);
+    };
     // symmetric 3x3 (xx yy zz xy xz yz) times a vector
     auto symv = [](const float* M, const float* w, float* out) {
       out[0] = M[0] * w[0] + M[3] * w[1] + M[4] * w[2];
@@ -158,9 +149,7 @@ cw_cell_kernel(CellOp op, const float* __restrict__ x, const int* __restrict__ s
 #pragma unroll
       for (int q = 0; q < 6; ++q) M[q] = c * (4.0f * s[j][q] - S[q]);
       symv(M, G[j], o);
-      yu[3 * j + 0] = o[0];
-      yu[3 * j + 1] = o[1];
-      yu[3 * j + 2] = o[2];
+      put(j, o);
     }
     constexpr int EA[6] = {2, 1, 1, 0, 0, 0}, EB[6] = {3, 3, 2, 3, 2, 1};
 #pragma unroll
@@ -174,18 +163,13 @@ cw_cell_kernel(CellOp op, const float* __restrict__ x, const int* __restrict__ s
       }
       symv(Mk, G[j], o1);
       symv(Mj, G[k], o2);
-      yu[3 * (4 + m) + 0] = o1[0] + o2[0];
-      yu[3 * (4 + m) + 1] = o1[1] + o2[1];
-      yu[3 * (4 + m) + 2] = o1[2] + o2[2];
+      const float o[3] = {o1[0] + o2[0], o1[1] + o2[1], o1[2] + o2[2]};
+      put(4 + m, o);
     }
     const float D = (tr[0] + tr[1]) + (tr[2] + tr[3]);
-    float* yp = sbuf + CW_NT * 30 + t * 4;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) yp[v] = c * (D + tr[v]);
+    for (int v = 0; v < 4; ++v) op.Yp[op.pos1[v * nC + e]] = c * (D + tr[v]);
   }
-  __syncthreads();
-  for (int i = t; i < ncell * 30; i += CW_NT) op.Yu[30 * e0 + i] = sbuf[i];
-  for (int i = t; i < ncell * 4; i += CW_NT) op.Yp[4 * e0 + i] = sbuf[CW_NT * 30 + i];
 }
 
 // Phase-2 epilogue of a plain application: y = A x, or y = b - A x with b;
@@ -226,20 +210,20 @@ cw_rows_kernel(CellOp op, const float* __restrict__ x, Epi epi, double* __restri
       if (a < op.nP2) {
         const int kb = op.inc2_rp[a], ke = op.inc2_rp[a + 1];
         float y0 = 0.f, y1 = 0.f, y2 = 0.f;
-        // two incident cells per lane per trip, loads issued before use
+        // the node's entries are contiguous: two per lane per trip, loads
+        // issued before use
         for (int k = kb + lane; k < ke; k += 2 * GU) {
-          const int k2 = k + GU;
-          const int sa = op.inc2_slot[k];
-          const int sb = op.inc2_slot[k2 < ke ? k2 : k];
-          const float a0 = Yu[3ll * sa], a1 = Yu[3ll * sa + 1], a2 = Yu[3ll * sa + 2];
-          float b0 = Yu[3ll * sb], b1 = Yu[3ll * sb + 1], b2 = Yu[3ll * sb + 2];
-          if (!(k2 < ke)) b0 = b1 = b2 = 0.f;
-          y0 += a0;
-          y1 += a1;
-          y2 += a2;
-          y0 += b0;
-          y1 += b1;
-          y2 += b2;
+          const int k2 = k + GU < ke ? k + GU : k;
+          float av[3], bv[3];
+          load_nc<float, 3>(Yu + 3ll * k, av);
+          load_nc<float, 3>(Yu + 3ll * k2, bv);
+          if (!(k + GU < ke)) bv[0] = bv[1] = bv[2] = 0.f;
+          y0 += av[0];
+          y1 += av[1];
+          y2 += av[2];
+          y0 += bv[0];
+          y1 += bv[1];
+          y2 += bv[2];
         }
 #pragma unroll
         for (int off = GU / 2; off > 0; off >>= 1) {
@@ -258,7 +242,7 @@ cw_rows_kernel(CellOp op, const float* __restrict__ x, Epi epi, double* __restri
       if (c < op.nP1) {
         float yd = 0.f;
         const int kb = op.inc1_rp[c], ke = op.inc1_rp[c + 1];
-        for (int k = kb + lane; k < ke; k += GP) yd += Yp[op.inc1_slot[k]];
+        for (int k = kb + lane; k < ke; k += GP) yd += Yp[k];
         float m[F], sv[F];
 #pragma unroll
         for (int g = 0; g < F; ++g) { m[g] = 0.f; sv[g] = 0.f; }
@@ -361,8 +345,8 @@ static int cw_launch(const CellOp& op, const float* x, Epi epi, double* partial,
 // the work arrays Yu = work, Yp = work + 30 nC; checks the descriptor.
 static int cw_from_desc(const MpetCellOpDesc* d, float* work, CellOp& op) {
   if (!d || d->nC <= 0 || d->nP2 <= 0 || d->nP1 <= 0 || d->A < 1 || d->A + 1 > MPET_MAXF ||
-      d->nnz11 <= 0 || !d->cell_p2 || !d->cell_p1 || !d->geom || !d->inc2_rp ||
-      !d->inc2_slot || !d->inc1_rp || !d->inc1_slot || !d->rp11 || !d->ci11 || !d->M1 ||
+      d->nnz11 <= 0 || !d->node2 || !d->node1 || !d->pos2 || !d->pos1 || !d->geom ||
+      !d->inc2_rp || !d->inc1_rp || !d->rp11 || !d->ci11 || !d->M1 ||
       !d->St || !work) {
     mpet_set_error("invalid cell-wise operator description");
     return ERR_INVALID;
@@ -371,13 +355,13 @@ static int cw_from_desc(const MpetCellOpDesc* d, float* work, CellOp& op) {
   op.nP2 = d->nP2;
   op.nP1 = d->nP1;
   op.A = d->A;
-  op.cell_p2 = d->cell_p2;
-  op.cell_p1 = d->cell_p1;
+  op.node2 = d->node2;
+  op.node1 = d->node1;
+  op.pos2 = d->pos2;
+  op.pos1 = d->pos1;
   op.geom = d->geom;
   op.inc2_rp = d->inc2_rp;
-  op.inc2_slot = d->inc2_slot;
   op.inc1_rp = d->inc1_rp;
-  op.inc1_slot = d->inc1_slot;
   op.rp11 = d->rp11;
   op.ci11 = d->ci11;
   op.nnz11 = d->nnz11;

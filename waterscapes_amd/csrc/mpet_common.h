@@ -28,6 +28,14 @@ enum Status : int {
   ERR_BUDGET = 6,       // admission refused (serving)
 };
 
+// NC consecutive values at p (NC * sizeof(T) bytes, element-aligned) as one
+// memory instruction where the width allows (FP32 x 3: dwordx3), instead of
+// NC separate gathers.
+template <typename T, int NC>
+__device__ __forceinline__ void load_nc(const T* __restrict__ p, T (&v)[NC]) {
+  __builtin_memcpy(v, p, sizeof(T) * NC);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);

@@ -52,8 +52,11 @@ __device__ __forceinline__ void urow_apply(const int* __restrict__ rp22,
       kb[m] = Kuu[m * n22 + ib];
     }
     const T d0 = Dv[ip], d1 = Dv[n21 + ip], d2 = Dv[2 * n21 + ip];
-    T a0 = x[ca], a1 = x[ca + 1], a2 = x[ca + 2];
-    T b0 = x[cb], b1 = x[cb + 1], b2 = x[cb + 2];
+    T xa[3], xb[3];
+    load_nc<T, 3>(x + ca, xa);
+    load_nc<T, 3>(x + cb, xb);
+    T a0 = xa[0], a1 = xa[1], a2 = xa[2];
+    T b0 = xb[0], b1 = xb[1], b2 = xb[2];
     T p = x[cp];
     if (!oa) a0 = a1 = a2 = 0;
     if (!ob) b0 = b1 = b2 = 0;
@@ -107,10 +110,8 @@ __device__ __forceinline__ void prow_apply(const int* __restrict__ rp12,
 #pragma unroll
     for (int j = 0; j < U; ++j) {
 #pragma unroll
-      for (int e = 0; e < 3; ++e) {
-        dt[j][e] = Dt[e * n12 + iu[j]];
-        xu[j][e] = x[cu[j] + e];
-      }
+      for (int e = 0; e < 3; ++e) dt[j][e] = Dt[e * n12 + iu[j]];
+      load_nc<T, 3>(x + cu[j], xu[j]);
     }
     const T mm = M1[ip];
     T xv[FM], st[FM];

@@ -163,9 +163,10 @@ class DeviceSystem:
 
     def cell_op(self):
         """Arrays of the cell-wise FP32 operator, built once: per-cell
-        geometry (FP32 gradients of the barycentric coordinates and |T|) and
-        the node -> cell-slot incidence lists (slots e*10 + k of P2 nodes,
-        e*4 + v of P1 nodes, ascending within a node)."""
+        geometry (FP32 gradients of the barycentric coordinates and |T|),
+        the cell -> node tables in structure-of-arrays layout, and for every
+        (cell, local node) its position in the list of all of them sorted by
+        node, then cell (the entries of one node contiguous)."""
         if getattr(self, "_cw", None) is None:
             dev = self.device
             nC = int(self.cells.shape[0])
@@ -173,14 +174,20 @@ class DeviceSystem:
             check(_lib.lib().mpet_cell_geometry(C.byref(self.mesh_desc), ptr(geom),
                                                 _lib.stream_ptr()))
 
-            def incidence(table, nnode):
-                nodes = table.reshape(-1).to(torch.int64)
-                slot = torch.argsort(nodes, stable=True).to(torch.int32)
+            def tables(cell_nodes, nnode):
+                k = cell_nodes.shape[1]
+                nodes = cell_nodes.reshape(-1).to(torch.int64)      # slot e*k + j
+                order = torch.argsort(nodes, stable=True)           # entries by node, cell
+                pos = torch.empty_like(order)
+                pos[order] = torch.arange(order.numel(), device=dev)
                 rp = torch.zeros(nnode + 1, dtype=torch.int64, device=dev)
                 rp[1:] = torch.cumsum(torch.bincount(nodes, minlength=nnode), 0)
-                return rp.to(torch.int32), slot
-            self._cw = {"nC": nC, "geom": geom, "inc2": incidence(self.cell_p2, self.nP2),
-                        "inc1": incidence(self.cells, self.nP1)}
+                soa = lambda t: t.reshape(nC, k).t().contiguous().to(torch.int32)
+                return soa(nodes), soa(pos), rp.to(torch.int32)
+            n2, p2, rp2 = tables(self.cell_p2, self.nP2)
+            n1, p1, rp1 = tables(self.cells, self.nP1)
+            self._cw = {"nC": nC, "geom": geom, "node2": n2, "pos2": p2, "rp2": rp2,
+                        "node1": n1, "pos1": p1, "rp1": rp1}
         return self._cw
 
     def cell_op_desc(self, masked=True, lanes=None, xcd_chunk=None):
@@ -193,9 +200,9 @@ class DeviceSystem:
         lu, lp = lanes or self.cell_lanes
         xc = self.cell_xcd_chunk if xcd_chunk is None else xcd_chunk
         p11 = self.pat_dev["11"]
-        return _lib.CellOpDesc(cw["nC"], self.nP2, self.nP1, self.A, ptr(self.cell_p2),
-                               ptr(self.cells), ptr(cw["geom"]), ptr(cw["inc2"][0]),
-                               ptr(cw["inc2"][1]), ptr(cw["inc1"][0]), ptr(cw["inc1"][1]),
+        return _lib.CellOpDesc(cw["nC"], self.nP2, self.nP1, self.A, ptr(cw["node2"]),
+                               ptr(cw["node1"]), ptr(cw["pos2"]), ptr(cw["pos1"]),
+                               ptr(cw["geom"]), ptr(cw["rp2"]), ptr(cw["rp1"]),
                                ptr(p11[0]), ptr(p11[1]), self.nnz["11"], ptr(v["M1"]),
                                ptr(v["St"]), _lib.make_coef(self.Cm, self.Cs, self.mu),
                                ptr(self.dmask) if masked else None, lu, lp, xc)

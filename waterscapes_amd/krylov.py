@@ -158,7 +158,11 @@ def _pick_lpr(avg, nc=1, rows=None):
     if rows is not None:
         while rows * lanes < fill and lanes < 64 and lanes < avg:
             lanes *= 2
-    return max(lanes, 4 if nc == 3 else 1)
+    if nc == 3 and lanes < 4:
+        # three-column rows of a few nonzeros (the prolongator from the P1
+        # level): one thread per row; otherwise one lane per component
+        return 1 if avg <= 4.0 else 4
+    return lanes
 
 
 def compact_level(pattern_host, values):
