@@ -103,17 +103,23 @@ def op_bytes(sys_, dtype):
 
 def cell_op_bytes(sys_):
     """Algorithmic HBM bytes of one cell-wise FP32 operator apply
-    (csrc/cellwise.h), each array once: the cell kernel reads the four cell
-    tables (14 node and 14 position ints per cell), the geometry (10 floats)
-    and x, and writes the 34 per-cell results; the row kernel reads the
-    node offsets, the 34 per-cell results, the P1 x P1 block (indices, M1,
-    A stiffness arrays), the pressure part of x, and writes y."""
-    nC = sys_.cell_op()["nC"]
+    (csrc/cellwise.h), each array once: the cell kernel reads the node
+    tables (14 ints per cell), the geometry (10 floats), x and the block
+    partial tables (block and slot offsets, entries, 2-byte local slots),
+    and writes the partials (3 floats per P2 partial, 1 per P1 partial); the
+    row kernel reads the node offsets, the partials, the P1 x P1 block
+    (indices, M1, A stiffness arrays) and the pressure part of x, and
+    writes y."""
+    cw = sys_.cell_op()
+    nC = cw["nC"]
     n, F, A = sys_.n, sys_.F, sys_.A
     nz11 = sys_.nnz["11"]
-    cells = nC * (28 * 4 + 10 * 4) + n * 4 + nC * 34 * 4
-    rows = ((sys_.nP2 + 1) + 2 * (sys_.nP1 + 1)) * 4 + nC * 34 * 4 + nz11 * (4 + 4 + 4 * A) \
-        + F * sys_.nP1 * 4 + n * 4
+    nb = (nC + sys_.CELLS_PER_BLOCK - 1) // sys_.CELLS_PER_BLOCK
+    p2, p1 = cw["p2"]["npart"], cw["p1"]["npart"]
+    tables = (nb + 1) * 4 * 2 + (2 * p2 + 1) * 4 + 10 * nC * 2 + (2 * p1 + 1) * 4 + 4 * nC * 2
+    cells = nC * (14 * 4 + 10 * 4) + n * 4 + tables + (3 * p2 + p1) * 4
+    rows = ((sys_.nP2 + 1) + 2 * (sys_.nP1 + 1)) * 4 + (3 * p2 + p1) * 4 \
+        + nz11 * (4 + 4 + 4 * A) + F * sys_.nP1 * 4 + n * 4
     return cells + rows
 
 
@@ -453,6 +459,8 @@ def fp64_service_times(specs):
             next(gen)
         torch.cuda.synchronize()
         out.append(time.perf_counter() - t0)
+        print(f"[serve] FP64 alone: request {len(out)}/{len(specs)} (N={N}, {steps} steps) "
+              f"{out[-1]:.2f} s", file=sys.stderr, flush=True)
         del gen, s
         gc.collect()
     return out
@@ -490,7 +498,7 @@ def main_serve(args):
             srv.submit(r)
         t0 = time.perf_counter()
         last = t0
-        while srv.waiting or srv.active:
+        while srv.busy():
             if not srv.tick():
                 time.sleep(1e-3)
             if time.perf_counter() - last > 10:
@@ -499,6 +507,7 @@ def main_serve(args):
                       f"waiting {len(srv.waiting)} finished {len(srv.finished)}",
                       file=sys.stderr, flush=True)
         srv._drop_batch()
+        srv.close()
         wall = time.perf_counter() - t0
         lat = srv.latencies()
         done = [r for r in srv.finished if not r.refused]
@@ -871,7 +880,7 @@ def main_probe(args):
     for _ in range(5):
         _lib.check(L.mpet_op_apply(C.byref(d32), _lib.ptr(x), None, _lib.ptr(y), sp))
     cd = s.engine().krylov.cell_desc
-    work = torch.empty(34 * sysm.cell_op()["nC"], dtype=torch.float32, device="cuda")
+    work = torch.empty(sysm.cell_work_floats(), dtype=torch.float32, device="cuda")
     for _ in range(5):
         _lib.check(L.mpet_cell_op_apply(C.byref(cd), _lib.ptr(x), None, _lib.ptr(y),
                                         _lib.ptr(work), sp))
@@ -1013,12 +1022,12 @@ def main():
     sp = _lib.stream_ptr()
     cdesc = eng.krylov.cell_desc
     if cdesc is not None:
-        work = torch.empty(34 * sysm.cell_op()["nC"], dtype=torch.float32, device=x32.device)
+        work = torch.empty(sysm.cell_work_floats(), dtype=torch.float32, device=x32.device)
         ms = time_kernel(lambda: L.mpet_cell_op_apply(C.byref(cdesc), _lib.ptr(x32), None,
                                                       _lib.ptr(y32), _lib.ptr(work), sp))
         abytes = cell_op_bytes(sysm)
-        rkernel = (f"cw_cell_kernel + cw_rows_kernel<{sysm.F},{cdesc.lanes_u},{cdesc.lanes_p}> "
-                   f"(cell-wise A, FP32)")
+        rkernel = (f"cw_cell_kernel<256> + cw_rows_kernel<{sysm.F},{cdesc.lanes_u},"
+                   f"{cdesc.lanes_p}> (cell-wise A, FP32)")
     d32 = sysm.op_desc(1)
     ms_s = time_kernel(lambda: L.mpet_op_apply(C.byref(d32), _lib.ptr(x32), None, _lib.ptr(y32),
                                                sp))

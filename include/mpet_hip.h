@@ -206,6 +206,17 @@ int mpet_assemble_primitives(const MpetMeshDesc* mesh, const MpetPatterns* pat,
                              const double* Kcell, int A, double mu,
                              const MpetPrimitives* out, hipStream_t stream);
 
+/* mpet_assemble_primitives of ninst instances at once (host arrays of
+ * ninst descriptors, per-instance device data): one launch per colour
+ * index assembles that colour of every instance, with the lanes and order
+ * of the single-instance assembly (each instance's values are bitwise those
+ * of its own assembly).  Synchronises the stream.  The reference assembles
+ * one problem at a time (:437, :444); no counterpart. */
+int mpet_assemble_primitives_batched(int ninst, const MpetMeshDesc* meshes,
+                                     const MpetPatterns* pats, const double* const* Kcell,
+                                     const int32_t* A, const double* mu,
+                                     const MpetPrimitives* outs, hipStream_t stream);
+
 /* A-posteriori error indicators of the two-field MPET step
  * (AdaptiveMPETSolver, mpet/adaptivempetsolver.py:82-185).  Mesh arrays as
  * in MpetMeshDesc (device); x, xp the current and previous two-field states
@@ -346,20 +357,31 @@ int mpet_op_apply(const MpetOpDesc* op, const void* x, const void* b, void* y,
  * integrated cell by cell from the geometry, exactly (P2 gradients are
  * linear); the P1 x P1 pressure block (C (x) M1 + stiff St, with the Robin
  * terms in St) is read from the FP32 structured copy.  Summation order is
- * fixed (each node's cell contributions are summed in cell order):
- * results are reproducible.  Cell tables structure-of-arrays, [k*nC + e]:
+ * fixed: results are reproducible.  Cells are taken in blocks of
+ * cells_per_block (= 256) consecutive cells; a block sums, per node its
+ * cells touch, their contributions in slot order, and writes one partial
+ * per (node, block); a row sums its node's partials in block order.
+ * Cell tables are structure-of-arrays, [k*nC + e]:
  *   node2 / node1  P2 node of local node k (UFC order) / P1 node of vertex k
- *   pos2 / pos1    entry of (cell e, local k) in the list of all (cell, k)
- *                  sorted by node then cell (the entries of a node are
- *                  contiguous, from inc2_rp[node] / inc1_rp[node])
  *   geom           10*nC floats from mpet_cell_geometry
- *   lanes_u in {4, 8, 16}, lanes_p in {8, 16} (others: 8)
+ * Partials, for P2 nodes (suffix 2, local slot c*10 + k, c the cell within
+ * its block) and P1 nodes (suffix 1, local slot c*4 + v):
+ *   bq[b] .. bq[b+1]-1   the partials of block b (nblocks + 1 entries)
+ *   lrp[q] .. lrp[q+1]-1 partial q's local slots in lsl (ascending)
+ *   gp[q]                partial q's entry in the node-sorted list
+ *   inc2_rp / inc1_rp    node -> first entry of its partials (n + 1)
+ * Work: 3*npart2 + npart1 floats.  lanes_u in {4, 8, 16}, lanes_p in
+ * {8, 16} (others: 8).
  * The reference always multiplies by the assembled PETSc matrix (:437). */
 typedef struct {
   int32_t nC, nP2, nP1, A;
   const int32_t *node2, *node1;  /* 10*nC, 4*nC */
-  const int32_t *pos2, *pos1;    /* 10*nC, 4*nC */
   const float* geom;
+  int32_t cells_per_block, npart2, npart1;
+  const int32_t *bq2, *lrp2, *gp2;
+  const int16_t* lsl2;
+  const int32_t *bq1, *lrp1, *gp1;
+  const int16_t* lsl1;
   const int32_t *inc2_rp, *inc1_rp;   /* nP2 + 1, nP1 + 1 */
   const int32_t *rp11, *ci11;
   int32_t nnz11;
@@ -375,7 +397,7 @@ typedef struct {
 int mpet_cell_geometry(const MpetMeshDesc* mesh, float* geom, hipStream_t stream);
 
 /* y = A x / y = b - A x with Dirichlet rows zeroed, as mpet_op_apply (FP32)
- * through the cell-wise operator; work: 34*nC floats (per-cell results). */
+ * through the cell-wise operator; work: 3*npart2 + npart1 floats. */
 int mpet_cell_op_apply(const MpetCellOpDesc* op, const float* x, const float* b, float* y,
                        float* work, hipStream_t stream);
 
@@ -520,7 +542,7 @@ int mpet_level_spmm(int dtype, int nc, int mode, int nrows, const int32_t* rp,
 int mpet_solver_set_tuning(void* h, int dtype, const MpetTuning* t);
 
 /* Inner FP32 MINRES of a mixed-precision solver handle on the cell-wise
- * operator (op != NULL; the handle allocates its 34*nC work floats and
+ * operator (op != NULL; the handle allocates its work floats and
  * keeps a copy of the descriptor, whose arrays must outlive it) or back on
  * the FP32 structured operator (op == NULL).  Captured graphs are dropped
  * and captured again by the next solve.  No counterpart in the reference. */

@@ -117,7 +117,7 @@ def test_assembly_is_bitwise_reproducible():
     params = cfg2_params(2)
     space, dev = _build(3, params, K_cells=lognormal_K(162, params["K"], seed=9))
     first = {k: v.clone() for k, v in dev.prim_t.items()}
-    dev._assemble()
+    dev.assemble_primitives()
     torch.cuda.synchronize()
     for k, v in dev.prim_t.items():
         assert torch.equal(v, first[k]), k

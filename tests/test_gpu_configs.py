@@ -159,7 +159,11 @@ def test_cfg5_stream_with_48cube_request_isolation():
     from waterscapes_amd.serve import MPETServer
     spec = _stream()
     assert any(N == 48 and p["J"] == 4 for N, p, *_ in spec)
-    srv = MPETServer(hbm_budget=200e9, precision="mixed")
+    # synchronous set-up: the small requests that arrive while the big one
+    # is set up and stepped join it mid-flight (with set-up threads they
+    # would finish before its 3 s set-up is done: tests/test_gpu_serve.py
+    # covers that form)
+    srv = MPETServer(hbm_budget=200e9, precision="mixed", setup_workers=0)
     for r in _requests(spec):
         srv.submit(r)
     srv.run()

@@ -39,13 +39,16 @@ def _requests(which):
 
 
 def _serve(reqs, budget=200e9):
+    """Synchronous set-up (setup_workers = 0), so that admission happens at
+    the tick a request arrives and the schedule is deterministic."""
     from waterscapes_amd.serve import MPETServer
     clock = VClock()
-    srv = MPETServer(hbm_budget=budget, precision="mixed", outer_tol=1e-11, clock=clock)
+    srv = MPETServer(hbm_budget=budget, precision="mixed", outer_tol=1e-11, clock=clock,
+                     setup_workers=0)
     for r in reqs:
         srv.submit(r)
     concurrency = []
-    while srv.waiting or srv.active:
+    while srv.busy():
         concurrency.append(srv.tick())          # instances stepped together
         clock.t += 1.0
     return srv, concurrency
@@ -106,3 +109,43 @@ def test_footprint_estimate_bounds_allocation():
     # torch's allocations only (native workspaces are extra): the estimate
     # must cover them and not overshoot grossly
     assert 0.5 * used <= est <= 6.0 * used, (est, used)
+
+
+def test_setup_threads_overlap_ticks_and_keep_results():
+    """With set-up threads (the default), a request whose set-up is slow
+    does not hold up the others: ticks keep stepping the running instances
+    while it is prepared, and every result equals the synchronous server's."""
+    import time
+    from waterscapes_amd.serve import MPETServer
+    sync, _ = _serve(_requests([0, 2, 3]))
+    ref = {r.rid: r.result for r in sync.finished}
+    reqs = _requests([0, 2, 3])
+    slow = reqs[1].make_solver
+
+    def slow_make():
+        time.sleep(1.0)                          # a long host set-up
+        return slow()
+    reqs[1].make_solver = slow_make
+    for r in reqs:
+        r.arrival = 0.0
+    srv = MPETServer(hbm_budget=200e9, precision="mixed", outer_tol=1e-11, setup_workers=2)
+    for r in reqs:
+        srv.submit(r)
+    ticks_while_preparing = 0
+    while srv.busy():
+        prep = bool(srv.preparing) or any(r._fp is not None and not r._fp.done()
+                                          for r in srv.waiting)
+        if srv.tick() and prep:
+            ticks_while_preparing += 1
+        elif not srv.active:
+            time.sleep(1e-3)
+    srv.close()
+    assert ticks_while_preparing >= 1
+    done = {r.rid: r for r in srv.finished}
+    assert len(done) == 3 and all(r.steps_done == r.steps for r in done.values())
+    from waterscapes_amd.fem import MixedSpace
+    from waterscapes_amd.mesh import UnitCubeMesh
+    for rid, r in done.items():
+        n, params = REQS[rid][0], REQS[rid][1]
+        space = MixedSpace(UnitCubeMesh(n, n, n), params["J"])
+        assert max(field_errors(space, r.result, ref[rid])) <= ISOLATION_RTOL, rid

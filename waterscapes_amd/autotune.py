@@ -200,7 +200,7 @@ def tune_cell_operator(sys_, cache=None, reps=10, repeats=3, margin=0.02):
     g = torch.Generator(device=sys_.device).manual_seed(0)
     x = torch.randn(sys_.n, generator=g, device=sys_.device, dtype=torch.float64).float()
     y = torch.empty_like(x)
-    work = torch.empty(34 * sys_.cell_op()["nC"], dtype=torch.float32, device=sys_.device)
+    work = torch.empty(sys_.cell_work_floats(), dtype=torch.float32, device=sys_.device)
     default = tuple(sys_.cell_lanes) + (sys_.cell_xcd_chunk,)
 
     def timer(c):

@@ -160,33 +160,45 @@ def pack_hierarchy(hiers, scales, skip_A0=False):
     return out, (inv, blk, _i32(off), ioff)
 
 
-def batch_engine(solver, precision="mixed"):
+def batch_engine(solver, precision="mixed", system=None):
     """The batch-ready device state of one solver (assembled system, Dirichlet
     data, RHS kernels, host AMG hierarchies), created once and kept on the
-    solver, so that re-packing a batch never repeats assembly or AMG setup."""
+    solver, so that re-packing a batch never repeats assembly or AMG setup.
+    system: its DeviceSystem, already assembled (by a batched assembly)."""
     eng = getattr(solver, "_engine", None)
     if eng is not None and getattr(eng, "batched", False) and eng.config.mode == precision:
         return eng
     p = solver.params
     cfg = SolverConfig(mode=precision, p0_weight=p["p0_weight"])
     eng = StepEngine(solver.problem, solver.up_.function_space(), p["dt"], p["theta"], cfg,
-                     p["device"], batched=True)
+                     p["device"], batched=True, system=system)
     solver._engine = eng
     return eng
 
 
 def build_engines(solvers, precision="mixed", workers=None):
-    """batch_engine of every solver.  Instances that have no engine yet are
-    set up concurrently by a pool of host threads (their hierarchy setups
-    run in native code without the interpreter lock, with the host cores
-    split among the workers), each worker enqueuing its device work on a
-    stream of its own and synchronising it before the engine is used."""
+    """batch_engine of every solver.  Instances that have no engine yet get
+    their systems assembled together (one launch per colour index for all
+    of them, device.assemble_batch) and are then set up concurrently by a
+    pool of host threads (their hierarchy setups run in native code without
+    the interpreter lock, with the host cores split among the workers), each
+    worker enqueuing its device work on a stream of its own and
+    synchronising it before the engine is used."""
     import concurrent.futures as cf
     import os
     import threading
+    from .device import assemble_batch
+    from .solver import batch_system
     todo = [s for s in solvers if not (getattr(s, "_engine", None) is not None
                                        and getattr(s._engine, "batched", False)
                                        and s._engine.config.mode == precision)]
+    systems = {}
+    if len(todo) > 1:
+        for s in todo:
+            systems[id(s)] = batch_system(s.problem, s.up_.function_space(), s.params["dt"],
+                                          s.params["theta"], s.params["device"],
+                                          assemble=False)
+        assemble_batch(list(systems.values()))
     ncpu = len(os.sched_getaffinity(0))
     workers = workers or max(1, min(8, len(todo), ncpu // 2))
     if workers > 1 and len(todo) > 1:
@@ -198,12 +210,12 @@ def build_engines(solvers, precision="mixed", workers=None):
                 local.stream = torch.cuda.Stream(device=dev)
                 amg.set_host_threads(max(1, ncpu // workers))
             with torch.cuda.stream(local.stream):
-                e = batch_engine(s, precision)
+                e = batch_engine(s, precision, systems.get(id(s)))
             local.stream.synchronize()
             return e
         with cf.ThreadPoolExecutor(max_workers=workers) as ex:
             list(ex.map(work, todo))
-    return [batch_engine(s, precision) for s in solvers]
+    return [batch_engine(s, precision, systems.get(id(s))) for s in solvers]
 
 
 class MPETBatch:

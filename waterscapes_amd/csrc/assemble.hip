@@ -13,6 +13,7 @@
 // summation order of every entry is fixed: assembly is bitwise reproducible
 // without atomics.
 #include "mpet_common.h"
+#include <vector>
 #include "../../include/mpet_hip.h"
 
 namespace mpet {
@@ -90,18 +91,14 @@ __device__ __forceinline__ void p2p1_div(const CellGeom& G, int b, int c, double
   }
 }
 
-// One lane per (cell, local row node): lanes 0..9 own the P2 rows of the
-// cell, lanes 10..13 its P1 rows; 16 lanes per cell, 4 cells per wave.
-__global__ void __launch_bounds__(256)
-assemble_colour_kernel(const double* __restrict__ X, const int* __restrict__ cells,
-                       const int* __restrict__ cell_p2, const int* __restrict__ colour_cells,
-                       int cbeg, int cend, const double* __restrict__ Kcell, int A, double mu,
-                       MpetPatterns pat, MpetPrimitives out) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  int ci = cbeg + t / 16;
-  int task = t % 16;
-  if (ci >= cend || task >= 14) return;
-  int e = colour_cells[ci];
+// Lane `task` of cell e: tasks 0..9 own the P2 rows of the cell, 10..13
+// its P1 rows (each row written by one lane of one cell of the colour).
+__device__ __forceinline__ void assemble_cell(const double* __restrict__ X,
+                                              const int* __restrict__ cells,
+                                              const int* __restrict__ cell_p2, int e, int task,
+                                              const double* __restrict__ Kcell, int A,
+                                              double mu, const MpetPatterns& pat,
+                                              const MpetPrimitives& out) {
   CellGeom G;
   cell_geometry(X, cells, e, G);
   if (task < 10) {
@@ -151,6 +148,74 @@ assemble_colour_kernel(const double* __restrict__ X, const int* __restrict__ cel
       for (int i = 0; i < A; ++i) out.St[i * nnz11 + pos] += Kcell[(long long)e * A + i] * s;
     }
   }
+}
+
+// One lane per (cell, local row node): lanes 0..9 own the P2 rows of the
+// cell, lanes 10..13 its P1 rows; 16 lanes per cell, 4 cells per wave.
+__global__ void __launch_bounds__(256)
+assemble_colour_kernel(const double* __restrict__ X, const int* __restrict__ cells,
+                       const int* __restrict__ cell_p2, const int* __restrict__ colour_cells,
+                       int cbeg, int cend, const double* __restrict__ Kcell, int A, double mu,
+                       MpetPatterns pat, MpetPrimitives out) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int ci = cbeg + t / 16;
+  int task = t % 16;
+  if (ci >= cend || task >= 14) return;
+  assemble_cell(X, cells, cell_p2, colour_cells[ci], task, Kcell, A, mu, pat, out);
+}
+
+// One instance of a batched assembly (device copies of its descriptors).
+struct AsmInst {
+  const double* X;
+  const int* cells;
+  const int* cell_p2;
+  const int* colour_cells;
+  const double* Kcell;
+  int A;
+  double mu;
+  MpetPatterns pat;
+  MpetPrimitives out;
+};
+
+// Colour k of every instance in one launch: cells g of the colour, counted
+// over the instances in order (off[i] .. off[i+1]-1 belong to instance i,
+// its colour starting at colour_cells[cb[i]]); the same lanes and order as
+// assemble_colour_kernel, so each instance's sums are those of its own
+// assembly.
+__global__ void __launch_bounds__(256)
+assemble_colour_batched_kernel(const AsmInst* __restrict__ inst, int ninst,
+                               const long long* __restrict__ off,
+                               const long long* __restrict__ cb) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = t / 16;
+  const int task = (int)(t % 16);
+  if (g >= off[ninst] || task >= 14) return;
+  int lo = 0, hi = ninst;                 // off[lo] <= g < off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= g) lo = mid; else hi = mid;
+  }
+  const AsmInst& I = inst[lo];
+  const long long ci = cb[lo] + (g - off[lo]);
+  assemble_cell(I.X, I.cells, I.cell_p2, I.colour_cells[ci], task, I.Kcell, I.A, I.mu, I.pat,
+                I.out);
+}
+
+// Zeroes segments seg[i] = (pointer, count) of doubles, one block row per
+// segment.
+struct ZeroSeg {
+  double* p;
+  long long n;
+};
+__global__ void __launch_bounds__(256)
+zero_segments_kernel(const ZeroSeg* __restrict__ seg, int nseg) {
+  const int s = blockIdx.y;
+  if (s >= nseg) return;
+  double* __restrict__ p = seg[s].p;
+  const long long n = seg[s].n;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0.0;
 }
 
 // P2 mass int phi_a phi_b on the P2 pattern, one colour of cells: lanes
@@ -355,6 +420,102 @@ extern "C" int mpet_assemble_primitives(const MpetMeshDesc* mesh, const MpetPatt
   }
   HIP_TRY(hipGetLastError());
   return OK;
+}
+
+extern "C" int mpet_assemble_primitives_batched(int ninst, const MpetMeshDesc* meshes,
+                                                const MpetPatterns* pats,
+                                                const double* const* Kcell, const int32_t* A,
+                                                const double* mu, const MpetPrimitives* outs,
+                                                hipStream_t stream) {
+  if (ninst < 1 || !meshes || !pats || !Kcell || !A || !mu || !outs) {
+    mpet_set_error("mpet_assemble_primitives_batched: invalid argument");
+    return ERR_INVALID;
+  }
+  int ncol = 0;
+  std::vector<AsmInst> inst(ninst);
+  std::vector<ZeroSeg> segs;
+  for (int i = 0; i < ninst; ++i) {
+    const MpetMeshDesc& m = meshes[i];
+    if (A[i] < 1 || A[i] > MPET_MAXF - 1 || m.ncolours < 0 || !m.colour_offsets) {
+      mpet_set_error("mpet_assemble_primitives_batched: invalid instance");
+      return ERR_INVALID;
+    }
+    ncol = m.ncolours > ncol ? m.ncolours : ncol;
+    inst[i] = AsmInst{m.coords, m.cells, m.cell_p2, m.colour_cells, Kcell[i], A[i], mu[i],
+                      pats[i], outs[i]};
+    const MpetPatterns& pt = pats[i];
+    const MpetPrimitives& o = outs[i];
+    segs.push_back({o.Kuu, 9ll * pt.nnz22});
+    segs.push_back({o.Lap, (long long)pt.nnz22});
+    segs.push_back({o.Dv, 3ll * pt.nnz21});
+    segs.push_back({o.Dt, 3ll * pt.nnz12});
+    segs.push_back({o.M1, (long long)pt.nnz11});
+    segs.push_back({o.St, (long long)A[i] * pt.nnz11});
+  }
+  // per colour: cell offsets over the instances (ninst + 1) and each
+  // instance's first cell of the colour (ninst)
+  std::vector<long long> tab((size_t)ncol * (2 * ninst + 1), 0);
+  std::vector<long long> total(ncol, 0);
+  for (int k = 0; k < ncol; ++k) {
+    long long* off = tab.data() + (size_t)k * (2 * ninst + 1);
+    long long* cb = off + ninst + 1;
+    off[0] = 0;
+    for (int i = 0; i < ninst; ++i) {
+      const MpetMeshDesc& m = meshes[i];
+      long long n = 0, b = 0;
+      if (k < m.ncolours) {
+        b = m.colour_offsets[k];
+        n = m.colour_offsets[k + 1] - b;
+      }
+      off[i + 1] = off[i] + n;
+      cb[i] = b;
+    }
+    total[k] = off[ninst];
+  }
+  AsmInst* d_inst = nullptr;
+  long long* d_tab = nullptr;
+  ZeroSeg* d_seg = nullptr;
+  int rc = OK;
+  if (hipMalloc(&d_inst, sizeof(AsmInst) * ninst) != hipSuccess ||
+      hipMalloc(&d_tab, sizeof(long long) * (tab.size() ? tab.size() : 1)) != hipSuccess ||
+      hipMalloc(&d_seg, sizeof(ZeroSeg) * segs.size()) != hipSuccess) {
+    mpet_set_error("mpet_assemble_primitives_batched: out of device memory");
+    rc = ERR_OOM;
+  }
+  if (!rc && (hipMemcpyAsync(d_inst, inst.data(), sizeof(AsmInst) * ninst,
+                             hipMemcpyHostToDevice, stream) != hipSuccess ||
+              (tab.size() && hipMemcpyAsync(d_tab, tab.data(), sizeof(long long) * tab.size(),
+                                            hipMemcpyHostToDevice, stream) != hipSuccess) ||
+              hipMemcpyAsync(d_seg, segs.data(), sizeof(ZeroSeg) * segs.size(),
+                             hipMemcpyHostToDevice, stream) != hipSuccess)) {
+    mpet_set_error("mpet_assemble_primitives_batched: upload failed");
+    rc = ERR_HIP;
+  }
+  if (!rc) {
+    hipLaunchKernelGGL(zero_segments_kernel, dim3(64, (unsigned)segs.size()), dim3(256), 0,
+                       stream, d_seg, (int)segs.size());
+    for (int k = 0; k < ncol; ++k) {
+      if (total[k] <= 0) continue;
+      const long long threads = 16ll * total[k];
+      const int blocks = (int)((threads + 255) / 256);
+      const long long* off = d_tab + (size_t)k * (2 * ninst + 1);
+      hipLaunchKernelGGL(assemble_colour_batched_kernel, dim3(blocks), dim3(256), 0, stream,
+                         d_inst, ninst, off, off + ninst + 1);
+    }
+    if (hipGetLastError() != hipSuccess) {
+      mpet_set_error("mpet_assemble_primitives_batched: launch failed");
+      rc = ERR_HIP;
+    }
+  }
+  // the tables are freed once the launches that read them have completed
+  if (hipStreamSynchronize(stream) != hipSuccess && !rc) {
+    mpet_set_error("mpet_assemble_primitives_batched: stream synchronisation failed");
+    rc = ERR_HIP;
+  }
+  if (d_inst) (void)hipFree(d_inst);
+  if (d_tab) (void)hipFree(d_tab);
+  if (d_seg) (void)hipFree(d_seg);
+  return rc;
 }
 
 extern "C" int mpet_assemble_p2_mass(const MpetMeshDesc* mesh, const MpetPatterns* pat,

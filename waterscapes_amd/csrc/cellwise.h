@@ -23,14 +23,16 @@
 // (u_j, g_n, s_n in 3-component / 3x3 form; y(.) are the 3 rows of a P2
 // node's u DOFs.)
 //
-// Two launches: cw_cell_kernel writes each cell's 30 + 4 local results to
-// their places in node-sorted entry lists (the entries of one node
-// contiguous, ordered by cell: Yu[3 pos2(e, k) + b], Yp[pos1(e, v)]);
-// cw_rows_kernel gives every row the sum of its contiguous run (fixed order,
-// so results are reproducible) plus the assembled P1 x P1 part, and hands
-// the row value to an epilogue (plain apply / masked residual, or the fused
-// MINRES step).  Cell tables are structure-of-arrays ([k * nC + e]) so that
-// every load of the cell kernel is one contiguous segment per wave.
+// Two launches.  cw_cell_kernel (one thread per cell, CW_NT cells per
+// block) computes each cell's 30 + 4 local results into LDS, then sums, per
+// node touched by the block's cells, that node's contributions from these
+// cells (in slot order) and writes one partial per (node, block) to its
+// place in node-sorted partial lists (a node's partials contiguous, in
+// block order: ~3x fewer stores than one per (cell, node)).  cw_rows_kernel
+// gives every row the sum of its contiguous run of partials (fixed order, so
+// results are reproducible) plus the assembled P1 x P1 part, and hands the
+// row value to an epilogue (plain apply / masked residual, or the fused
+// MINRES step).  Cell tables are structure-of-arrays ([k * nC + e]).
 #pragma once
 #include "amg_cycle.h"
 
@@ -41,8 +43,13 @@ struct CellOp {
   int nC, nP2, nP1, A;
   const int* node2;       // [k * nC + e]: P2 node of local node k (UFC order)
   const int* node1;       // [v * nC + e]: P1 node of vertex v
-  const int* pos2;        // [k * nC + e]: entry of (e, k) in the P2-node-sorted list
-  const int* pos1;        // [v * nC + e]: entry of (e, v) in the P1-node-sorted list
+  // partials of block b: q in [bq[b], bq[b+1]); partial q sums the local
+  // slots lsl[lrp[q] .. lrp[q+1]) (c * 10 + k for P2, c * 4 + v for P1, c the
+  // cell within the block) and goes to entry gp[q] of the node-sorted list
+  const int *bq2, *lrp2, *gp2;
+  const int16_t* lsl2;
+  const int *bq1, *lrp1, *gp1;
+  const int16_t* lsl1;
   const float* geom;      // SoA: [d * nC + e] = G_{1 + d/3}[d % 3] (d < 9); |T| at d = 9
   const int *inc2_rp, *inc1_rp;     // node -> first entry (nP2 + 1, nP1 + 1)
   const int *rp11, *ci11;
@@ -53,11 +60,15 @@ struct CellOp {
   float mu;
   const uint8_t* dmask;
   int lanes_u, lanes_p, xcd_chunk;
-  float* Yu;              // 3 x 10 nC, node-sorted
-  float* Yp;              // 4 nC, node-sorted
+  int npart2, npart1;     // partials of all blocks (P2, P1)
+  float* Yu;              // 3 x npart2, node-sorted
+  float* Yp;              // npart1, node-sorted
 };
 
-constexpr int CW_NT = 256;   // cells per block of the cell kernel
+#ifndef MPET_CW_CELLS
+#define MPET_CW_CELLS 256
+#endif
+constexpr int CW_NT = MPET_CW_CELLS;   // cells per block of the cell kernel
 
 // Index of the UFC edge joining local vertices j != k.
 __device__ __forceinline__ int cw_edge(int j, int k) {
@@ -74,7 +85,10 @@ __global__ void __launch_bounds__(NTB)
 cw_cell_kernel(CellOp op, const float* __restrict__ x, const int* __restrict__ skip) {
   static_assert(NTB == CW_NT, "one cell per thread of a CW_NT block");
   if (skip && *skip) return;
-  const long long e = (long long)blockIdx.x * CW_NT + threadIdx.x;
+  __shared__ float su[CW_NT * 30];
+  __shared__ float sp[CW_NT * 4];
+  const int t = threadIdx.x;
+  const long long e = (long long)blockIdx.x * CW_NT + t;
   if (e < op.nC) {
     const long long nC = op.nC;
     int n2[10], n1[4];
@@ -133,9 +147,10 @@ cw_cell_kernel(CellOp op, const float* __restrict__ x, const int* __restrict__ s
 #pragma unroll
     for (int q = 0; q < 6; ++q) S[q] = (s[0][q] + s[1][q]) + (s[2][q] + s[3][q]);
     const float c = vol * (1.0f / 20.0f);
-    float* __restrict__ Yu = op.Yu;
-    auto put = [&](int k, const float* o) {   // one 12-byte store
-      __builtin_memcpy(Yu + 3ll * op.pos2[k * nC + e], o, 12);
+    auto put = [&](int k, const float* o) {
+      su[t * 30 + 3 * k] = o[0];
+      su[t * 30 + 3 * k + 1] = o[1];
+      su[t * 30 + 3 * k + 2] = o[2];
     };
     // symmetric 3x3 (xx yy zz xy xz yz) times a vector
     auto symv = [](const float* M, const float* w, float* out) {
@@ -168,7 +183,29 @@ cw_cell_kernel(CellOp op, const float* __restrict__ x, const int* __restrict__ s
     }
     const float D = (tr[0] + tr[1]) + (tr[2] + tr[3]);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) op.Yp[op.pos1[v * nC + e]] = c * (D + tr[v]);
+    for (int v = 0; v < 4; ++v) sp[t * 4 + v] = c * (D + tr[v]);
+  }
+  __syncthreads();
+  // one partial per (node, block): the node's slots of this block, summed in
+  // slot order
+  const int b = blockIdx.x;
+  float* __restrict__ Yu = op.Yu;
+  for (int q = op.bq2[b] + t; q < op.bq2[b + 1]; q += CW_NT) {
+    const int lb = op.lrp2[q], le = op.lrp2[q + 1];
+    float a[3] = {0.f, 0.f, 0.f};
+    for (int i = lb; i < le; ++i) {
+      const int sl = op.lsl2[i];
+      a[0] += su[3 * sl];
+      a[1] += su[3 * sl + 1];
+      a[2] += su[3 * sl + 2];
+    }
+    __builtin_memcpy(Yu + 3ll * op.gp2[q], a, 12);
+  }
+  for (int q = op.bq1[b] + t; q < op.bq1[b + 1]; q += CW_NT) {
+    const int lb = op.lrp1[q], le = op.lrp1[q + 1];
+    float a = 0.f;
+    for (int i = lb; i < le; ++i) a += sp[op.lsl1[i]];
+    op.Yp[op.gp1[q]] = a;
   }
 }
 
@@ -341,11 +378,18 @@ static int cw_launch(const CellOp& op, const float* x, Epi epi, double* partial,
   return np;
 }
 
+// Work floats of a descriptor (the P2 and P1 partials).
+static inline long long cw_work_floats(const MpetCellOpDesc* d) {
+  return 3ll * d->npart2 + d->npart1;
+}
+
 // Host: device view of a descriptor (coefficients rounded to FP32) with
-// the work arrays Yu = work, Yp = work + 30 nC; checks the descriptor.
+// the work arrays Yu = work, Yp = work + 3 npart2; checks the descriptor.
 static int cw_from_desc(const MpetCellOpDesc* d, float* work, CellOp& op) {
   if (!d || d->nC <= 0 || d->nP2 <= 0 || d->nP1 <= 0 || d->A < 1 || d->A + 1 > MPET_MAXF ||
-      d->nnz11 <= 0 || !d->node2 || !d->node1 || !d->pos2 || !d->pos1 || !d->geom ||
+      d->nnz11 <= 0 || !d->node2 || !d->node1 || !d->geom || d->cells_per_block != CW_NT ||
+      d->npart2 <= 0 || d->npart1 <= 0 || !d->bq2 || !d->lrp2 || !d->gp2 || !d->lsl2 ||
+      !d->bq1 || !d->lrp1 || !d->gp1 || !d->lsl1 ||
       !d->inc2_rp || !d->inc1_rp || !d->rp11 || !d->ci11 || !d->M1 ||
       !d->St || !work) {
     mpet_set_error("invalid cell-wise operator description");
@@ -357,8 +401,16 @@ static int cw_from_desc(const MpetCellOpDesc* d, float* work, CellOp& op) {
   op.A = d->A;
   op.node2 = d->node2;
   op.node1 = d->node1;
-  op.pos2 = d->pos2;
-  op.pos1 = d->pos1;
+  op.bq2 = d->bq2;
+  op.lrp2 = d->lrp2;
+  op.gp2 = d->gp2;
+  op.lsl2 = d->lsl2;
+  op.bq1 = d->bq1;
+  op.lrp1 = d->lrp1;
+  op.gp1 = d->gp1;
+  op.lsl1 = d->lsl1;
+  op.npart2 = d->npart2;
+  op.npart1 = d->npart1;
   op.geom = d->geom;
   op.inc2_rp = d->inc2_rp;
   op.inc1_rp = d->inc1_rp;
@@ -375,7 +427,7 @@ static int cw_from_desc(const MpetCellOpDesc* d, float* work, CellOp& op) {
   op.lanes_p = cw_lanes_p(d->lanes_p);
   op.xcd_chunk = d->xcd_chunk;
   op.Yu = work;
-  op.Yp = work + 30ll * d->nC;
+  op.Yp = work + 3ll * d->npart2;
   return OK;
 }
 

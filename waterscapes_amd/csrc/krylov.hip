@@ -1240,7 +1240,11 @@ extern "C" int mpet_solver_set_cell_operator(void* h, const MpetCellOpDesc* op) 
     mpet_set_error("mpet_solver_set_cell_operator: sizes do not match the solver's operator");
     return ERR_INVALID;
   }
-  const long long need = 34ll * (op->nC > 0 ? op->nC : 1);
+  if (op->npart2 <= 0 || op->npart1 <= 0) {
+    mpet_set_error("invalid cell-wise operator description");
+    return ERR_INVALID;
+  }
+  const long long need = cw_work_floats(op);
   if (S->cw_work_n < need) {
     int rc = alloc_dev(S->allocs, &S->cw_work, (size_t)need);
     if (rc) return rc;

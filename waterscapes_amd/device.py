@@ -22,11 +22,74 @@ def _dev(a, dtype, device):
     return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(device)
 
 
+def block_partial_tables(cell_nodes, nnode, B):
+    """Block partials of the cell-wise operator for a cell -> node table
+    (nC x k tensor, any device): cells are taken in blocks of B; every
+    (node, block) pair is one partial, numbered in the order (node, block)
+    (its entry: a node's partials contiguous).  Returns the node table in
+    structure-of-arrays layout, node -> first entry (rp), the number of
+    partials, and per block b the partials q in [bq[b], bq[b+1]) (ordered by
+    entry) with their entry gp[q] and their local slots
+    lsl[lrp[q] .. lrp[q+1]) (c * k + j, ascending; c the cell in the block)."""
+    i32, i64 = torch.int32, torch.int64
+    dev = cell_nodes.device
+    nC, k = cell_nodes.shape
+    nodes = cell_nodes.reshape(-1).to(i64)              # slot e*k + j
+    ns = nodes.numel()
+    slot = torch.arange(ns, device=dev, dtype=i64)
+    blk = slot // (B * k)
+    nb = (nC + B - 1) // B
+    key = nodes * nb + blk                              # partial (node, block)
+    order = torch.argsort(key, stable=True)
+    ks = key[order]
+    first = torch.ones(ns, dtype=torch.bool, device=dev)
+    first[1:] = ks[1:] != ks[:-1]
+    pid_sorted = torch.cumsum(first.to(i64), 0) - 1
+    npart = int(pid_sorted[-1].item()) + 1
+    pid = torch.empty_like(pid_sorted)
+    pid[order] = pid_sorted                             # entry of each slot
+    pnode = nodes[order][first]
+    rp = torch.zeros(nnode + 1, dtype=i64, device=dev)
+    rp[1:] = torch.cumsum(torch.bincount(pnode, minlength=nnode), 0)
+    pblk = blk[order][first]                            # block of each partial
+    gp = torch.argsort(pblk, stable=True)               # block-major -> entry
+    bq = torch.zeros(nb + 1, dtype=i64, device=dev)
+    bq[1:] = torch.cumsum(torch.bincount(pblk, minlength=nb), 0)
+    order2 = torch.argsort(blk * npart + pid, stable=True)   # by (block, entry, slot)
+    lsl = (order2 - blk[order2] * (B * k)).to(torch.int16)
+    cnt = torch.bincount(pid, minlength=npart)[gp]
+    lrp = torch.zeros(npart + 1, dtype=i64, device=dev)
+    lrp[1:] = torch.cumsum(cnt, 0)
+    soa = nodes.reshape(nC, k).t().contiguous().to(i32)
+    return dict(node=soa, rp=rp.to(i32), npart=npart, bq=bq.to(i32), lrp=lrp.to(i32),
+                gp=gp.to(i32), lsl=lsl.contiguous())
+
+
+def assemble_batch(systems):
+    """Assemble the primitives of many systems created with assemble=False
+    in one launch per colour index (mpet_assemble_primitives_batched), then
+    finish each system's set-up."""
+    if not systems:
+        return
+    n = len(systems)
+    L = _lib.lib()
+    meshes = (_lib.MeshDesc * n)(*[s.mesh_desc for s in systems])
+    pats = (_lib.Patterns * n)(*[s.pat for s in systems])
+    kc = (C.c_void_p * n)(*[ptr(s.Kcell) for s in systems])
+    A = (C.c_int32 * n)(*[s.A for s in systems])
+    mu = (C.c_double * n)(*[s.mu for s in systems])
+    outs = (_lib.Primitives * n)(*[s.prim for s in systems])
+    check(L.mpet_assemble_primitives_batched(n, meshes, pats, kc, A, mu, outs,
+                                             _lib.stream_ptr()))
+    for s in systems:
+        s.finish_assembly()
+
+
 class DeviceSystem:
     """Assembled total-pressure system of one MPET instance on one GPU."""
 
     def __init__(self, space: MixedSpace, params, dt, theta, K_cells=None, device="cuda",
-                 build_csr=False, need_fp32=True, structured=True, robin=None):
+                 build_csr=False, need_fp32=True, structured=True, robin=None, assemble=True):
         if not torch.cuda.is_available():
             raise RuntimeError("DeviceSystem needs a GPU (no CPU fallback)")
         self.device = torch.device(device)
@@ -79,15 +142,34 @@ class DeviceSystem:
                                  int(self.pat_host["12"][1].size), int(self.pat_host["11"][1].size))
         self.nnz = {k: int(v[1].size) for k, v in self.pat_host.items()}
         self.dmask = torch.zeros(self.n, dtype=torch.uint8, device=dev)
-        self._assemble()
-        self._add_robin(robin or [])
+        self._alloc_primitives()
+        self._rest = (robin or [], structured, need_fp32, build_csr)
+        self.op64 = self.op32 = self.csr = None
+        self.assembled = False
+        if assemble:
+            self.assemble_primitives()
+            self.finish_assembly()
+
+    def assemble_primitives(self):
+        """(Re)assemble the FP64 primitives of this instance alone
+        (mpet_assemble_primitives), before any Robin term is added."""
+        check(_lib.lib().mpet_assemble_primitives(
+            C.byref(self.mesh_desc), C.byref(self.pat), ptr(self.Kcell), self.A, self.mu,
+            C.byref(self.prim), _lib.stream_ptr()))
+
+    def finish_assembly(self):
+        """After the primitives are assembled (here or by assemble_batch):
+        Robin terms, structured copies, scalar CSR."""
+        robin, structured, need_fp32, build_csr = self._rest
+        self._add_robin(robin)
         # structured operator copies (a batch packs its own from the primitives)
         self.op64 = self._structured(0) if structured else None
         self.op32 = self._structured(1) if structured and need_fp32 else None
         self.csr = self.expand_csr(0) if build_csr else None
+        self.assembled = True
 
     # -- assembly --------------------------------------------------------------
-    def _assemble(self):
+    def _alloc_primitives(self):
         dev = self.device
         f64 = torch.float64
         nz = self.nnz
@@ -103,9 +185,6 @@ class DeviceSystem:
         self.prim = _lib.Primitives(ptr(t["Kuu"]), ptr(t["Lap"]), ptr(t["Dv"]), ptr(t["Dt"]),
                                     ptr(t["M1"]), ptr(t["St"]))
         self.primK = self.prim
-        check(_lib.lib().mpet_assemble_primitives(
-            C.byref(self.mesh_desc), C.byref(self.pat), ptr(self.Kcell), self.A, self.mu,
-            C.byref(self.prim), _lib.stream_ptr()))
 
     def _add_robin(self, robin):
         """Robin terms of lhs(L2_i) (:324, added to A at :439-441): A gets
@@ -161,34 +240,31 @@ class DeviceSystem:
     cell_lanes = (8, 8)
     cell_xcd_chunk = 4
 
+    CELLS_PER_BLOCK = 256          # cells of one block of the cell kernel (CW_NT)
+
     def cell_op(self):
         """Arrays of the cell-wise FP32 operator, built once: per-cell
         geometry (FP32 gradients of the barycentric coordinates and |T|),
-        the cell -> node tables in structure-of-arrays layout, and for every
-        (cell, local node) its position in the list of all of them sorted by
-        node, then cell (the entries of one node contiguous)."""
+        the cell -> node tables in structure-of-arrays layout, and the
+        block partials: blocks of CELLS_PER_BLOCK consecutive cells write
+        one partial per (node, block), at its entry in the list of all
+        partials sorted by node then block, summing the node's local slots
+        of the block in slot order."""
         if getattr(self, "_cw", None) is None:
             dev = self.device
             nC = int(self.cells.shape[0])
+            B = self.CELLS_PER_BLOCK
             geom = torch.empty(10 * nC, dtype=torch.float32, device=dev)
             check(_lib.lib().mpet_cell_geometry(C.byref(self.mesh_desc), ptr(geom),
                                                 _lib.stream_ptr()))
-
-            def tables(cell_nodes, nnode):
-                k = cell_nodes.shape[1]
-                nodes = cell_nodes.reshape(-1).to(torch.int64)      # slot e*k + j
-                order = torch.argsort(nodes, stable=True)           # entries by node, cell
-                pos = torch.empty_like(order)
-                pos[order] = torch.arange(order.numel(), device=dev)
-                rp = torch.zeros(nnode + 1, dtype=torch.int64, device=dev)
-                rp[1:] = torch.cumsum(torch.bincount(nodes, minlength=nnode), 0)
-                soa = lambda t: t.reshape(nC, k).t().contiguous().to(torch.int32)
-                return soa(nodes), soa(pos), rp.to(torch.int32)
-            n2, p2, rp2 = tables(self.cell_p2, self.nP2)
-            n1, p1, rp1 = tables(self.cells, self.nP1)
-            self._cw = {"nC": nC, "geom": geom, "node2": n2, "pos2": p2, "rp2": rp2,
-                        "node1": n1, "pos1": p1, "rp1": rp1}
+            self._cw = {"nC": nC, "geom": geom,
+                        "p2": block_partial_tables(self.cell_p2, self.nP2, B),
+                        "p1": block_partial_tables(self.cells, self.nP1, B)}
         return self._cw
+
+    def cell_work_floats(self):
+        cw = self.cell_op()
+        return 3 * cw["p2"]["npart"] + cw["p1"]["npart"]
 
     def cell_op_desc(self, masked=True, lanes=None, xcd_chunk=None):
         """MpetCellOpDesc of the cell-wise FP32 operator (the P1 x P1 block
@@ -196,13 +272,16 @@ class DeviceSystem:
         if self.op32 is None:
             raise ValueError("the cell-wise operator needs the FP32 structured copy")
         cw = self.cell_op()
+        t2, t1 = cw["p2"], cw["p1"]
         v = self.op32
         lu, lp = lanes or self.cell_lanes
         xc = self.cell_xcd_chunk if xcd_chunk is None else xcd_chunk
         p11 = self.pat_dev["11"]
-        return _lib.CellOpDesc(cw["nC"], self.nP2, self.nP1, self.A, ptr(cw["node2"]),
-                               ptr(cw["node1"]), ptr(cw["pos2"]), ptr(cw["pos1"]),
-                               ptr(cw["geom"]), ptr(cw["rp2"]), ptr(cw["rp1"]),
+        return _lib.CellOpDesc(cw["nC"], self.nP2, self.nP1, self.A, ptr(t2["node"]),
+                               ptr(t1["node"]), ptr(cw["geom"]), self.CELLS_PER_BLOCK,
+                               t2["npart"], t1["npart"], ptr(t2["bq"]), ptr(t2["lrp"]),
+                               ptr(t2["gp"]), ptr(t2["lsl"]), ptr(t1["bq"]), ptr(t1["lrp"]),
+                               ptr(t1["gp"]), ptr(t1["lsl"]), ptr(t2["rp"]), ptr(t1["rp"]),
                                ptr(p11[0]), ptr(p11[1]), self.nnz["11"], ptr(v["M1"]),
                                ptr(v["St"]), _lib.make_coef(self.Cm, self.Cs, self.mu),
                                ptr(self.dmask) if masked else None, lu, lp, xc)
@@ -210,7 +289,7 @@ class DeviceSystem:
     def cell_apply(self, x32, b=None, masked=True, lanes=None):
         """y = A x (or b - A x) in FP32 through the cell-wise operator."""
         y = torch.empty_like(x32)
-        work = torch.empty(34 * self.cell_op()["nC"], dtype=torch.float32, device=self.device)
+        work = torch.empty(self.cell_work_floats(), dtype=torch.float32, device=self.device)
         d = self.cell_op_desc(masked, lanes)
         check(_lib.lib().mpet_cell_op_apply(C.byref(d), ptr(x32), ptr(b), ptr(y), ptr(work),
                                             _lib.stream_ptr()))

@@ -8,7 +8,11 @@ resident until the request completes.  At every scheduling tick it
 
   1. admits waiting requests in arrival order while their estimated HBM
      footprint fits the budget (a request larger than the whole budget is
-     refused with MPET_ERR_BUDGET);
+     refused with MPET_ERR_BUDGET).  An admitted request's set-up (mesh and
+     DOF maps, assembly, host AMG setup, uploads) runs on a pool of host
+     threads, each on a stream of its own, while the ticks go on; it joins
+     the batch at the first tick after its set-up finished (setup_workers =
+     0: set up synchronously inside the tick);
   2. re-packs the batch when requests were admitted (no assembly or AMG
      setup is repeated: each instance's device state is reused);
   3. runs one batched step of every unfinished instance (requests that
@@ -26,7 +30,9 @@ request's steps are that solver's steps.
 """
 from __future__ import annotations
 
+import concurrent.futures as cf
 import gc
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -36,6 +42,16 @@ import torch
 from .batch import MPETBatch, batch_engine
 
 ERR_BUDGET = 6
+
+
+class _null:
+    """A context manager that does nothing (synchronous set-up: the current
+    stream)."""
+    def __enter__(self):
+        return None
+
+    def __exit__(self, *a):
+        return False
 
 
 class BudgetError(RuntimeError):
@@ -60,6 +76,7 @@ class Request:
     result: np.ndarray = None
     niter: list = field(default_factory=list)
     refused: bool = False
+    _fp: object = None                 # footprint estimate running on a set-up thread
 
 
 def footprint_bytes(space, precision="mixed", robin=False):
@@ -88,7 +105,7 @@ class MPETServer:
     """Continuous-batching server over one GPU (see module docstring)."""
 
     def __init__(self, hbm_budget=200e9, precision="mixed", outer_tol=3e-11, inner_rtol=1e-6,
-                 rtol=1e-10, max_instances=512, clock=time.perf_counter):
+                 rtol=1e-10, max_instances=512, clock=time.perf_counter, setup_workers=2):
         if not torch.cuda.is_available():
             raise RuntimeError("MPETServer needs a GPU (no CPU fallback)")
         self.budget = float(hbm_budget)
@@ -97,7 +114,11 @@ class MPETServer:
         self.max_instances = max_instances
         self.clock = clock
         self.waiting = []          # arrival order
+        self.preparing = []        # (request, future): admitted, set-up running
         self.active = []
+        self.setup_workers = int(setup_workers)
+        self._pool = None
+        self._local = threading.local()
         self.finished = []
         self.used = 0
         self.batch = None
@@ -111,18 +132,83 @@ class MPETServer:
     def submit(self, req: Request):
         self.waiting.append(req)
 
+    def _on_stream(self, fn, req):
+        """fn(req) on a set-up thread's own stream, synchronised before the
+        instance is used by the batch (synchronous set-up: the current
+        stream)."""
+        st = None
+        if self.setup_workers > 0:
+            if not hasattr(self._local, "stream"):
+                self._local.stream = torch.cuda.Stream()
+            st = self._local.stream
+        with torch.cuda.stream(st) if st is not None else _null():
+            out = fn(req)
+        if st is not None:
+            st.synchronize()
+        return out
+
+    def _setup(self, req):
+        """The request's resident device state (assembly, AMG setup)."""
+        def work(r):
+            if r.solver is None:
+                r.solver = r.make_solver()
+            batch_engine(r.solver, self.precision)
+            return r
+        return self._on_stream(work, req)
+
+    def _footprint(self, req):
+        """The request's solver (host mesh and DOF maps) and its estimated
+        HBM footprint."""
+        def work(r):
+            if r.footprint == 0:
+                if r.solver is None:
+                    r.solver = r.make_solver()
+                prob = r.solver.problem
+                robin = any(not getattr(b, "is_zero", lambda: False)() for b in prob.beta)
+                r.footprint = footprint_bytes(r.solver.up_.function_space(), self.precision,
+                                              robin)
+            return r.footprint
+        return self._on_stream(work, req)
+
+    def _submit(self, fn, req):
+        if self._pool is None:
+            self._pool = cf.ThreadPoolExecutor(max_workers=self.setup_workers)
+        return self._pool.submit(fn, req)
+
+    def _footprint_ready(self, req, now):
+        """True when the head request's footprint is known; with set-up
+        threads, its estimate (and those of the next arrived requests) is
+        computed off the scheduling thread."""
+        if req.footprint:
+            return True
+        if self.setup_workers <= 0:
+            self._footprint(req)
+            return True
+        for r in self.waiting[:2 * self.setup_workers]:
+            if r.arrival <= now and r.footprint == 0 and r._fp is None:
+                r._fp = self._submit(self._footprint, r)
+        if req._fp is None or not req._fp.done():
+            return False
+        req._fp.result()
+        return True
+
     def _admit(self, now):
         changed = False
-        while self.waiting and len(self.active) < self.max_instances:
+        # set-ups that finished join the active set
+        still = []
+        for req, fut in self.preparing:
+            if fut.done():
+                fut.result()                   # re-raises a set-up failure
+                req.admitted = now
+                self.active.append(req)
+                changed = True
+            else:
+                still.append((req, fut))
+        self.preparing = still
+        while self.waiting and len(self.active) + len(self.preparing) < self.max_instances:
             req = self.waiting[0]
-            if req.arrival > now:
+            if req.arrival > now or not self._footprint_ready(req, now):
                 break
-            if req.footprint == 0:
-                s = req.make_solver()
-                req.solver = s
-                prob = s.problem
-                robin = any(not getattr(b, "is_zero", lambda: False)() for b in prob.beta)
-                req.footprint = footprint_bytes(s.up_.function_space(), self.precision, robin)
             if req.footprint > self.budget:
                 self.waiting.pop(0)
                 req.refused = True
@@ -135,14 +221,19 @@ class MPETServer:
             if self.used + req.footprint > self.budget:
                 break                      # wait until memory frees up
             self.waiting.pop(0)
-            if req.solver is None:
-                req.solver = req.make_solver()
-            batch_engine(req.solver, self.precision)     # resident device state
-            req.admitted = now
             self.used += req.footprint
-            self.active.append(req)
-            changed = True
+            if self.setup_workers > 0:
+                self.preparing.append((req, self._submit(self._setup, req)))
+            else:
+                self._setup(req)           # resident device state
+                req.admitted = now
+                self.active.append(req)
+                changed = True
         return changed
+
+    def busy(self):
+        """Requests waiting, being set up or running."""
+        return bool(self.waiting or self.preparing or self.active)
 
     def _drop_batch(self):
         """Close the batch and release the memory of retired instances."""
@@ -207,9 +298,13 @@ class MPETServer:
 
     def run(self, until_idle=True, idle_sleep=1e-3):
         """Tick until every submitted request has finished."""
-        while self.waiting or self.active:
+        while self.busy():
             if not self.tick():
-                if self.waiting and self.waiting[0].arrival > self.clock() - self.t0:
+                head = self.waiting[0] if self.waiting else None
+                if self.preparing or (head is not None and head._fp is not None
+                                      and not head._fp.done()):
+                    time.sleep(idle_sleep)         # a set-up or estimate is running
+                elif self.waiting and self.waiting[0].arrival > self.clock() - self.t0:
                     time.sleep(min(idle_sleep, self.waiting[0].arrival -
                                    (self.clock() - self.t0)))
                 elif self.waiting and not self.active:
@@ -217,7 +312,14 @@ class MPETServer:
                     # only possible when it exceeds the budget (refused above)
                     raise BudgetError("request cannot be admitted")
         self._drop_batch()
+        self.close()
         return self.finished
+
+    def close(self):
+        """Stop the set-up threads (after the running set-ups finish)."""
+        if self._pool is not None:
+            self._pool.shutdown(wait=True)
+            self._pool = None
 
     def latencies(self):
         lat = [r.done - r.arrival for r in self.finished if not r.refused]

@@ -178,11 +178,41 @@ def _is_zero(e):
     return bool(getattr(e, "is_zero", lambda: False)())
 
 
+def robin_facets(problem, space, device):
+    """Robin facets of every network (markers 2 of the continuity boundary
+    markers, :324) with a nonzero beta: [(i, FacetSet)] and the matching
+    left-hand-side entries [(i, FacetSet, beta, None)]."""
+    robin, lhs = [], []
+    for i in range(space.A):
+        if _is_zero(problem.beta[i]):
+            continue
+        fs = FacetSet(space, problem.continuity_boundary_markers[i], ROBIN_MARKER,
+                      torch.device(device))
+        if fs.n:
+            robin.append((i, fs))
+            lhs.append((i, fs, problem.beta[i], None))
+    return robin, lhs
+
+
+def batch_system(problem, space, dt, theta, device="cuda", assemble=True):
+    """The DeviceSystem a batched StepEngine of this instance uses (no
+    structured copies, no CSR); assemble=False defers the assembly to a
+    batched one (device.assemble_batch)."""
+    if not space.total_pressure:
+        raise NotImplementedError("batches of two-field instances")
+    params, cellK = resolved_params(problem)
+    return DeviceSystem(space, params, dt, theta, K_cells=cellK, device=device,
+                        build_csr=False, structured=False,
+                        robin=robin_facets(problem, space, device)[1], assemble=assemble)
+
+
 class StepEngine:
-    """Device-resident state of one MPET instance and its time step."""
+    """Device-resident state of one MPET instance and its time step
+    (system: an already built DeviceSystem of a batched engine, from
+    batch_system)."""
 
     def __init__(self, problem, space, dt, theta, config: SolverConfig, device="cuda",
-                 extrapolate=False, batched=False, extrapolate_order=4):
+                 extrapolate=False, batched=False, extrapolate_order=4, system=None):
         self.problem = problem
         self.batched = bool(batched)
         self.extrapolate = bool(extrapolate)
@@ -208,20 +238,15 @@ class StepEngine:
             raise NotImplementedError("batches of instances with nullspace multipliers")
         # exterior-facet terms (markers 1 = Neumann, 2 = Robin, :309-324)
         self.t0 = float(problem.time)
-        self.robin = []
-        robin_lhs = []
-        for i in range(A):
-            if _is_zero(problem.beta[i]):
-                continue
-            fs = FacetSet(space, problem.continuity_boundary_markers[i], ROBIN_MARKER,
-                          self.device)
-            if fs.n:
-                self.robin.append((i, fs))
-                robin_lhs.append((i, fs, problem.beta[i], None))
+        self.robin, robin_lhs = robin_facets(problem, space, self.device)
         # batched: the batch packs the operator and hierarchies of all its
         # instances, so no per-instance structured copies or solver handle
         _t0 = _time.perf_counter()
-        if space.total_pressure:
+        if system is not None:
+            if not batched:
+                raise ValueError("a prebuilt system is for batched engines")
+            self.system = system
+        elif space.total_pressure:
             self.system = DeviceSystem(space, params, dt, theta, K_cells=cellK, device=device,
                                        build_csr=(not batched and config.mode == "fp64"
                                                   and config.use_csr is not False),
