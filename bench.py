@@ -131,20 +131,23 @@ def csr_bytes(sys_):
 
 def _mat_bytes(nnz, rows, s, compact=False):
     """Values + column indices + row pointers of one CSR level matrix (the
-    compact FP32 P2 level: half values and 16-bit column offsets)."""
+    compact FP32 finest levels: half values and 16-bit column offsets)."""
     return nnz * (4 if compact else s + 4) + (rows + 1) * 4
 
 
-def vcycle_bytes(levels, amgp, NC, s, compact0=False, scaled=False):
+def vcycle_bytes(levels, amgp, NC, s, compact=(), scaled=False):
     """Algorithmic HBM bytes of one V-cycle (amg_cycle.h: vcycle / smooth /
     spmm modes), each array read once per launch: matrices, the gathered
-    vector once, the epilogue vectors; levels = the host hierarchy dicts."""
+    vector once, the epilogue vectors; levels = the host hierarchy dicts,
+    compact = the levels stored as half values with 16-bit offsets.  The
+    Chebyshev direction is not read by the first step from x and not written
+    by a sequence's last step."""
     nl = len(levels)
     tot = 0
 
-    def spmm(mode, nnz, rows, cols, compact=False):
-        mat = _mat_bytes(nnz, rows, s, compact) + NC * cols * s
-        epi = {0: 1, 1: 2, 2: 2, 3: 4, 4: 3}[mode] * NC * rows * s
+    def spmm(mode, nnz, rows, cols, cmp=False, dvec=2):
+        mat = _mat_bytes(nnz, rows, s, cmp) + NC * cols * s
+        epi = {0: 1, 1: 2, 2: 2, 3: 2 + dvec, 4: 3}[mode] * NC * rows * s
         if mode in (3, 4):
             epi += rows * s                      # D^-1
         return mat + epi
@@ -157,20 +160,35 @@ def vcycle_bytes(levels, amgp, NC, s, compact0=False, scaled=False):
             tot += n * n * s + 2 * NC * n * s      # dense coarsest solve
             return
         nnz = len(lev["A_ci"])
-        cmp = compact0 and l == 0
+        cmp = l in compact
         deg = amgp.degree(l)
         if not entered:                            # first Chebyshev step from 0
-            tot += (3 + (1 if (scaled and l == 0) else 0)) * NC * n * s + n * s
-        tot += (deg - 1) * spmm(3, nnz, n, n, cmp)
+            tot += ((3 if deg > 1 else 2) + (1 if (scaled and l == 0) else 0)) * NC * n * s \
+                + n * s
+        if deg > 1:                                # the last one writes no direction
+            tot += (deg - 2) * spmm(3, nnz, n, n, cmp) + spmm(3, nnz, n, n, cmp, 1)
         tot += spmm(1, nnz, n, n, cmp)             # residual
         nc = lev["nc"]
         nnzP = len(lev["P_ci"])
         tot += spmm(4 if l + 1 < nl - 1 else 0, nnzP, nc, n)   # restriction
         level(l + 1, l + 1 < nl - 1)
         tot += spmm(2, nnzP, n, nc)                # prolongation
-        tot += deg * spmm(3, nnz, n, n, cmp)       # post-smoothing from x
+        # post-smoothing from x: the first step reads no direction
+        if deg == 1:
+            tot += spmm(3, nnz, n, n, cmp, 0)
+        else:
+            tot += (spmm(3, nnz, n, n, cmp, 1) + (deg - 2) * spmm(3, nnz, n, n, cmp)
+                    + spmm(3, nnz, n, n, cmp, 1))
     level(0, False)
     return tot
+
+
+def _compact_u(kr, mixed):
+    return (0,) if mixed and getattr(kr, "compact", None) is not None else ()
+
+
+def _compact_p(kr, mixed):
+    return tuple(getattr(kr, "pcompact_levels", ())) if mixed else ()
 
 
 def iteration_bytes(eng, mixed):
@@ -189,9 +207,9 @@ def iteration_bytes(eng, mixed):
         k1 = op_bytes(sysm, 1 if mixed else 0) + 2 * n * s
     else:
         k1 = csr_bytes(sysm) + 2 * n * s
-    vu = vcycle_bytes(kr.levels, kr.cfg.amg, 3, s, compact0=mixed and hasattr(kr, "compact"),
-                      scaled=True)
-    vp = vcycle_bytes(kr.plevels, kr.cfg.pamg, 1, s) + 4 * sysm.F * sysm.nP1 * s
+    vu = vcycle_bytes(kr.levels, kr.cfg.amg, 3, s, compact=_compact_u(kr, mixed), scaled=True)
+    vp = vcycle_bytes(kr.plevels, kr.cfg.pamg, 1, s,
+                      compact=_compact_p(kr, mixed)) + 4 * sysm.F * sysm.nP1 * s
     blas = 11 * n * s
     parts = {"k1": int(k1), "vcycle_u": int(vu), "vcycle_p": int(vp), "blas1": int(blas)}
     return int(k1 + vu + vp + blas), parts
@@ -208,8 +226,9 @@ def step_bytes(eng, mixed, inner, outer):
     n = sysm.n
     kr = eng.krylov
     op64 = op_bytes(sysm, 0) if (mixed or kr.csr_desc is None) else csr_bytes(sysm)
-    pc = vcycle_bytes(kr.levels, kr.cfg.amg, 3, s, compact0=mixed and hasattr(kr, "compact"),
-                      scaled=True) + vcycle_bytes(kr.plevels, kr.cfg.pamg, 1, s)
+    pc = vcycle_bytes(kr.levels, kr.cfg.amg, 3, s, compact=_compact_u(kr, mixed),
+                      scaled=True) + vcycle_bytes(kr.plevels, kr.cfg.pamg, 1, s,
+                                                  compact=_compact_p(kr, mixed))
     starts = 2 + (outer if mixed else 0)
     solve_overhead = starts * (op64 + n * 8 + pc + 2 * n * s)
     nz11 = sysm.nnz["11"]
@@ -912,6 +931,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=8)
     ap.add_argument("--cpu-iters", type=int, default=4)
+    # the whole configuration from t = 0 (80 steps of dt = 0.0125) on the
+    # mixed path after the timed window, reported beside it (0: off)
+    ap.add_argument("--full-steps", type=int, default=80)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-reference", action="store_true")
     ap.add_argument("--tf-n", type=int, default=32)
@@ -1036,6 +1058,19 @@ def main():
         ms, abytes = ms_s, bytes_s
         rkernel = f"op_apply_kernel<float,{sysm.F},16,16> (structured A, FP32)"
     achieved = abytes / (ms * 1e-3) / 1e9
+    full = None
+    if args.full_steps > 0:
+        # the full run: zero initial state at t = 0, T = full_steps * dt
+        mixed.up_.data.zero_()
+        mixed.problem.time.assign(0.0)
+        mixed.params["T"] = args.full_steps * 0.0125
+        fres = run_steps(mixed, 0, args.full_steps)
+        full = {"steps": args.full_steps, "seconds": round(fres[0], 3),
+                "steps_per_s": round(args.full_steps / fres[0], 4),
+                "mean_inner_iterations": round(float(np.mean(fres[3])), 1),
+                "max_outer_iterations": int(max(fres[4]))}
+        print(f"[bench] mixed full run {args.full_steps} steps {fres[0]:.2f}s", file=sys.stderr,
+              flush=True)
     mix_sps = K / mix_res[0]
     lat = np.array(mix_res[1]) * 1e3
 
@@ -1076,6 +1111,7 @@ def main():
         "outer_iterations": mix_res[4],
         "max_rel_l2_err_vs_fp64_reference": max(errs) if errs else None,
         "hbm": bytes_report(eng, mix_res, True),
+        "full_run": full,
         "autotune": {"picks": {str(k): v for k, v in eng.krylov.tuning.items()},
                      "legend": "operator: [lanes per P2 row, lanes per P1 row, xcd_chunk]; "
                                "levels: [A, P, R] lanes per row per level (0: unused)"},

@@ -360,7 +360,9 @@ int mpet_op_apply(const MpetOpDesc* op, const void* x, const void* b, void* y,
  * fixed: results are reproducible.  Cells are taken in blocks of
  * cells_per_block (= 256) consecutive cells; a block sums, per node its
  * cells touch, their contributions in slot order, and writes one partial
- * per (node, block); a row sums its node's partials in block order.
+ * per (node, block), the block's partials contiguous (block-major); a row
+ * sums its node's partials in block order, reached through the node-sorted
+ * entry list.
  * Cell tables are structure-of-arrays, [k*nC + e]:
  *   node2 / node1  P2 node of local node k (UFC order) / P1 node of vertex k
  *   geom           10*nC floats from mpet_cell_geometry
@@ -368,7 +370,7 @@ int mpet_op_apply(const MpetOpDesc* op, const void* x, const void* b, void* y,
  * its block) and P1 nodes (suffix 1, local slot c*4 + v):
  *   bq[b] .. bq[b+1]-1   the partials of block b (nblocks + 1 entries)
  *   lrp[q] .. lrp[q+1]-1 partial q's local slots in lsl (ascending)
- *   gp[q]                partial q's entry in the node-sorted list
+ *   pe[k]                the partial at entry k of the node-sorted list
  *   inc2_rp / inc1_rp    node -> first entry of its partials (n + 1)
  * Work: 3*npart2 + npart1 floats.  lanes_u in {4, 8, 16}, lanes_p in
  * {8, 16} (others: 8).
@@ -378,9 +380,9 @@ typedef struct {
   const int32_t *node2, *node1;  /* 10*nC, 4*nC */
   const float* geom;
   int32_t cells_per_block, npart2, npart1;
-  const int32_t *bq2, *lrp2, *gp2;
+  const int32_t *bq2, *lrp2, *pe2;
   const int16_t* lsl2;
-  const int32_t *bq1, *lrp1, *gp1;
+  const int32_t *bq1, *lrp1, *pe1;
   const int16_t* lsl1;
   const int32_t *inc2_rp, *inc1_rp;   /* nP2 + 1, nP1 + 1 */
   const int32_t *rp11, *ci11;

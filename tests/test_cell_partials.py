@@ -1,8 +1,9 @@
 """CPU checks of the block-partial tables of the cell-wise operator
 (waterscapes_amd.device.block_partial_tables), emulating what the kernels
-do with them: the cell kernel sums, per partial of its block, the local
-slots lsl[lrp[q] .. lrp[q+1]) and stores the sum at entry gp[q]; the row
-kernel sums a node's entries rp[a] .. rp[a+1].  The result must equal a
+do with them: the cell kernel sums, per partial q of its block, the local
+slots lsl[lrp[q] .. lrp[q+1]) and stores the sum at q (block-major); the
+row kernel sums the partials pe[e] of a node's entries e in rp[a] ..
+rp[a+1].  The result must equal a
 direct scatter-add of every (cell, local node) value to its node, every
 index must stay in range, and each node's entries must come from distinct
 blocks in increasing order."""
@@ -17,12 +18,13 @@ def _emulate(cell_nodes, nnode, B, vals):
     t = block_partial_tables(torch.as_tensor(cell_nodes), nnode, B)
     nC, k = cell_nodes.shape
     nb = (nC + B - 1) // B
-    bq, lrp, gp = t["bq"].numpy(), t["lrp"].numpy(), t["gp"].numpy()
+    bq, lrp, gp, pe = t["bq"].numpy(), t["lrp"].numpy(), t["gp"].numpy(), t["pe"].numpy()
     lsl, rp = t["lsl"].numpy().astype(np.int64), t["rp"].numpy()
     npart = t["npart"]
     assert bq[0] == 0 and bq[-1] == npart and np.all(np.diff(bq) >= 0) and bq.size == nb + 1
     assert lrp[0] == 0 and lrp[-1] == nC * k and np.all(np.diff(lrp) >= 1)
     assert np.array_equal(np.sort(gp), np.arange(npart))
+    assert np.array_equal(pe[gp], np.arange(npart))          # inverse permutations
     assert rp[0] == 0 and rp[-1] == npart and rp.size == nnode + 1
     assert np.array_equal(t["node"].numpy().T, cell_nodes)
     part = np.zeros(npart)
@@ -34,8 +36,8 @@ def _emulate(cell_nodes, nnode, B, vals):
             assert np.all(np.diff(sl) > 0)                      # slot order
             nodes = cell_nodes.reshape(-1)[b * B * k + sl]
             assert np.all(nodes == nodes[0])                    # one node per partial
-            part[gp[q]] = vals.reshape(-1)[b * B * k + sl].sum()
-    out = np.array([part[rp[a]:rp[a + 1]].sum() for a in range(nnode)])
+            part[q] = vals.reshape(-1)[b * B * k + sl].sum()
+    out = np.array([part[pe[rp[a]:rp[a + 1]]].sum() for a in range(nnode)])
     return out, t
 
 

@@ -55,9 +55,9 @@ class CellOpDesc(C.Structure):
     _fields_ = [("nC", C.c_int32), ("nP2", C.c_int32), ("nP1", C.c_int32), ("A", C.c_int32),
                 ("node2", C.c_void_p), ("node1", C.c_void_p), ("geom", C.c_void_p),
                 ("cells_per_block", C.c_int32), ("npart2", C.c_int32), ("npart1", C.c_int32),
-                ("bq2", C.c_void_p), ("lrp2", C.c_void_p), ("gp2", C.c_void_p),
+                ("bq2", C.c_void_p), ("lrp2", C.c_void_p), ("pe2", C.c_void_p),
                 ("lsl2", C.c_void_p), ("bq1", C.c_void_p), ("lrp1", C.c_void_p),
-                ("gp1", C.c_void_p), ("lsl1", C.c_void_p), ("inc2_rp", C.c_void_p),
+                ("pe1", C.c_void_p), ("lsl1", C.c_void_p), ("inc2_rp", C.c_void_p),
                 ("inc1_rp", C.c_void_p), ("rp11", C.c_void_p), ("ci11", C.c_void_p),
                 ("nnz11", C.c_int32), ("M1", C.c_void_p), ("St", C.c_void_p), ("coef", Coef),
                 ("dmask", C.c_void_p), ("lanes_u", C.c_int32), ("lanes_p", C.c_int32),

@@ -247,7 +247,7 @@ def tune_hierarchy(levels_host, levels_dev, nc, dtype, heuristic, mask0=None, ca
         for j, (which, rp, ci, v, rows, cols) in enumerate(mats):
             if l == len(levels_host) - 1 and which == "A":
                 continue          # the coarsest level is a dense solve
-            compact = which == "A" and l == 0 and "A_v16" in ld
+            compact = which == "A" and "A_v16" in ld
             nnz = int(ci.numel())
             key = level_key(dtype, nc, which, rows, cols, nnz, compact)
             hit = cache.get(key)

@@ -84,7 +84,11 @@ static int grid_for(long long items, int per_block, int cap) {
 // pressure blocks).  LPR lanes per row; after an xor all-reduce, lane c of
 // the row group finishes component c.
 // MODE 0: y = A x        MODE 1: y = b - A x       MODE 2: y += A x
-// MODE 3: Chebyshev step d = cd d + cr Dinv (b - A x); y = x + d
+// MODE 3: Chebyshev step d' = cd d + cr Dinv (b - A x); y = x + d'; d is
+//         read only when cd != 0 (not by the first step from x), d' is
+//         written to y2 when y2 is given (not by the last step of a
+//         smoothing sequence; y2 may be d: each entry is read, then written,
+//         by one thread)
 // MODE 4: restriction entering the coarse level's smoother: y = A x, then
 //         its first Chebyshev step from zero d = y2 = cr Dinv y (dinv and
 //         cr of the coarse level)
@@ -93,9 +97,9 @@ template <typename T, int NC, int LPR, int MODE, bool COMPACT = false>
 __global__ void __launch_bounds__(NT)
 spmm_kernel(int nrows, const int* __restrict__ rp, const int* __restrict__ ci,
             const T* __restrict__ val, const T* __restrict__ x, const T* __restrict__ b,
-            T* __restrict__ y, const T* __restrict__ dinv, T* __restrict__ d, T cd, T cr,
+            T* __restrict__ y, const T* __restrict__ dinv, T* d, T cd, T cr,
             const uint8_t* __restrict__ mask, Skip sk, const int* __restrict__ grp,
-            T* __restrict__ y2, const __half* __restrict__ val16 = nullptr,
+            T* y2, const __half* __restrict__ val16 = nullptr,
             const int16_t* __restrict__ dci = nullptr, T vscale = (T)1) {
   // COMPACT: values stored as half precision times 1/vscale, columns as
   // signed 16-bit offsets from the row (half the bytes per nonzero)
@@ -126,7 +130,7 @@ spmm_kernel(int nrows, const int* __restrict__ rp, const int* __restrict__ ci,
   if constexpr (MODE == 2) px = y[o];
   if constexpr (MODE == 3) {
     px = x[o];
-    pd = d[o];
+    if (cd != (T)0) pd = d[o];
     pdinv = dinv[r];
   }
   if constexpr (MODE == 4) pdinv = dinv[r];
@@ -180,7 +184,7 @@ spmm_kernel(int nrows, const int* __restrict__ rp, const int* __restrict__ ci,
     y2[o] = v;
   } else {
     T dd = m ? (T)0 : cd * pd + cr * pdinv * (pb - mine);
-    d[o] = dd;
+    if (y2) y2[o] = dd;
     y[o] = m ? (T)0 : px + dd;
   }
 }
@@ -192,9 +196,9 @@ template <typename T, int NC, int MODE>
 __global__ void __launch_bounds__(NT)
 spmm_row_kernel(int nrows, const int* __restrict__ rp, const int* __restrict__ ci,
                 const T* __restrict__ val, const T* __restrict__ x, const T* __restrict__ b,
-                T* __restrict__ y, const T* __restrict__ dinv, T* __restrict__ d, T cd, T cr,
+                T* __restrict__ y, const T* __restrict__ dinv, T* d, T cd, T cr,
                 const uint8_t* __restrict__ mask, Skip sk, const int* __restrict__ grp,
-                T* __restrict__ y2) {
+                T* y2) {
   if (sk.all && *sk.all) return;
   const int r = blockIdx.x * NT + threadIdx.x;
   if (r >= nrows || row_skipped(sk, grp, r)) return;
@@ -240,29 +244,51 @@ spmm_row_kernel(int nrows, const int* __restrict__ rp, const int* __restrict__ c
       d[o] = v;
       y2[o] = v;
     } else {
-      const T dd = m ? (T)0 : cd * d[o] + cr * di * (b[o] - mine);
-      d[o] = dd;
+      const T dp = cd != (T)0 ? cd * d[o] : (T)0;
+      const T dd = m ? (T)0 : dp + cr * di * (b[o] - mine);
+      if (y2) y2[o] = dd;
       y[o] = m ? (T)0 : x[o] + dd;
     }
   }
 }
 
-// First Chebyshev step from x0 = 0: d = y = cr * Dinv * (scale * b); the
-// scaled right-hand side is kept in bcopy when given.
+// A MINRES vector update fused into the first kernels of a preconditioner
+// apply that read its input (off when y is null): the input entry i is
+// r[i] = y[i] - (alfa / beta) r2o[i], computed where it is first read and
+// stored to r[i] for the later readers.
+template <typename T>
+struct K2In {
+  const T* y = nullptr;
+  const T* r2o = nullptr;
+  T* r = nullptr;
+  const double* alfa = nullptr;
+  const double* beta = nullptr;
+  __device__ __forceinline__ T make(long long i) const {
+    const T c = (T)(*alfa / *beta);
+    const T v = y[i] - c * r2o[i];
+    r[i] = v;
+    return v;
+  }
+};
+
+// First Chebyshev step from x0 = 0: d = y = cr * Dinv * (scale * b) (d
+// only when given); the scaled right-hand side is kept in bcopy when given;
+// with k2 on, b is made by k2 (and b itself is not read).
 template <typename T, int NC>
 __global__ void __launch_bounds__(NT)
-cheb0_kernel(int nrows, const T* __restrict__ b, T scale, const T* __restrict__ dinv, T cr,
+cheb0_kernel(int nrows, const T* b, T scale, const T* __restrict__ dinv, T cr,
              T* __restrict__ d, T* __restrict__ y, T* __restrict__ bcopy,
-             const uint8_t* __restrict__ mask, Skip sk, const int* __restrict__ grp) {
+             const uint8_t* __restrict__ mask, Skip sk, const int* __restrict__ grp,
+             K2In<T> k2 = K2In<T>()) {
   if (sk.all && *sk.all) return;
   long long i = (long long)blockIdx.x * NT + threadIdx.x;
   if (i >= (long long)NC * nrows) return;
   int r = (int)(i / NC);
   if (row_skipped(sk, grp, r)) return;
-  T bv = scale * b[i];
+  T bv = scale * (k2.y ? k2.make(i) : b[i]);
   if (bcopy) bcopy[i] = bv;
   T v = (mask && mask[r]) ? (T)0 : cr * dinv[r] * bv;
-  d[i] = v;
+  if (d) d[i] = v;
   y[i] = v;
 }
 
@@ -304,13 +330,13 @@ coarse_kernel(int n, const T* __restrict__ cinv, const T* __restrict__ b, T* __r
 // Pressure vectors: node-interleaved [c][F] <-> field-major [f][c].
 template <typename T>
 __global__ void __launch_bounds__(NT)
-p_gather_kernel(int nP1, int F, const T* __restrict__ src, T* __restrict__ dst,
-                const int* __restrict__ skip) {
+p_gather_kernel(int nP1, int F, const T* src, T* __restrict__ dst,
+                const int* __restrict__ skip, K2In<T> k2 = K2In<T>()) {
   if (skip && *skip) return;
   long long i = (long long)blockIdx.x * NT + threadIdx.x;
   if (i >= (long long)F * nP1) return;
   int c = (int)(i / F), f = (int)(i % F);
-  dst[(long long)f * nP1 + c] = src[i];
+  dst[(long long)f * nP1 + c] = k2.y ? k2.make(i) : src[i];
 }
 template <typename T>
 __global__ void __launch_bounds__(NT)
@@ -540,7 +566,8 @@ static T* first_dst(const LevelDev<T>& L, T* out) {
 // in first_dst, made by the restriction) or from x0; result in out.
 template <typename T, int NC>
 static void smooth(const Hier<T>& H, int l, const T* b, T bscale, T* bcopy, const T* x0,
-                   T* out, Skip sk, hipStream_t s, bool entered = false) {
+                   T* out, Skip sk, hipStream_t s, bool entered = false,
+                   K2In<T> k2 = K2In<T>()) {
   const LevelDev<T>& L = H.lev[l];
   const int k = L.deg;
   const T* cur = x0;
@@ -553,7 +580,8 @@ static void smooth(const Hier<T>& H, int l, const T* b, T bscale, T* bcopy, cons
     int blocks = (int)(((long long)NC * L.n + NT - 1) / NT);
     if (!entered)
       hipLaunchKernelGGL((cheb0_kernel<T, NC>), dim3(blocks), dim3(NT), 0, s, L.n, b, bscale,
-                         L.dinv, (T)L.cr[0], L.D, dst, bcopy, L.mask, sk, L.grp);
+                         L.dinv, (T)L.cr[0], k == 1 ? nullptr : L.D, dst, bcopy, L.mask, sk,
+                         L.grp, k2);
     if (bcopy) bb = bcopy;
     cur = dst;
     i = 1;
@@ -562,7 +590,9 @@ static void smooth(const Hier<T>& H, int l, const T* b, T bscale, T* bcopy, cons
   for (; i < k; ++i) {
     T* dst = (i == k - 1) ? out : bufs[flip];
     if (dst == cur) dst = bufs[flip ^ 1];
-    spmm<T, NC, 3>(L, 0, cur, bb, dst, L.D, (T)L.cd[i], (T)L.cr[i], sk, s);
+    // the last step's direction is not needed again
+    spmm<T, NC, 3>(L, 0, cur, bb, dst, L.D, (T)L.cd[i], (T)L.cr[i], sk, s, nullptr,
+                   i == k - 1 ? nullptr : L.D);
     cur = dst;
     flip ^= 1;
   }
@@ -570,7 +600,8 @@ static void smooth(const Hier<T>& H, int l, const T* b, T bscale, T* bcopy, cons
 
 template <typename T, int NC>
 static void vcycle(const Hier<T>& H, int l, const T* b, T bscale, T* out, Skip sk,
-                   hipStream_t s, bool entered = false) {
+                   hipStream_t s, bool entered = false, K2In<T> k2 = K2In<T>()) {
+  // k2 (a fused input update) only on a level with a smoother
   const LevelDev<T>& L = H.lev[l];
   if (l == H.nlev - 1) {
     const T* bb = b;
@@ -590,7 +621,7 @@ static void vcycle(const Hier<T>& H, int l, const T* b, T bscale, T* out, Skip s
     return;
   }
   T* bcopy = (bscale != (T)1) ? L.B : nullptr;
-  smooth<T, NC>(H, l, b, bscale, bcopy, nullptr, L.X, sk, s, entered);
+  smooth<T, NC>(H, l, b, bscale, bcopy, nullptr, L.X, sk, s, entered, k2);
   const T* bb = bcopy ? bcopy : b;
   spmm<T, NC, 1>(L, 0, L.X, bb, L.R, nullptr, 0, 0, sk, s);
   const LevelDev<T>& C = H.lev[l + 1];
@@ -608,8 +639,18 @@ static void vcycle(const Hier<T>& H, int l, const T* b, T bscale, T* out, Skip s
   smooth<T, NC>(H, l, bb, (T)1, nullptr, L.X, out, sk, s);
 }
 
+// Whether precond_apply can take a fused input update (k2): the first
+// kernels reading the input are the displacement level-0 smoother and the
+// pressure gather.
 template <typename T>
-static void precond_apply(const Precond<T>& P, const T* r, T* z, Skip sk, hipStream_t s) {
+static bool k2_fusable(const Precond<T>& P) {
+  return P.u.nlev > 1 && !P.pmap;
+}
+
+// z = M^-1 r; with k2 (k2_fusable only) r is first made as k2 describes.
+template <typename T>
+static void precond_apply(const Precond<T>& P, const T* r, T* z, Skip sk, hipStream_t s,
+                          const K2In<T>* k2 = nullptr) {
   const int* skip = sk.all;
   hipStream_t sp = s;
   if (P.ps) {                      // pressure blocks on the second stream
@@ -617,7 +658,12 @@ static void precond_apply(const Precond<T>& P, const T* r, T* z, Skip sk, hipStr
     (void)hipStreamWaitEvent(P.ps, P.ev_fork, 0);
     sp = P.ps;
   }
-  vcycle<T, 3>(P.u, 0, r, (T)(1.0 / P.u.scale), z, sk, s);
+  K2In<T> ku, kp;
+  if (k2 && k2->y) {
+    ku = *k2;
+    kp = K2In<T>{k2->y + P.nu, k2->r2o + P.nu, k2->r + P.nu, k2->alfa, k2->beta};
+  }
+  vcycle<T, 3>(P.u, 0, r, (T)(1.0 / P.u.scale), z, sk, s, false, ku);
   long long np = (long long)P.F * P.nP1;
   int eb = (int)((np + NT - 1) / NT);
   if (P.pmap)
@@ -625,7 +671,7 @@ static void precond_apply(const Precond<T>& P, const T* r, T* z, Skip sk, hipStr
                        P.pin, skip);
   else
     hipLaunchKernelGGL((p_gather_kernel<T>), dim3(eb), dim3(NT), 0, sp, P.nP1, P.F, r + P.nu,
-                       P.pin, skip);
+                       P.pin, skip, kp);
   vcycle<T, 1>(P.p, 0, P.pin, (T)1, P.pout, sk, sp);
   if (P.pmap)
     hipLaunchKernelGGL((p_scatter_map_kernel<T>), dim3(eb), dim3(NT), 0, sp, np, P.pmap, P.pout,

@@ -26,13 +26,13 @@
 // Two launches.  cw_cell_kernel (one thread per cell, CW_NT cells per
 // block) computes each cell's 30 + 4 local results into LDS, then sums, per
 // node touched by the block's cells, that node's contributions from these
-// cells (in slot order) and writes one partial per (node, block) to its
-// place in node-sorted partial lists (a node's partials contiguous, in
-// block order: ~3x fewer stores than one per (cell, node)).  cw_rows_kernel
-// gives every row the sum of its contiguous run of partials (fixed order, so
-// results are reproducible) plus the assembled P1 x P1 part, and hands the
-// row value to an epilogue (plain apply / masked residual, or the fused
-// MINRES step).  Cell tables are structure-of-arrays ([k * nC + e]).
+// cells (in slot order) and writes one partial per (node, block), a block's
+// partials stored contiguously (~3x fewer stores than one per (cell,
+// node)).  cw_rows_kernel gives every row the sum of its node's partials,
+// listed contiguously in block order by a node-sorted entry list (fixed
+// order, so results are reproducible), plus the assembled P1 x P1 part, and
+// hands the row value to an epilogue (plain apply / masked residual, or the
+// fused MINRES step).  Cell tables are structure-of-arrays ([k * nC + e]).
 #pragma once
 #include "amg_cycle.h"
 
@@ -43,12 +43,13 @@ struct CellOp {
   int nC, nP2, nP1, A;
   const int* node2;       // [k * nC + e]: P2 node of local node k (UFC order)
   const int* node1;       // [v * nC + e]: P1 node of vertex v
-  // partials of block b: q in [bq[b], bq[b+1]); partial q sums the local
-  // slots lsl[lrp[q] .. lrp[q+1]) (c * 10 + k for P2, c * 4 + v for P1, c the
-  // cell within the block) and goes to entry gp[q] of the node-sorted list
-  const int *bq2, *lrp2, *gp2;
+  // partials of block b: q in [bq[b], bq[b+1]) (block-major, so a block's
+  // stores are contiguous); partial q sums the local slots
+  // lsl[lrp[q] .. lrp[q+1]) (c * 10 + k for P2, c * 4 + v for P1, c the
+  // cell within the block); entry k of the node-sorted list is partial pe[k]
+  const int *bq2, *lrp2, *pe2;
   const int16_t* lsl2;
-  const int *bq1, *lrp1, *gp1;
+  const int *bq1, *lrp1, *pe1;
   const int16_t* lsl1;
   const float* geom;      // SoA: [d * nC + e] = G_{1 + d/3}[d % 3] (d < 9); |T| at d = 9
   const int *inc2_rp, *inc1_rp;     // node -> first entry (nP2 + 1, nP1 + 1)
@@ -61,8 +62,8 @@ struct CellOp {
   const uint8_t* dmask;
   int lanes_u, lanes_p, xcd_chunk;
   int npart2, npart1;     // partials of all blocks (P2, P1)
-  float* Yu;              // 3 x npart2, node-sorted
-  float* Yp;              // npart1, node-sorted
+  float* Yu;              // 3 x npart2, block-major
+  float* Yp;              // npart1, block-major
 };
 
 #ifndef MPET_CW_CELLS
@@ -187,7 +188,9 @@ cw_cell_kernel(CellOp op, const float* __restrict__ x, const int* __restrict__ s
   }
   __syncthreads();
   // one partial per (node, block): the node's slots of this block, summed in
-  // slot order
+  // slot order, stored at the block's contiguous run of partials (scattered
+  // 12-byte stores to node-sorted lists measured 3.7x their bytes in memory
+  // writes: lines shared by blocks on different XCDs)
   const int b = blockIdx.x;
   float* __restrict__ Yu = op.Yu;
   for (int q = op.bq2[b] + t; q < op.bq2[b + 1]; q += CW_NT) {
@@ -199,13 +202,13 @@ cw_cell_kernel(CellOp op, const float* __restrict__ x, const int* __restrict__ s
       a[1] += su[3 * sl + 1];
       a[2] += su[3 * sl + 2];
     }
-    __builtin_memcpy(Yu + 3ll * op.gp2[q], a, 12);
+    __builtin_memcpy(Yu + 3ll * q, a, 12);
   }
   for (int q = op.bq1[b] + t; q < op.bq1[b + 1]; q += CW_NT) {
     const int lb = op.lrp1[q], le = op.lrp1[q + 1];
     float a = 0.f;
     for (int i = lb; i < le; ++i) a += sp[op.lsl1[i]];
-    op.Yp[op.gp1[q]] = a;
+    op.Yp[q] = a;
   }
 }
 
@@ -248,12 +251,12 @@ cw_rows_kernel(CellOp op, const float* __restrict__ x, Epi epi, double* __restri
         const int kb = op.inc2_rp[a], ke = op.inc2_rp[a + 1];
         float y0 = 0.f, y1 = 0.f, y2 = 0.f;
         // the node's entries are contiguous: two per lane per trip, loads
-        // issued before use
+        // (entry -> partial -> value) issued before use
         for (int k = kb + lane; k < ke; k += 2 * GU) {
           const int k2 = k + GU < ke ? k + GU : k;
           float av[3], bv[3];
-          load_nc<float, 3>(Yu + 3ll * k, av);
-          load_nc<float, 3>(Yu + 3ll * k2, bv);
+          load_nc<float, 3>(Yu + 3ll * op.pe2[k], av);
+          load_nc<float, 3>(Yu + 3ll * op.pe2[k2], bv);
           if (!(k + GU < ke)) bv[0] = bv[1] = bv[2] = 0.f;
           y0 += av[0];
           y1 += av[1];
@@ -279,7 +282,7 @@ cw_rows_kernel(CellOp op, const float* __restrict__ x, Epi epi, double* __restri
       if (c < op.nP1) {
         float yd = 0.f;
         const int kb = op.inc1_rp[c], ke = op.inc1_rp[c + 1];
-        for (int k = kb + lane; k < ke; k += GP) yd += Yp[k];
+        for (int k = kb + lane; k < ke; k += GP) yd += Yp[op.pe1[k]];
         float m[F], sv[F];
 #pragma unroll
         for (int g = 0; g < F; ++g) { m[g] = 0.f; sv[g] = 0.f; }
@@ -388,8 +391,8 @@ static inline long long cw_work_floats(const MpetCellOpDesc* d) {
 static int cw_from_desc(const MpetCellOpDesc* d, float* work, CellOp& op) {
   if (!d || d->nC <= 0 || d->nP2 <= 0 || d->nP1 <= 0 || d->A < 1 || d->A + 1 > MPET_MAXF ||
       d->nnz11 <= 0 || !d->node2 || !d->node1 || !d->geom || d->cells_per_block != CW_NT ||
-      d->npart2 <= 0 || d->npart1 <= 0 || !d->bq2 || !d->lrp2 || !d->gp2 || !d->lsl2 ||
-      !d->bq1 || !d->lrp1 || !d->gp1 || !d->lsl1 ||
+      d->npart2 <= 0 || d->npart1 <= 0 || !d->bq2 || !d->lrp2 || !d->pe2 || !d->lsl2 ||
+      !d->bq1 || !d->lrp1 || !d->pe1 || !d->lsl1 ||
       !d->inc2_rp || !d->inc1_rp || !d->rp11 || !d->ci11 || !d->M1 ||
       !d->St || !work) {
     mpet_set_error("invalid cell-wise operator description");
@@ -403,11 +406,11 @@ static int cw_from_desc(const MpetCellOpDesc* d, float* work, CellOp& op) {
   op.node1 = d->node1;
   op.bq2 = d->bq2;
   op.lrp2 = d->lrp2;
-  op.gp2 = d->gp2;
+  op.pe2 = d->pe2;
   op.lsl2 = d->lsl2;
   op.bq1 = d->bq1;
   op.lrp1 = d->lrp1;
-  op.gp1 = d->gp1;
+  op.pe1 = d->pe1;
   op.lsl1 = d->lsl1;
   op.npart2 = d->npart2;
   op.npart1 = d->npart1;

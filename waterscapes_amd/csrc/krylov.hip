@@ -550,6 +550,17 @@ static int bgrid(long long n, int m) { return grid_for(n, NT * 4, MAX_PARTIALS /
 
 // z = M^-1 r.  Mode 2 applies the FP32 preconditioner to FP64 vectors
 // (convert, V-cycles in FP32, convert back).
+// Whether apply_M is the plain preconditioner apply and takes the K2
+// update fused (not for the FP32 preconditioner of the FP64 solve or the
+// bordered systems).
+template <typename T>
+static bool k2_direct(const MpetSolver* S, const Precond<T>& P) {
+  if constexpr (sizeof(T) == 8) {
+    if (S->opt.mode == 2 || S->mb) return false;
+  }
+  return k2_fusable(P);
+}
+
 template <typename T>
 static void apply_M(MpetSolver* S, const Precond<T>& P, const T* r, T* z, const int* skip,
                     hipStream_t s) {
@@ -704,8 +715,17 @@ static void minres_iter(MpetSolver* S, Minres<T>& M, const Precond<T>& P, bool c
   }
   if (!(csr && S->mb))
     hipLaunchKernelGGL(k1f_kernel, dim3(1), dim3(NT), 0, s, M.partial, np, M.st, skip);
-  hipLaunchKernelGGL((k2_kernel<T>), dim3(eb), dim3(NT), 0, s, n, M.y, Rb, Ra, M.st, skip);
-  apply_M(S, P, Ra, M.z, skip, s);
+  if (k2_direct(S, P)) {
+    // K2 made by the preconditioner's first kernels (one vector pass and
+    // one launch fewer on the iteration's critical path)
+    const K2In<T> k2{M.y, Rb, Ra, &M.st->alfa, &M.st->beta};
+    Skip sk;
+    sk.all = skip;
+    precond_apply(P, Ra, M.z, sk, s, &k2);
+  } else {
+    hipLaunchKernelGGL((k2_kernel<T>), dim3(eb), dim3(NT), 0, s, n, M.y, Rb, Ra, M.st, skip);
+    apply_M(S, P, Ra, M.z, skip, s);
+  }
   int dg = dot_grid(n);
   hipLaunchKernelGGL((dot_kernel<T>), dim3(dg), dim3(NT), 0, s, n, Ra, M.z, skip, M.partial);
   hipLaunchKernelGGL(k4f_kernel, dim3(1), dim3(NT), 0, s, M.partial, dg, M.st, cur, next);
@@ -1151,7 +1171,8 @@ static int level_spmm_t(int mode, int nrows, const int* rp, const int* ci, const
   switch (mode) {
     case 0: spmm<T, NC, 0>(L, 0, (const T*)x, nullptr, (T*)y, nullptr, 0, 0, sk, s); break;
     case 1: spmm<T, NC, 1>(L, 0, (const T*)x, (const T*)b, (T*)y, nullptr, 0, 0, sk, s); break;
-    case 3: spmm<T, NC, 3>(L, 0, (const T*)x, (const T*)b, (T*)y, (T*)d, (T)cd, (T)cr, sk, s);
+    case 3: spmm<T, NC, 3>(L, 0, (const T*)x, (const T*)b, (T*)y, (T*)d, (T)cd, (T)cr, sk, s,
+                           nullptr, (T*)d);
             break;
     default: return ERR_INVALID;
   }

@@ -25,12 +25,13 @@ def _dev(a, dtype, device):
 def block_partial_tables(cell_nodes, nnode, B):
     """Block partials of the cell-wise operator for a cell -> node table
     (nC x k tensor, any device): cells are taken in blocks of B; every
-    (node, block) pair is one partial, numbered in the order (node, block)
-    (its entry: a node's partials contiguous).  Returns the node table in
-    structure-of-arrays layout, node -> first entry (rp), the number of
-    partials, and per block b the partials q in [bq[b], bq[b+1]) (ordered by
-    entry) with their entry gp[q] and their local slots
-    lsl[lrp[q] .. lrp[q+1]) (c * k + j, ascending; c the cell in the block)."""
+    (node, block) pair is one partial.  Partials are stored block-major:
+    block b's are q in [bq[b], bq[b+1]) (ordered by node), partial q sums
+    the local slots lsl[lrp[q] .. lrp[q+1]) (c * k + j, ascending; c the
+    cell in the block).  Entries list the partials in the order (node,
+    block), a node's contiguous from rp[node]: entry e holds partial pe[e],
+    and partial q sits at entry gp[q].  Also returns the node table in
+    structure-of-arrays layout and the number of partials."""
     i32, i64 = torch.int32, torch.int64
     dev = cell_nodes.device
     nC, k = cell_nodes.shape
@@ -61,8 +62,10 @@ def block_partial_tables(cell_nodes, nnode, B):
     lrp = torch.zeros(npart + 1, dtype=i64, device=dev)
     lrp[1:] = torch.cumsum(cnt, 0)
     soa = nodes.reshape(nC, k).t().contiguous().to(i32)
+    pe = torch.empty_like(gp)
+    pe[gp] = torch.arange(npart, device=dev, dtype=i64)
     return dict(node=soa, rp=rp.to(i32), npart=npart, bq=bq.to(i32), lrp=lrp.to(i32),
-                gp=gp.to(i32), lsl=lsl.contiguous())
+                gp=gp.to(i32), lsl=lsl.contiguous(), pe=pe.to(i32))
 
 
 def assemble_batch(systems):
@@ -280,8 +283,8 @@ class DeviceSystem:
         return _lib.CellOpDesc(cw["nC"], self.nP2, self.nP1, self.A, ptr(t2["node"]),
                                ptr(t1["node"]), ptr(cw["geom"]), self.CELLS_PER_BLOCK,
                                t2["npart"], t1["npart"], ptr(t2["bq"]), ptr(t2["lrp"]),
-                               ptr(t2["gp"]), ptr(t2["lsl"]), ptr(t1["bq"]), ptr(t1["lrp"]),
-                               ptr(t1["gp"]), ptr(t1["lsl"]), ptr(t2["rp"]), ptr(t1["rp"]),
+                               ptr(t2["pe"]), ptr(t2["lsl"]), ptr(t1["bq"]), ptr(t1["lrp"]),
+                               ptr(t1["pe"]), ptr(t1["lsl"]), ptr(t2["rp"]), ptr(t1["rp"]),
                                ptr(p11[0]), ptr(p11[1]), self.nnz["11"], ptr(v["M1"]),
                                ptr(v["St"]), _lib.make_coef(self.Cm, self.Cs, self.mu),
                                ptr(self.dmask) if masked else None, lu, lp, xc)

@@ -120,6 +120,11 @@ class SolverConfig:
     # fixed symmetric operator, so MINRES is unaffected beyond the slightly
     # perturbed preconditioner
     compact_level0: bool = True
+    # the same for the pressure levels (the finest: field-major
+    # block-diagonal P_f) but the dense coarsest, each when every row's
+    # diagonal stays in half's normal range under one scale (the coarser
+    # ones only with the finest)
+    compact_plevel0: bool = True
     # launch shapes picked by the autotuner (waterscapes_amd/autotune.py):
     # None = on for systems of at least autotune_min_dofs unknowns (below,
     # launches are latency-bound and the fixed heuristic is kept)
@@ -165,19 +170,29 @@ def _pick_lpr(avg, nc=1, rows=None):
     return lanes
 
 
-def compact_level(pattern_host, values):
+def compact_level(pattern_host, values, min_diag_ratio=None):
     """Half-precision copy of a level operator on a host CSR pattern
     (rp, ci) with device FP64 values: values * 2^e rounded to half (the
     largest magnitude mapped near 2^10), columns as int16 offsets from the
-    row.  None if some offset does not fit in 16 bits."""
+    row.  None if some offset does not fit in 16 bits, or (min_diag_ratio
+    given) if some nonzero diagonal entry is smaller than min_diag_ratio
+    times the largest value: half is absolutely accurate to 2^-25 (scaled)
+    below its normal range, so with every nonzero diagonal normal each
+    row's entries keep half's relative precision measured on the row."""
     rp, ci = pattern_host
     rows = np.repeat(np.arange(rp.size - 1, dtype=np.int64), np.diff(rp))
     off = ci.astype(np.int64) - rows
     if off.size == 0 or np.abs(off).max() > 32767:
         return None
-    vmax = float(values.abs().max().item())
+    mag = values.abs()
+    vmax = float(mag.max().item())
     if not vmax > 0:
         return None
+    if min_diag_ratio is not None:
+        dg = mag[torch.as_tensor(np.flatnonzero(off == 0)).to(mag.device)]
+        dg = dg[dg > 0]
+        if dg.numel() == 0 or float(dg.min().item()) < min_diag_ratio * vmax:
+            return None
     e = 10 - int(np.ceil(np.log2(vmax)))
     v16 = (values * (2.0 ** e)).to(torch.float16)
     dci = torch.as_tensor(off.astype(np.int16)).to(values.device)
@@ -421,6 +436,32 @@ class KrylovSolver:
                 u[0][0].update(c)
                 self.compact = c
         p = self._upload_levels(self.plevels, self.pcinv, dtype)
+        if dtype == 1 and self.cfg.compact_plevel0:
+            # half keeps 11 bits down to 2^-14: with the largest value near
+            # 2^10, diagonals down to 2^-22 of it stay normal
+            # (Dirichlet rows, identity rows of the eliminated P_f, are
+            # stored as zeros and masked: the pressure residual is zero there)
+            lev = self.plevels[0]
+            n1, F = self.sys.nP1, self.sys.F
+            dm = self.pmask.view(n1, F).t().contiguous().view(-1)
+            rows = np.repeat(np.arange(lev["n"], dtype=np.int64), np.diff(lev["A_rp"]))
+            v = torch.as_tensor(lev["A_v"]).to(self.sys.device)
+            v[dm[torch.as_tensor(rows).to(v.device)].bool()] = 0.0
+            c = compact_level((lev["A_rp"], lev["A_ci"]), v, 2.0 ** -22)
+            self.pcompact_levels = []
+            if c is not None:
+                p[0][0].update(c, mask0=dm)
+                self.pcompact_levels.append(0)
+                # the coarser levels but the dense coarsest: their SpMMs
+                # read as many bytes per V-cycle as the finest level's
+                for l in range(1, len(self.plevels) - 1):
+                    lv = self.plevels[l]
+                    cl = compact_level((lv["A_rp"], lv["A_ci"]),
+                                       torch.as_tensor(lv["A_v"]).to(self.sys.device),
+                                       2.0 ** -22)
+                    if cl is not None:
+                        p[0][l].update(cl)
+                        self.pcompact_levels.append(l)
         self.level_dev[dtype] = (u, p)
         return u, p
 
@@ -485,7 +526,8 @@ class KrylovSolver:
         self.keep += [ulevs, ucinv, plevs, pcinv]
         d = PrecDesc()
         self._hier_desc(d.u, self.levels, ulevs, ucinv, self.cfg.amg, s.mu, self.mask0, 3)
-        self._hier_desc(d.p, self.plevels, plevs, pcinv, self.cfg.pamg, 1.0, None, 1)
+        self._hier_desc(d.p, self.plevels, plevs, pcinv, self.cfg.pamg, 1.0,
+                        plevs[0].get("mask0"), 1)
         d.nP1 = s.nP1
         d.F = s.F
         return d
