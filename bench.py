@@ -105,20 +105,20 @@ def cell_op_bytes(sys_):
     """Algorithmic HBM bytes of one cell-wise FP32 operator apply
     (csrc/cellwise.h), each array once: the cell kernel reads the node
     tables (14 ints per cell), the geometry (10 floats), x and the block
-    partial tables (block and slot offsets, entries, 2-byte local slots),
-    and writes the partials (3 floats per P2 partial, 1 per P1 partial); the
-    row kernel reads the node offsets, the partials, the P1 x P1 block
-    (indices, M1, A stiffness arrays) and the pressure part of x, and
-    writes y."""
+    partial tables (block and slot offsets, 2-byte local slots), and writes
+    the partials (3 floats per P2 partial, 1 per P1 partial, block-major);
+    the row kernel reads the node offsets, the entries (one partial index
+    each), the partials, the P1 x P1 block (indices, M1, A stiffness
+    arrays) and the pressure part of x, and writes y."""
     cw = sys_.cell_op()
     nC = cw["nC"]
     n, F, A = sys_.n, sys_.F, sys_.A
     nz11 = sys_.nnz["11"]
     nb = (nC + sys_.CELLS_PER_BLOCK - 1) // sys_.CELLS_PER_BLOCK
     p2, p1 = cw["p2"]["npart"], cw["p1"]["npart"]
-    tables = (nb + 1) * 4 * 2 + (2 * p2 + 1) * 4 + 10 * nC * 2 + (2 * p1 + 1) * 4 + 4 * nC * 2
+    tables = (nb + 1) * 4 * 2 + (p2 + 1) * 4 + 10 * nC * 2 + (p1 + 1) * 4 + 4 * nC * 2
     cells = nC * (14 * 4 + 10 * 4) + n * 4 + tables + (3 * p2 + p1) * 4
-    rows = ((sys_.nP2 + 1) + 2 * (sys_.nP1 + 1)) * 4 + (3 * p2 + p1) * 4 \
+    rows = ((sys_.nP2 + 1) + 2 * (sys_.nP1 + 1)) * 4 + (p2 + p1) * 4 + (3 * p2 + p1) * 4 \
         + nz11 * (4 + 4 + 4 * A) + F * sys_.nP1 * 4 + n * 4
     return cells + rows
 
@@ -512,7 +512,8 @@ def main_serve(args):
                 for k, ((N, p, st), a) in enumerate(zip(specs, arrivals))]
         torch.cuda.reset_peak_memory_stats()
         srv = MPETServer(hbm_budget=args.budget * 1e9, precision="mixed",
-                         outer_tol=args.outer_tol, inner_rtol=args.inner_rtol)
+                         outer_tol=args.outer_tol, inner_rtol=args.inner_rtol,
+                         policy=args.policy, tick_dofs=args.tick_dofs)
         for r in reqs:
             srv.submit(r)
         t0 = time.perf_counter()
@@ -541,7 +542,8 @@ def main_serve(args):
             "config": {"workload": f"Poisson arrivals at {rate} req/s of {len(specs)} requests, "
                                    f"uneven family + 1 in 20 at 48^3 A=4, budget "
                                    f"{args.budget} GB", "rate": rate, "requests": len(specs),
-                       "seed": args.seed},
+                       "seed": args.seed, "policy": args.policy,
+                       "tick_dofs": args.tick_dofs},
             "latency_s": {"p50": round(float(np.percentile(lat, 50)), 3),
                           "p99": round(float(np.percentile(lat, 99)), 3)} if lat.size else None,
             "refused": sum(r.refused for r in srv.finished),
@@ -948,6 +950,8 @@ def main():
     ap.add_argument("--requests", type=int, default=40)
     ap.add_argument("--steps-scale", type=float, default=1.0)
     ap.add_argument("--budget", type=float, default=200.0)
+    ap.add_argument("--policy", choices=("srpt", "all"), default="srpt")
+    ap.add_argument("--tick-dofs", type=float, default=6e6)
     args = ap.parse_args()
     if args.inner_rtol is None:
         args.inner_rtol = 1e-6

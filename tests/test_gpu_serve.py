@@ -38,13 +38,13 @@ def _requests(which):
     return out
 
 
-def _serve(reqs, budget=200e9):
+def _serve(reqs, budget=200e9, **kw):
     """Synchronous set-up (setup_workers = 0), so that admission happens at
     the tick a request arrives and the schedule is deterministic."""
     from waterscapes_amd.serve import MPETServer
     clock = VClock()
     srv = MPETServer(hbm_budget=budget, precision="mixed", outer_tol=1e-11, clock=clock,
-                     setup_workers=0)
+                     setup_workers=0, **kw)
     for r in reqs:
         srv.submit(r)
     concurrency = []
@@ -73,6 +73,34 @@ def test_continuous_batching_and_isolation():
         space = MixedSpace(UnitCubeMesh(n, n, n), params["J"])
         errs = field_errors(space, done[i].result, a.result)
         assert max(errs) <= ISOLATION_RTOL, (i, errs)
+
+
+def test_srpt_policy_least_remaining_work_first():
+    """With a tick quota of one unknown, each tick steps exactly one
+    instance: the one with the least remaining work (steps left x DOFs), so
+    requests finish in the order of their work; results equal those of the
+    every-instance policy."""
+    which = [0, 1, 2, 3]
+    reqs = _requests(which)
+    for r in reqs:
+        r.arrival = 0.0
+    srv, conc = _serve(reqs, tick_dofs=1)
+    assert set(conc) == {1}
+    work = {r.rid: r.steps * r.ndofs for r in srv.finished}
+    order = [r.rid for r in sorted(srv.finished, key=lambda r: r.done)]
+    assert order == sorted(which, key=lambda i: (work[i], i)), (order, work)
+    ref_reqs = _requests(which)
+    for r in ref_reqs:
+        r.arrival = 0.0
+    ref, conc_all = _serve(ref_reqs, policy="all")
+    assert conc_all[0] == len(which)
+    by = {r.rid: r.result for r in ref.finished}
+    from waterscapes_amd.fem import MixedSpace
+    from waterscapes_amd.mesh import UnitCubeMesh
+    for r in srv.finished:
+        n, params = REQS[r.rid][0], REQS[r.rid][1]
+        space = MixedSpace(UnitCubeMesh(n, n, n), params["J"])
+        assert max(field_errors(space, r.result, by[r.rid])) <= ISOLATION_RTOL, r.rid
 
 
 def test_admission_by_footprint():

@@ -15,10 +15,15 @@ resident until the request completes.  At every scheduling tick it
      0: set up synchronously inside the tick);
   2. re-packs the batch when requests were admitted (no assembly or AMG
      setup is repeated: each instance's device state is reused);
-  3. runs one batched step of every unfinished instance (requests that
-     arrived while others were running join mid-flight: their steps overlap
-     on the device with the others'); finished instances stay packed but
-     inactive, so a retirement costs no re-pack;
+  3. runs one batched step of the unfinished instances the tick policy
+     picks (requests that arrived while others were running join
+     mid-flight: their steps overlap on the device with the others');
+     "all" steps every unfinished instance; "srpt" (the default) steps
+     those with the least remaining work (steps left x DOFs) first, as many
+     as make up tick_dofs unknowns together (at least one), so a short
+     request is not held behind long ones at every tick while the device
+     still gets a full batch; instances not picked, and finished ones, stay
+     packed but inactive, so neither costs a re-pack;
   4. retires finished requests (final state copied to the host); their
      memory is released when the batch is next re-packed — at the next
      admission, when half of the packed instances have finished, or when a
@@ -73,6 +78,7 @@ class Request:
     admitted: float = None
     done: float = None
     steps_done: int = 0
+    ndofs: int = 0
     result: np.ndarray = None
     niter: list = field(default_factory=list)
     refused: bool = False
@@ -105,7 +111,8 @@ class MPETServer:
     """Continuous-batching server over one GPU (see module docstring)."""
 
     def __init__(self, hbm_budget=200e9, precision="mixed", outer_tol=3e-11, inner_rtol=1e-6,
-                 rtol=1e-10, max_instances=512, clock=time.perf_counter, setup_workers=2):
+                 rtol=1e-10, max_instances=512, clock=time.perf_counter, setup_workers=2,
+                 policy="srpt", tick_dofs=6e6):
         if not torch.cuda.is_available():
             raise RuntimeError("MPETServer needs a GPU (no CPU fallback)")
         self.budget = float(hbm_budget)
@@ -117,6 +124,10 @@ class MPETServer:
         self.preparing = []        # (request, future): admitted, set-up running
         self.active = []
         self.setup_workers = int(setup_workers)
+        if policy not in ("all", "srpt"):
+            raise ValueError("policy must be 'all' or 'srpt'")
+        self.policy = policy
+        self.tick_dofs = float(tick_dofs)
         self._pool = None
         self._local = threading.local()
         self.finished = []
@@ -153,6 +164,7 @@ class MPETServer:
             if r.solver is None:
                 r.solver = r.make_solver()
             batch_engine(r.solver, self.precision)
+            r.ndofs = int(r.solver.up.data.numel())
             return r
         return self._on_stream(work, req)
 
@@ -281,8 +293,7 @@ class MPETServer:
         if changed or self.batch is None:
             self._repack()
         b = self.batch
-        act = np.array([1 if r.steps_done < r.steps else 0 for r in self.batch_reqs],
-                       dtype=np.int32)
+        act = self._pick([1 if r.steps_done < r.steps else 0 for r in self.batch_reqs])
         stats = b.step(act)
         b.commit(None if act.all() else act)
         for k, r in enumerate(self.batch_reqs):
@@ -295,6 +306,24 @@ class MPETServer:
         torch.cuda.synchronize()
         self._retire(self.clock() - self.t0)
         return stepped
+
+    def _pick(self, unfinished):
+        """Instances (0/1 per packed request) the tick steps under the policy."""
+        act = np.array(unfinished, dtype=np.int32)
+        if self.policy == "all" or act.sum() <= 1:
+            return act
+        reqs = self.batch_reqs
+        cand = sorted((k for k in range(len(reqs)) if act[k]),
+                      key=lambda k: ((reqs[k].steps - reqs[k].steps_done) * reqs[k].ndofs,
+                                     reqs[k].arrival, k))
+        sel = np.zeros_like(act)
+        tot = 0
+        for k in cand:
+            sel[k] = 1
+            tot += reqs[k].ndofs
+            if tot >= self.tick_dofs:
+                break
+        return sel
 
     def run(self, until_idle=True, idle_sleep=1e-3):
         """Tick until every submitted request has finished."""
