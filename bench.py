@@ -101,8 +101,9 @@ def op_bytes(sys_, dtype):
     return vals + idx + vec
 
 
-def cell_op_bytes(sys_):
-    """Algorithmic HBM bytes of one cell-wise FP32 operator apply
+def cell_op_bytes(sys_, s=4):
+    """Algorithmic HBM bytes of one cell-wise operator apply (s = 4: FP32,
+    8: FP64 with FP64 geometry, partials, P1 x P1 values and vectors)
     (csrc/cellwise.h), each array once: the cell kernel reads the node
     tables (14 ints per cell), the geometry (10 floats), x and the block
     partial tables (block and slot offsets, 2-byte local slots), and writes
@@ -117,9 +118,9 @@ def cell_op_bytes(sys_):
     nb = (nC + sys_.CELLS_PER_BLOCK - 1) // sys_.CELLS_PER_BLOCK
     p2, p1 = cw["p2"]["npart"], cw["p1"]["npart"]
     tables = (nb + 1) * 4 * 2 + (p2 + 1) * 4 + 10 * nC * 2 + (p1 + 1) * 4 + 4 * nC * 2
-    cells = nC * (14 * 4 + 10 * 4) + n * 4 + tables + (3 * p2 + p1) * 4
-    rows = ((sys_.nP2 + 1) + 2 * (sys_.nP1 + 1)) * 4 + (p2 + p1) * 4 + (3 * p2 + p1) * 4 \
-        + nz11 * (4 + 4 + 4 * A) + F * sys_.nP1 * 4 + n * 4
+    cells = nC * (14 * 4 + 10 * s) + n * s + tables + (3 * p2 + p1) * s
+    rows = ((sys_.nP2 + 1) + 2 * (sys_.nP1 + 1)) * 4 + (p2 + p1) * 4 + (3 * p2 + p1) * s \
+        + nz11 * (4 + s + s * A) + F * sys_.nP1 * s + n * s
     return cells + rows
 
 
@@ -217,7 +218,8 @@ def iteration_bytes(eng, mixed):
 
 def step_bytes(eng, mixed, inner, outer):
     """Algorithmic bytes of one time step: inner iterations, the FP64
-    residuals and MINRES starts of the solve (two for the norm of the
+    residuals (through the cell-wise FP64 operator on the mixed path) and
+    MINRES starts of the solve (two for the norm of the
     lifted right-hand side and the first residual, one more per refinement
     step) and the right-hand-side update (P1-pattern kernel)."""
     sysm = eng.system
@@ -225,7 +227,10 @@ def step_bytes(eng, mixed, inner, outer):
     s = 4 if mixed else 8
     n = sysm.n
     kr = eng.krylov
-    op64 = op_bytes(sysm, 0) if (mixed or kr.csr_desc is None) else csr_bytes(sysm)
+    if mixed and getattr(kr, "cell_desc64", None) is not None:
+        op64 = cell_op_bytes(sysm, 8)
+    else:
+        op64 = op_bytes(sysm, 0) if (mixed or kr.csr_desc is None) else csr_bytes(sysm)
     pc = vcycle_bytes(kr.levels, kr.cfg.amg, 3, s, compact=_compact_u(kr, mixed),
                       scaled=True) + vcycle_bytes(kr.plevels, kr.cfg.pamg, 1, s,
                                                   compact=_compact_p(kr, mixed))

@@ -351,12 +351,14 @@ int mpet_facet_scatter_mat(int nrows, const int* row_node, const int* inc_rp, co
 int mpet_op_apply(const MpetOpDesc* op, const void* x, const void* b, void* y,
                   hipStream_t stream);
 
-/* Cell-wise (matrix-free) FP32 form of the same operator, the MatMult of
- * the inner FP32 MINRES (:493): the u rows and the divergence part of the
- * p0 rows (2 mu eps(u):eps(v) + p0 div v + div(u) w0, :267-269) are
- * integrated cell by cell from the geometry, exactly (P2 gradients are
- * linear); the P1 x P1 pressure block (C (x) M1 + stiff St, with the Robin
- * terms in St) is read from the FP32 structured copy.  Summation order is
+/* Cell-wise (matrix-free) form of the same operator, in FP32 (fp64 = 0:
+ * the MatMult of the inner FP32 MINRES, :493) or FP64 (fp64 = 1: the
+ * refinement's FP64 residuals, b - A x): the u rows and the divergence
+ * part of the p0 rows (2 mu eps(u):eps(v) + p0 div v + div(u) w0,
+ * :267-269) are integrated cell by cell from the geometry, exactly (P2
+ * gradients are linear); the P1 x P1 pressure block (C (x) M1 + stiff St,
+ * with the Robin terms in St) is read from the structured copy of the same
+ * precision.  Summation order is
  * fixed: results are reproducible.  Cells are taken in blocks of
  * cells_per_block (= 256) consecutive cells; a block sums, per node its
  * cells touch, their contributions in slot order, and writes one partial
@@ -365,20 +367,21 @@ int mpet_op_apply(const MpetOpDesc* op, const void* x, const void* b, void* y,
  * entry list.
  * Cell tables are structure-of-arrays, [k*nC + e]:
  *   node2 / node1  P2 node of local node k (UFC order) / P1 node of vertex k
- *   geom           10*nC floats from mpet_cell_geometry
+ *   geom           10*nC values from mpet_cell_geometry (FP32) or
+ *                  mpet_cell_geometry64 (FP64)
  * Partials, for P2 nodes (suffix 2, local slot c*10 + k, c the cell within
  * its block) and P1 nodes (suffix 1, local slot c*4 + v):
  *   bq[b] .. bq[b+1]-1   the partials of block b (nblocks + 1 entries)
  *   lrp[q] .. lrp[q+1]-1 partial q's local slots in lsl (ascending)
  *   pe[k]                the partial at entry k of the node-sorted list
  *   inc2_rp / inc1_rp    node -> first entry of its partials (n + 1)
- * Work: 3*npart2 + npart1 floats.  lanes_u in {4, 8, 16}, lanes_p in
- * {8, 16} (others: 8).
+ * Work: 3*npart2 + npart1 values of the descriptor's precision.  lanes_u
+ * in {4, 8, 16}, lanes_p in {8, 16} (others: 8).
  * The reference always multiplies by the assembled PETSc matrix (:437). */
 typedef struct {
   int32_t nC, nP2, nP1, A;
   const int32_t *node2, *node1;  /* 10*nC, 4*nC */
-  const float* geom;
+  const void* geom;               /* float, or double when fp64 */
   int32_t cells_per_block, npart2, npart1;
   const int32_t *bq2, *lrp2, *pe2;
   const int16_t* lsl2;
@@ -387,10 +390,11 @@ typedef struct {
   const int32_t *inc2_rp, *inc1_rp;   /* nP2 + 1, nP1 + 1 */
   const int32_t *rp11, *ci11;
   int32_t nnz11;
-  const float *M1, *St;          /* FP32 structured copy (nnz11, A*nnz11) */
+  const void *M1, *St;           /* structured copy (nnz11, A*nnz11), float or double */
   MpetCoef coef;
   const uint8_t* dmask;
   int32_t lanes_u, lanes_p, xcd_chunk;
+  int32_t fp64;                  /* 0: FP32 arrays and vectors, 1: FP64 */
 } MpetCellOpDesc;
 
 /* Per-cell geometry of the cell-wise operator, FP32 structure-of-arrays:
@@ -398,10 +402,15 @@ typedef struct {
  * = |T_e|, computed in FP64 from mesh->coords and rounded. */
 int mpet_cell_geometry(const MpetMeshDesc* mesh, float* geom, hipStream_t stream);
 
-/* y = A x / y = b - A x with Dirichlet rows zeroed, as mpet_op_apply (FP32)
- * through the cell-wise operator; work: 3*npart2 + npart1 floats. */
-int mpet_cell_op_apply(const MpetCellOpDesc* op, const float* x, const float* b, float* y,
-                       float* work, hipStream_t stream);
+/* The same geometry kept in FP64 (for the FP64 cell-wise operator).  No
+ * counterpart in the reference. */
+int mpet_cell_geometry64(const MpetMeshDesc* mesh, double* geom, hipStream_t stream);
+
+/* y = A x / y = b - A x with Dirichlet rows zeroed, as mpet_op_apply,
+ * through the cell-wise operator; x, b, y and work (3*npart2 + npart1
+ * values) in the descriptor's precision. */
+int mpet_cell_op_apply(const MpetCellOpDesc* op, const void* x, const void* b, void* y,
+                       void* work, hipStream_t stream);
 
 /* Scalar CSR SpMV (PETSc MatMult on AIJ, :493), int64 rowptr, lpr lanes
  * per row in {4,8,16,32,64}; optional residual b and row mask as above. */
@@ -549,6 +558,15 @@ int mpet_solver_set_tuning(void* h, int dtype, const MpetTuning* t);
  * the FP32 structured operator (op == NULL).  Captured graphs are dropped
  * and captured again by the next solve.  No counterpart in the reference. */
 int mpet_solver_set_cell_operator(void* h, const MpetCellOpDesc* op);
+
+/* FP64 residuals b - A x of a solver handle (the refinement's outer
+ * residuals in mixed mode, the start residuals in FP64 mode) through the
+ * FP64 cell-wise operator (op != NULL, fp64 = 1; the handle allocates its
+ * work and keeps a copy of the descriptor, whose arrays must outlive it)
+ * or back through the FP64 structured operator (op == NULL).  Not used
+ * when the handle's FP64 operator is a CSR matrix.  No counterpart in the
+ * reference (PETSc's MatMult on the assembled matrix, :493). */
+int mpet_solver_set_cell_operator64(void* h, const MpetCellOpDesc* op);
 
 /* z = P^-1 r with the FP64 (dtype 0) or FP32 (1) preconditioner. */
 int mpet_precond_apply(void* h, int dtype, const void* r, void* z, hipStream_t stream);

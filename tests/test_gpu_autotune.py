@@ -33,12 +33,13 @@ def test_solver_picks_are_valid_reused_and_accurate(tmp_path, monkeypatch):
     tun = tuned.engine().krylov.tuning
     assert 1 in tun and cache.timed > 0
     # the inner FP32 MINRES runs the cell-wise operator (its row kernel's
-    # lanes are tuned), the FP64 residual the structured FP64 operator
-    assert tun[1]["operator"] is None
+    # lanes are tuned), and so do the FP64 residuals (same launch shapes):
+    # no structured operator is tuned
+    assert tun[1]["operator"] is None and "op64" not in tun
     cell = tun["cell"]
     assert cell[0] in at.LANES_CELL_U and cell[1] in at.LANES_CELL_P and cell[2] in at.XCD_CHUNKS
-    op = tun["op64"]
-    assert op[0] in at.LANES_OP and op[1] in at.LANES_OP and op[2] in at.XCD_CHUNKS
+    d64 = tuned.engine().krylov.cell_desc64
+    assert d64.fp64 == 1 and [d64.lanes_u, d64.lanes_p, d64.xcd_chunk] == cell
     for rows in (tun[1]["u_levels"], tun[1]["p_levels"]):
         for r in rows:
             assert all(v == 0 or v in at.LPR_ALL for v in r)

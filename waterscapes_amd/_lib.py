@@ -61,7 +61,7 @@ class CellOpDesc(C.Structure):
                 ("inc1_rp", C.c_void_p), ("rp11", C.c_void_p), ("ci11", C.c_void_p),
                 ("nnz11", C.c_int32), ("M1", C.c_void_p), ("St", C.c_void_p), ("coef", Coef),
                 ("dmask", C.c_void_p), ("lanes_u", C.c_int32), ("lanes_p", C.c_int32),
-                ("xcd_chunk", C.c_int32)]
+                ("xcd_chunk", C.c_int32), ("fp64", C.c_int32)]
 
 
 class TwoFieldCoef(C.Structure):
@@ -130,8 +130,10 @@ def lib():
                               C.c_longlong, C.c_int, C.c_int, P, P], C.c_int),
         "mpet_op_apply": ([P, P, P, P, P], C.c_int),
         "mpet_cell_geometry": ([P, P, P], C.c_int),
+        "mpet_cell_geometry64": ([P, P, P], C.c_int),
         "mpet_cell_op_apply": ([P, P, P, P, P, P], C.c_int),
         "mpet_solver_set_cell_operator": ([P, P], C.c_int),
+        "mpet_solver_set_cell_operator64": ([P, P], C.c_int),
         "mpet_csr_spmv": ([C.c_int, C.c_longlong, P, P, P, P, P, P, P, C.c_int, P], C.c_int),
         "mpet_soa_convert": ([C.c_int, P, C.c_int, C.c_longlong, P, P], C.c_int),
     }

@@ -1,4 +1,6 @@
-// Cell-wise (matrix-free) FP32 application of the total-pressure operator.
+// Cell-wise (matrix-free) application of the total-pressure operator, in
+// FP32 (the inner MINRES of the mixed path) or FP64 (the refinement's
+// residuals: the same formulas in FP64 with FP64 geometry and partials).
 //
 // The u rows and the divergence part of the p0 rows, i.e. the bilinear form
 //     2 mu eps(u):eps(v) + p0 div v + div(u) w0
@@ -38,8 +40,10 @@
 
 namespace mpet {
 
-// Device view of MpetCellOpDesc plus the two work arrays.
-struct CellOp {
+// Device view of MpetCellOpDesc plus the two work arrays, in precision T
+// (FP32 for the inner MINRES, FP64 for the refinement's residuals).
+template <typename T>
+struct CellOpT {
   int nC, nP2, nP1, A;
   const int* node2;       // [k * nC + e]: P2 node of local node k (UFC order)
   const int* node1;       // [v * nC + e]: P1 node of vertex v
@@ -51,20 +55,21 @@ struct CellOp {
   const int16_t* lsl2;
   const int *bq1, *lrp1, *pe1;
   const int16_t* lsl1;
-  const float* geom;      // SoA: [d * nC + e] = G_{1 + d/3}[d % 3] (d < 9); |T| at d = 9
+  const T* geom;          // SoA: [d * nC + e] = G_{1 + d/3}[d % 3] (d < 9); |T| at d = 9
   const int *inc2_rp, *inc1_rp;     // node -> first entry (nP2 + 1, nP1 + 1)
   const int *rp11, *ci11;
   int nnz11;
-  const float *M1, *St;             // St: A x nnz11, network-major
-  float C[MPET_MAXF * MPET_MAXF];
-  float stiff[MPET_MAXF];
-  float mu;
+  const T *M1, *St;                 // St: A x nnz11, network-major
+  T C[MPET_MAXF * MPET_MAXF];
+  T stiff[MPET_MAXF];
+  T mu;
   const uint8_t* dmask;
   int lanes_u, lanes_p, xcd_chunk;
   int npart2, npart1;     // partials of all blocks (P2, P1)
-  float* Yu;              // 3 x npart2, block-major
-  float* Yp;              // npart1, block-major
+  T* Yu;                  // 3 x npart2, block-major
+  T* Yp;                  // npart1, block-major
 };
+using CellOp = CellOpT<float>;
 
 #ifndef MPET_CW_CELLS
 #define MPET_CW_CELLS 256
@@ -81,13 +86,13 @@ __device__ __forceinline__ int cw_edge(int j, int k) {
 // Phase 1: one thread per cell.
 // (A template, so that the translation units including this header share
 // one instance.)
-template <int NTB>
+template <int NTB, typename T>
 __global__ void __launch_bounds__(NTB)
-cw_cell_kernel(CellOp op, const float* __restrict__ x, const int* __restrict__ skip) {
+cw_cell_kernel(CellOpT<T> op, const T* __restrict__ x, const int* __restrict__ skip) {
   static_assert(NTB == CW_NT, "one cell per thread of a CW_NT block");
   if (skip && *skip) return;
-  __shared__ float su[CW_NT * 30];
-  __shared__ float sp[CW_NT * 4];
+  __shared__ T su[CW_NT * 30];
+  __shared__ T sp[CW_NT * 4];
   const int t = threadIdx.x;
   const long long e = (long long)blockIdx.x * CW_NT + t;
   if (e < op.nC) {
@@ -97,36 +102,36 @@ cw_cell_kernel(CellOp op, const float* __restrict__ x, const int* __restrict__ s
     for (int k = 0; k < 10; ++k) n2[k] = op.node2[k * nC + e];
 #pragma unroll
     for (int v = 0; v < 4; ++v) n1[v] = op.node1[v * nC + e];
-    float G[4][3];
+    T G[4][3];
 #pragma unroll
     for (int d = 0; d < 9; ++d) G[1 + d / 3][d % 3] = op.geom[d * nC + e];
-    const float vol = op.geom[9 * nC + e];
+    const T vol = op.geom[9 * nC + e];
 #pragma unroll
     for (int m = 0; m < 3; ++m) G[0][m] = -(G[1][m] + G[2][m] + G[3][m]);
     const int F = op.A + 1;
     const long long nu = 3ll * op.nP2;
-    float u[10][3], p0[4];
+    T u[10][3], p0[4];
 #pragma unroll
-    for (int k = 0; k < 10; ++k) load_nc<float, 3>(x + 3ll * n2[k], u[k]);
+    for (int k = 0; k < 10; ++k) load_nc<T, 3>(x + 3ll * n2[k], u[k]);
 #pragma unroll
     for (int v = 0; v < 4; ++v) p0[v] = x[nu + (long long)F * n1[v]];
     // U = sum_j u_j (x) G_j
-    float U[3][3];
+    T U[3][3];
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
       for (int d = 0; d < 3; ++d)
         U[b][d] = u[0][b] * G[0][d] + u[1][b] * G[1][d] + u[2][b] * G[2][d] + u[3][b] * G[3][d];
     // vertex stresses (symmetric: xx yy zz xy xz yz) and divergences
-    float s[4][6], tr[4];
-    const float mu = op.mu;
+    T s[4][6], tr[4];
+    const T mu = op.mu;
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
-      float g[3][3];
+      T g[3][3];
 #pragma unroll
       for (int b = 0; b < 3; ++b)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) g[b][d] = 4.0f * u[n][b] * G[n][d] - U[b][d];
+        for (int d = 0; d < 3; ++d) g[b][d] = (T)4 * u[n][b] * G[n][d] - U[b][d];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         if (k == n) continue;
@@ -134,36 +139,36 @@ cw_cell_kernel(CellOp op, const float* __restrict__ x, const int* __restrict__ s
 #pragma unroll
         for (int b = 0; b < 3; ++b)
 #pragma unroll
-          for (int d = 0; d < 3; ++d) g[b][d] += 4.0f * u[m][b] * G[k][d];
+          for (int d = 0; d < 3; ++d) g[b][d] += (T)4 * u[m][b] * G[k][d];
       }
       tr[n] = g[0][0] + g[1][1] + g[2][2];
-      s[n][0] = 2.0f * mu * g[0][0] + p0[n];
-      s[n][1] = 2.0f * mu * g[1][1] + p0[n];
-      s[n][2] = 2.0f * mu * g[2][2] + p0[n];
+      s[n][0] = (T)2 * mu * g[0][0] + p0[n];
+      s[n][1] = (T)2 * mu * g[1][1] + p0[n];
+      s[n][2] = (T)2 * mu * g[2][2] + p0[n];
       s[n][3] = mu * (g[0][1] + g[1][0]);
       s[n][4] = mu * (g[0][2] + g[2][0]);
       s[n][5] = mu * (g[1][2] + g[2][1]);
     }
-    float S[6];
+    T S[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) S[q] = (s[0][q] + s[1][q]) + (s[2][q] + s[3][q]);
-    const float c = vol * (1.0f / 20.0f);
-    auto put = [&](int k, const float* o) {
+    const T c = vol * ((T)1 / (T)20);
+    auto put = [&](int k, const T* o) {
       su[t * 30 + 3 * k] = o[0];
       su[t * 30 + 3 * k + 1] = o[1];
       su[t * 30 + 3 * k + 2] = o[2];
     };
     // symmetric 3x3 (xx yy zz xy xz yz) times a vector
-    auto symv = [](const float* M, const float* w, float* out) {
+    auto symv = [](const T* M, const T* w, T* out) {
       out[0] = M[0] * w[0] + M[3] * w[1] + M[4] * w[2];
       out[1] = M[3] * w[0] + M[1] * w[1] + M[5] * w[2];
       out[2] = M[4] * w[0] + M[5] * w[1] + M[2] * w[2];
     };
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      float M[6], o[3];
+      T M[6], o[3];
 #pragma unroll
-      for (int q = 0; q < 6; ++q) M[q] = c * (4.0f * s[j][q] - S[q]);
+      for (int q = 0; q < 6; ++q) M[q] = c * ((T)4 * s[j][q] - S[q]);
       symv(M, G[j], o);
       put(j, o);
     }
@@ -171,18 +176,18 @@ cw_cell_kernel(CellOp op, const float* __restrict__ x, const int* __restrict__ s
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
       const int j = EA[m], k = EB[m];
-      float Mk[6], Mj[6], o1[3], o2[3];
+      T Mk[6], Mj[6], o1[3], o2[3];
 #pragma unroll
       for (int q = 0; q < 6; ++q) {
-        Mk[q] = 4.0f * c * (S[q] + s[k][q]);
-        Mj[q] = 4.0f * c * (S[q] + s[j][q]);
+        Mk[q] = (T)4 * c * (S[q] + s[k][q]);
+        Mj[q] = (T)4 * c * (S[q] + s[j][q]);
       }
       symv(Mk, G[j], o1);
       symv(Mj, G[k], o2);
-      const float o[3] = {o1[0] + o2[0], o1[1] + o2[1], o1[2] + o2[2]};
+      const T o[3] = {o1[0] + o2[0], o1[1] + o2[1], o1[2] + o2[2]};
       put(4 + m, o);
     }
-    const float D = (tr[0] + tr[1]) + (tr[2] + tr[3]);
+    const T D = (tr[0] + tr[1]) + (tr[2] + tr[3]);
 #pragma unroll
     for (int v = 0; v < 4; ++v) sp[t * 4 + v] = c * (D + tr[v]);
   }
@@ -192,21 +197,21 @@ cw_cell_kernel(CellOp op, const float* __restrict__ x, const int* __restrict__ s
   // 12-byte stores to node-sorted lists measured 3.7x their bytes in memory
   // writes: lines shared by blocks on different XCDs)
   const int b = blockIdx.x;
-  float* __restrict__ Yu = op.Yu;
+  T* __restrict__ Yu = op.Yu;
   for (int q = op.bq2[b] + t; q < op.bq2[b + 1]; q += CW_NT) {
     const int lb = op.lrp2[q], le = op.lrp2[q + 1];
-    float a[3] = {0.f, 0.f, 0.f};
+    T a[3] = {(T)0, (T)0, (T)0};
     for (int i = lb; i < le; ++i) {
       const int sl = op.lsl2[i];
       a[0] += su[3 * sl];
       a[1] += su[3 * sl + 1];
       a[2] += su[3 * sl + 2];
     }
-    __builtin_memcpy(Yu + 3ll * q, a, 12);
+    __builtin_memcpy(Yu + 3ll * q, a, 3 * sizeof(T));
   }
   for (int q = op.bq1[b] + t; q < op.bq1[b + 1]; q += CW_NT) {
     const int lb = op.lrp1[q], le = op.lrp1[q + 1];
-    float a = 0.f;
+    T a = (T)0;
     for (int i = lb; i < le; ++i) a += sp[op.lsl1[i]];
     op.Yp[q] = a;
   }
@@ -214,33 +219,35 @@ cw_cell_kernel(CellOp op, const float* __restrict__ x, const int* __restrict__ s
 
 // Phase-2 epilogue of a plain application: y = A x, or y = b - A x with b;
 // rows with dmask set give 0.
-struct CwEpiApply {
-  float* y;
-  const float* b;
+template <typename T>
+struct CwEpiApplyT {
+  T* y;
+  const T* b;
   const uint8_t* mask;
   __device__ __forceinline__ bool skipped() const { return false; }
   __device__ __forceinline__ void prep() {}
-  __device__ __forceinline__ double operator()(long long i, float ax) const {
-    float v = b ? b[i] - ax : ax;
-    if (mask && mask[i]) v = 0.0f;
+  __device__ __forceinline__ double operator()(long long i, T ax) const {
+    T v = b ? b[i] - ax : ax;
+    if (mask && mask[i]) v = (T)0;
     y[i] = v;
     return 0.0;
   }
 };
+using CwEpiApply = CwEpiApplyT<float>;
 
 // Phase 2: GU lanes per P2 node row, GP lanes per P1 node row, grid-stride
 // over row tiles (u tiles first), XCD-aware tile order.  The epilogue's
 // per-row results are summed per block into partial[blockIdx.x] (when given).
-template <int F, int GU, int GP, typename Epi>
+template <int F, int GU, int GP, typename Epi, typename T>
 __global__ void __launch_bounds__(256)
-cw_rows_kernel(CellOp op, const float* __restrict__ x, Epi epi, double* __restrict__ partial) {
+cw_rows_kernel(CellOpT<T> op, const T* __restrict__ x, Epi epi, double* __restrict__ partial) {
   if (epi.skipped()) return;
   epi.prep();
   constexpr int RU = 256 / GU, RP = 256 / GP;
   const long long nu = 3ll * op.nP2;
   const int ntu = (op.nP2 + RU - 1) / RU, ntp = (op.nP1 + RP - 1) / RP;
-  const float* __restrict__ Yu = op.Yu;
-  const float* __restrict__ Yp = op.Yp;
+  const T* __restrict__ Yu = op.Yu;
+  const T* __restrict__ Yp = op.Yp;
   double acc = 0.0;
   for (int tt = blockIdx.x; tt < ntu + ntp; tt += gridDim.x) {
     const int tile = xcd_tile(tt, ntu + ntp, op.xcd_chunk);
@@ -249,15 +256,15 @@ cw_rows_kernel(CellOp op, const float* __restrict__ x, Epi epi, double* __restri
       const int a = tile * RU + threadIdx.x / GU;
       if (a < op.nP2) {
         const int kb = op.inc2_rp[a], ke = op.inc2_rp[a + 1];
-        float y0 = 0.f, y1 = 0.f, y2 = 0.f;
+        T y0 = (T)0, y1 = (T)0, y2 = (T)0;
         // the node's entries are contiguous: two per lane per trip, loads
         // (entry -> partial -> value) issued before use
         for (int k = kb + lane; k < ke; k += 2 * GU) {
           const int k2 = k + GU < ke ? k + GU : k;
-          float av[3], bv[3];
-          load_nc<float, 3>(Yu + 3ll * op.pe2[k], av);
-          load_nc<float, 3>(Yu + 3ll * op.pe2[k2], bv);
-          if (!(k + GU < ke)) bv[0] = bv[1] = bv[2] = 0.f;
+          T av[3], bv[3];
+          load_nc<T, 3>(Yu + 3ll * op.pe2[k], av);
+          load_nc<T, 3>(Yu + 3ll * op.pe2[k2], bv);
+          if (!(k + GU < ke)) bv[0] = bv[1] = bv[2] = (T)0;
           y0 += av[0];
           y1 += av[1];
           y2 += av[2];
@@ -272,7 +279,7 @@ cw_rows_kernel(CellOp op, const float* __restrict__ x, Epi epi, double* __restri
           y2 += __shfl_xor(y2, off, 64);
         }
         if (lane < 3) {
-          const float yc = lane == 0 ? y0 : (lane == 1 ? y1 : y2);
+          const T yc = lane == 0 ? y0 : (lane == 1 ? y1 : y2);
           acc += epi(3ll * a + lane, yc);
         }
       }
@@ -280,22 +287,22 @@ cw_rows_kernel(CellOp op, const float* __restrict__ x, Epi epi, double* __restri
       const int lane = threadIdx.x % GP;
       const int c = (tile - ntu) * RP + threadIdx.x / GP;
       if (c < op.nP1) {
-        float yd = 0.f;
+        T yd = (T)0;
         const int kb = op.inc1_rp[c], ke = op.inc1_rp[c + 1];
         for (int k = kb + lane; k < ke; k += GP) yd += Yp[op.pe1[k]];
-        float m[F], sv[F];
+        T m[F], sv[F];
 #pragma unroll
-        for (int g = 0; g < F; ++g) { m[g] = 0.f; sv[g] = 0.f; }
+        for (int g = 0; g < F; ++g) { m[g] = (T)0; sv[g] = (T)0; }
         const int qb = op.rp11[c], qe = op.rp11[c + 1];
         const long long n11 = op.nnz11;
         for (int k = qb + lane; k < qe; k += GP) {
-          const float* xp = x + nu + (long long)F * op.ci11[k];
-          const float mm = op.M1[k];
-          float xv[F], st[F];
+          const T* xp = x + nu + (long long)F * op.ci11[k];
+          const T mm = op.M1[k];
+          T xv[F], st[F];
 #pragma unroll
           for (int g = 0; g < F; ++g) {
             xv[g] = xp[g];
-            st[g] = g >= 1 ? op.St[(g - 1) * n11 + k] : 0.f;
+            st[g] = g >= 1 ? op.St[(g - 1) * n11 + k] : (T)0;
           }
 #pragma unroll
           for (int g = 0; g < F; ++g) {
@@ -305,10 +312,10 @@ cw_rows_kernel(CellOp op, const float* __restrict__ x, Epi epi, double* __restri
         }
 #pragma unroll
         for (int off = GP / 2; off > 0; off >>= 1) yd += __shfl_xor(yd, off, 64);
-        float mine = 0.f;
+        T mine = (T)0;
 #pragma unroll
         for (int g = 0; g < F; ++g) {
-          float mg = m[g], sg = sv[g];
+          T mg = m[g], sg = sv[g];
 #pragma unroll
           for (int off = GP / 2; off > 0; off >>= 1) {
             mg += __shfl_xor(mg, off, 64);
@@ -334,18 +341,18 @@ cw_rows_kernel(CellOp op, const float* __restrict__ x, Epi epi, double* __restri
 // Launch of phase 2 for F fields and the lane counts of the operator (GU in
 // {4, 8, 16}, GP in {8, 16}; others fall back to 8).  Returns the grid
 // (number of partials written when partial is given).
-template <typename Epi, int F, int GU>
-static void cw_rows_gp(const CellOp& op, int gp, dim3 grid, hipStream_t s, const float* x,
+template <typename Epi, int F, int GU, typename T>
+static void cw_rows_gp(const CellOpT<T>& op, int gp, dim3 grid, hipStream_t s, const T* x,
                        Epi epi, double* partial) {
   if (gp == 16)
-    hipLaunchKernelGGL((cw_rows_kernel<F, GU, 16, Epi>), grid, dim3(256), 0, s, op, x, epi,
+    hipLaunchKernelGGL((cw_rows_kernel<F, GU, 16, Epi, T>), grid, dim3(256), 0, s, op, x, epi,
                        partial);
   else
-    hipLaunchKernelGGL((cw_rows_kernel<F, GU, 8, Epi>), grid, dim3(256), 0, s, op, x, epi,
+    hipLaunchKernelGGL((cw_rows_kernel<F, GU, 8, Epi, T>), grid, dim3(256), 0, s, op, x, epi,
                        partial);
 }
-template <typename Epi, int F>
-static void cw_rows_f(const CellOp& op, dim3 grid, hipStream_t s, const float* x, Epi epi,
+template <typename Epi, int F, typename T>
+static void cw_rows_f(const CellOpT<T>& op, dim3 grid, hipStream_t s, const T* x, Epi epi,
                       double* partial) {
   switch (op.lanes_u) {
     case 4: cw_rows_gp<Epi, F, 4>(op, op.lanes_p, grid, s, x, epi, partial); break;
@@ -355,17 +362,18 @@ static void cw_rows_f(const CellOp& op, dim3 grid, hipStream_t s, const float* x
 }
 static inline int cw_lanes_u(int g) { return (g == 4 || g == 8 || g == 16) ? g : 8; }
 static inline int cw_lanes_p(int g) { return (g == 8 || g == 16) ? g : 8; }
-static inline int cw_row_tiles(const CellOp& op) {
+template <typename T>
+static inline int cw_row_tiles(const CellOpT<T>& op) {
   const int gu = cw_lanes_u(op.lanes_u), gp = cw_lanes_p(op.lanes_p);
   return (op.nP2 + 256 / gu - 1) / (256 / gu) + (op.nP1 + 256 / gp - 1) / (256 / gp);
 }
 // Both phases: cells (x -> Yu, Yp), then rows through epi.  Returns the
 // number of blocks of phase 2 (one partial each when partial is given).
-template <typename Epi>
-static int cw_launch(const CellOp& op, const float* x, Epi epi, double* partial,
+template <typename Epi, typename T>
+static int cw_launch(const CellOpT<T>& op, const T* x, Epi epi, double* partial,
                      const int* skip, hipStream_t s) {
   const int cb = (op.nC + CW_NT - 1) / CW_NT;
-  hipLaunchKernelGGL(cw_cell_kernel<CW_NT>, dim3(cb), dim3(CW_NT), 0, s, op, x, skip);
+  hipLaunchKernelGGL((cw_cell_kernel<CW_NT, T>), dim3(cb), dim3(CW_NT), 0, s, op, x, skip);
   const int np = grid_for(cw_row_tiles(op), 1, MAX_PARTIALS);
   const dim3 grid(np);
   switch (op.A + 1) {
@@ -381,15 +389,18 @@ static int cw_launch(const CellOp& op, const float* x, Epi epi, double* partial,
   return np;
 }
 
-// Work floats of a descriptor (the P2 and P1 partials).
+// Work values (of the descriptor's precision) of a descriptor: the P2 and
+// P1 partials.
 static inline long long cw_work_floats(const MpetCellOpDesc* d) {
   return 3ll * d->npart2 + d->npart1;
 }
 
-// Host: device view of a descriptor (coefficients rounded to FP32) with
-// the work arrays Yu = work, Yp = work + 3 npart2; checks the descriptor.
-static int cw_from_desc(const MpetCellOpDesc* d, float* work, CellOp& op) {
-  if (!d || d->nC <= 0 || d->nP2 <= 0 || d->nP1 <= 0 || d->A < 1 || d->A + 1 > MPET_MAXF ||
+// Host: device view of a descriptor (coefficients rounded to T; the
+// descriptor's fp64 flag must match T) with the work arrays Yu = work,
+// Yp = work + 3 npart2; checks the descriptor.
+template <typename T>
+static int cw_from_desc(const MpetCellOpDesc* d, T* work, CellOpT<T>& op) {
+  if (!d || (d->fp64 != 0) != (sizeof(T) == 8) || d->nC <= 0 || d->nP2 <= 0 || d->nP1 <= 0 || d->A < 1 || d->A + 1 > MPET_MAXF ||
       d->nnz11 <= 0 || !d->node2 || !d->node1 || !d->geom || d->cells_per_block != CW_NT ||
       d->npart2 <= 0 || d->npart1 <= 0 || !d->bq2 || !d->lrp2 || !d->pe2 || !d->lsl2 ||
       !d->bq1 || !d->lrp1 || !d->pe1 || !d->lsl1 ||
@@ -414,17 +425,17 @@ static int cw_from_desc(const MpetCellOpDesc* d, float* work, CellOp& op) {
   op.lsl1 = d->lsl1;
   op.npart2 = d->npart2;
   op.npart1 = d->npart1;
-  op.geom = d->geom;
+  op.geom = (const T*)d->geom;
   op.inc2_rp = d->inc2_rp;
   op.inc1_rp = d->inc1_rp;
   op.rp11 = d->rp11;
   op.ci11 = d->ci11;
   op.nnz11 = d->nnz11;
-  op.M1 = d->M1;
-  op.St = d->St;
-  for (int i = 0; i < MPET_MAXF * MPET_MAXF; ++i) op.C[i] = (float)d->coef.C[i];
-  for (int i = 0; i < MPET_MAXF; ++i) op.stiff[i] = (float)d->coef.stiff[i];
-  op.mu = (float)d->coef.mu;
+  op.M1 = (const T*)d->M1;
+  op.St = (const T*)d->St;
+  for (int i = 0; i < MPET_MAXF * MPET_MAXF; ++i) op.C[i] = (T)d->coef.C[i];
+  for (int i = 0; i < MPET_MAXF; ++i) op.stiff[i] = (T)d->coef.stiff[i];
+  op.mu = (T)d->coef.mu;
   op.dmask = d->dmask;
   op.lanes_u = cw_lanes_u(d->lanes_u);
   op.lanes_p = cw_lanes_p(d->lanes_p);

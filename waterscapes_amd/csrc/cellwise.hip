@@ -1,15 +1,18 @@
-// C entry points of the cell-wise (matrix-free) FP32 operator: geometry
-// set-up and the plain application (the fused MINRES step lives in
-// krylov.hip).  The kernels are in cellwise.h.
+// C entry points of the cell-wise (matrix-free) operator, FP32 and FP64:
+// geometry set-up and the plain application (the fused MINRES step and the
+// refinement's residuals live in krylov.hip).  The kernels are in
+// cellwise.h.
 #include "cellwise.h"
 
 namespace mpet {
 
 // One thread per cell: gradients of the barycentric coordinates (rows of
-// J^-1, J's columns v_k - v_0) and |T|, in FP64, stored rounded to FP32.
+// J^-1, J's columns v_k - v_0) and |T|, in FP64, stored as G (FP32:
+// rounded).
+template <typename G>
 __global__ void __launch_bounds__(256)
 cw_geometry_kernel(int nC, const double* __restrict__ X, const int* __restrict__ cells,
-                   float* __restrict__ geom) {
+                   G* __restrict__ geom) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nC) return;
   const int v0 = cells[4 * e], v1 = cells[4 * e + 1], v2 = cells[4 * e + 2],
@@ -37,8 +40,8 @@ cw_geometry_kernel(int nC, const double* __restrict__ X, const int* __restrict__
   Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
   const long long n = nC;
   for (int r = 0; r < 3; ++r)
-    for (int m = 0; m < 3; ++m) geom[(3 * r + m) * n + e] = (float)Ji[r][m];
-  geom[9 * n + e] = (float)(fabs(det) / 6.0);
+    for (int m = 0; m < 3; ++m) geom[(3 * r + m) * n + e] = (G)Ji[r][m];
+  geom[9 * n + e] = (G)(fabs(det) / 6.0);
 }
 
 }  // namespace mpet
@@ -48,23 +51,45 @@ extern "C" int mpet_cell_geometry(const MpetMeshDesc* mesh, float* geom, hipStre
     mpet_set_error("mpet_cell_geometry: invalid arguments");
     return ERR_INVALID;
   }
-  hipLaunchKernelGGL(cw_geometry_kernel, dim3((mesh->nC + 255) / 256), dim3(256), 0, stream,
-                     mesh->nC, mesh->coords, mesh->cells, geom);
+  hipLaunchKernelGGL((cw_geometry_kernel<float>), dim3((mesh->nC + 255) / 256), dim3(256), 0,
+                     stream, mesh->nC, mesh->coords, mesh->cells, geom);
   HIPCHK(hipGetLastError());
   return OK;
 }
 
-extern "C" int mpet_cell_op_apply(const MpetCellOpDesc* d, const float* x, const float* b,
-                                  float* y, float* work, hipStream_t stream) {
-  CellOp op;
-  int rc = cw_from_desc(d, work, op);
+extern "C" int mpet_cell_geometry64(const MpetMeshDesc* mesh, double* geom, hipStream_t stream) {
+  if (!mesh || mesh->nC <= 0 || !mesh->coords || !mesh->cells || !geom) {
+    mpet_set_error("mpet_cell_geometry64: invalid arguments");
+    return ERR_INVALID;
+  }
+  hipLaunchKernelGGL((cw_geometry_kernel<double>), dim3((mesh->nC + 255) / 256), dim3(256), 0,
+                     stream, mesh->nC, mesh->coords, mesh->cells, geom);
+  HIPCHK(hipGetLastError());
+  return OK;
+}
+
+template <typename T>
+static int cell_op_apply_t(const MpetCellOpDesc* d, const void* x, const void* b, void* y,
+                           void* work, hipStream_t stream) {
+  CellOpT<T> op;
+  int rc = cw_from_desc<T>(d, (T*)work, op);
   if (rc) return rc;
   if (!x || !y) {
     mpet_set_error("mpet_cell_op_apply: null vector");
     return ERR_INVALID;
   }
-  CwEpiApply epi{y, b, op.dmask};
-  cw_launch(op, x, epi, (double*)nullptr, (const int*)nullptr, stream);
+  CwEpiApplyT<T> epi{(T*)y, (const T*)b, op.dmask};
+  cw_launch(op, (const T*)x, epi, (double*)nullptr, (const int*)nullptr, stream);
   HIPCHK(hipGetLastError());
   return OK;
+}
+
+extern "C" int mpet_cell_op_apply(const MpetCellOpDesc* d, const void* x, const void* b,
+                                  void* y, void* work, hipStream_t stream) {
+  if (!d) {
+    mpet_set_error("invalid cell-wise operator description");
+    return ERR_INVALID;
+  }
+  return d->fp64 ? cell_op_apply_t<double>(d, x, b, y, work, stream)
+                 : cell_op_apply_t<float>(d, x, b, y, work, stream);
 }

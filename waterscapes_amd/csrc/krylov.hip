@@ -55,12 +55,12 @@ struct MinresState {
 //   y = s (A z) - c1 r1,  v = s z,  returns v . y  (partial of alfa).
 template <typename T>
 struct EpiK1 {
-  T* y; T* v; const T* z; const T* r1; const uint8_t* mask;
+  T* y; T* v; const T* z; const T* r1; const uint8_t* mask;   // v may be null
   __device__ __forceinline__ double operator()(long long i, T ax, T s, T c1) const {
     T yy = (mask && mask[i]) ? (T)0 : s * ax - c1 * r1[i];
     T vv = s * z[i];
     y[i] = yy;
-    v[i] = vv;
+    if (v) v[i] = vv;
     return (double)vv * (double)yy;
   }
 };
@@ -259,6 +259,22 @@ k5_kernel(long long n, const T* __restrict__ v, T* __restrict__ w1, const T* __r
   if (i >= n) return;
   const T oe = (T)st->oldeps, de = (T)st->delta, ig = (T)(1.0 / st->gamma), ph = (T)st->phi;
   T w = (v[i] - oe * w1[i] - de * w2[i]) * ig;
+  w1[i] = w;
+  x[i] += ph * w;
+}
+
+// K5 with v = s z formed here from this iteration's z (s = 1 / oldb: the
+// 1 / beta that K1 scaled z with, bit for bit), so that K1 need not store v.
+template <typename T>
+__global__ void __launch_bounds__(NT)
+k5z_kernel(long long n, const T* __restrict__ z, T* __restrict__ w1, const T* __restrict__ w2,
+           T* __restrict__ x, const MinresState* st, const int* __restrict__ skip) {
+  if (*skip) return;
+  long long i = (long long)blockIdx.x * NT + threadIdx.x;
+  if (i >= n) return;
+  const T sv = (T)(1.0 / st->oldb);
+  const T oe = (T)st->oldeps, de = (T)st->delta, ig = (T)(1.0 / st->gamma), ph = (T)st->phi;
+  T w = (sv * z[i] - oe * w1[i] - de * w2[i]) * ig;
   w1[i] = w;
   x[i] += ph * w;
 }
@@ -490,7 +506,9 @@ template <typename T>
 struct Minres {
   long long n = 0, nu = 0;
   T *R[2] = {nullptr, nullptr}, *W[2] = {nullptr, nullptr};
-  T *z = nullptr, *y = nullptr, *v = nullptr, *x = nullptr, *b = nullptr;
+  // z alternates with z1 between iterations (the preconditioner writes the
+  // next iteration's z while this iteration's is still read by K5)
+  T *z = nullptr, *z1 = nullptr, *y = nullptr, *v = nullptr, *x = nullptr, *b = nullptr;
   MinresState* st = nullptr;
   MinresState* st_host = nullptr;   // pinned
   double* partial = nullptr;        // MAX_PARTIALS
@@ -508,6 +526,11 @@ struct MpetSolver {
   CellOp cw;
   float* cw_work = nullptr;
   long long cw_work_n = 0;
+  // optional cell-wise FP64 operator of the FP64 residuals (op_residual64)
+  bool has_cw64 = false;
+  CellOpT<double> cw64;
+  double* cw64_work = nullptr;
+  long long cw64_work_n = 0;
   // optional scalar CSR (plain FP64 path)
   long long csr_rows = 0;
   const long long* csr_rp = nullptr;
@@ -592,7 +615,8 @@ static int build_minres(MpetSolver* S, Minres<T>& M) {
   int rc;
   if ((rc = alloc_dev(S->allocs, &M.R[0], M.n)) || (rc = alloc_dev(S->allocs, &M.R[1], M.n)) ||
       (rc = alloc_dev(S->allocs, &M.W[0], M.n)) || (rc = alloc_dev(S->allocs, &M.W[1], M.n)) ||
-      (rc = alloc_dev(S->allocs, &M.z, M.n)) || (rc = alloc_dev(S->allocs, &M.y, M.n)) ||
+      (rc = alloc_dev(S->allocs, &M.z, M.n)) || (rc = alloc_dev(S->allocs, &M.z1, M.n)) ||
+      (rc = alloc_dev(S->allocs, &M.y, M.n)) ||
       (rc = alloc_dev(S->allocs, &M.v, M.n)) || (rc = alloc_dev(S->allocs, &M.x, M.n)) ||
       (rc = alloc_dev(S->allocs, &M.b, M.n)) || (rc = alloc_dev(S->allocs, &M.st, 1)) ||
       (rc = alloc_dev(S->allocs, &M.partial, MAX_PARTIALS)))
@@ -650,21 +674,28 @@ static void minres_iter(MpetSolver* S, Minres<T>& M, const Precond<T>& P, bool c
   T* Wb = M.W[(i + 1) & 1];
   const int cur = i & 1, next = (i + 1) & 1;
   const int* skip = &M.st->inactive[cur];
-  EpiK1<T> epi{M.y, M.v, M.z, Ra, nullptr};
+  // this iteration's z (read by K1 and K5) and the next one's (written by
+  // the preconditioner)
+  T* zc = (i & 1) ? M.z1 : M.z;
+  T* zn = (i & 1) ? M.z : M.z1;
+  // v = z / beta is stored for K5 only by the bordered path; otherwise K5
+  // forms it from zc
+  const bool keep_v = csr && S->mb;
+  EpiK1<T> epi{M.y, keep_v ? M.v : nullptr, zc, Ra, nullptr};
   long long n = M.n;
   int eb = (int)((n + NT - 1) / NT);
   int np = 1;
   if constexpr (sizeof(T) == 8) {
     if (csr && S->mb) {
-      launch_csr_spmv<double>(S->csr_rows, S->csr_rp, S->csr_ci, S->csr_v, M.z,
+      launch_csr_spmv<double>(S->csr_rows, S->csr_rp, S->csr_ci, S->csr_v, zc,
                               (const double*)nullptr, (const uint8_t*)nullptr, S->yraw,
                               S->csr_lpr, s);
       const int bg = bgrid(S->n, S->mb);
       hipLaunchKernelGGL(k1b_kernel, dim3(bg), dim3(NT), 0, s, S->n, S->mb, S->B,
-                         (const double*)S->yraw, (const double*)M.z, (const double*)Ra,
+                         (const double*)S->yraw, (const double*)zc, (const double*)Ra,
                          S->opt_mask64, M.y, M.v, M.st, skip, M.partial);
       hipLaunchKernelGGL(k1bf_kernel, dim3(1), dim3(NT), 0, s, M.partial, bg, S->mb, S->n,
-                         (const double*)M.z, (const double*)Ra, M.y, M.v, M.st, skip);
+                         (const double*)zc, (const double*)Ra, M.y, M.v, M.st, skip);
     } else if (csr) {
       epi.mask = S->opt_mask64;
       int lpr = S->csr_lpr;
@@ -674,7 +705,7 @@ static void minres_iter(MpetSolver* S, Minres<T>& M, const Precond<T>& P, bool c
 #define K1CSR(LP)                                                                              \
   case LP:                                                                                     \
     hipLaunchKernelGGL((k1_csr_kernel<double, LP>), dim3(np), dim3(NT), 0, s, S->csr_rows,     \
-                       S->csr_rp, S->csr_ci, S->csr_v, M.z, epi, M.st, skip, M.partial);      \
+                       S->csr_rp, S->csr_ci, S->csr_v, zc, epi, M.st, skip, M.partial);        \
     break;
       switch (lpr) { K1CSR(4) K1CSR(8) K1CSR(16) K1CSR(32) K1CSR(64) default: break; }
 #undef K1CSR
@@ -689,7 +720,7 @@ static void minres_iter(MpetSolver* S, Minres<T>& M, const Precond<T>& P, bool c
 #define K1CSR(LP)                                                                              \
   case LP:                                                                                     \
     hipLaunchKernelGGL((k1_csr_kernel<float, LP>), dim3(np), dim3(NT), 0, s, S->csr_rows,      \
-                       S->csr_rp, S->csr_ci, (const float*)S->csr_v32, M.z, epi, M.st, skip,   \
+                       S->csr_rp, S->csr_ci, (const float*)S->csr_v32, zc, epi, M.st, skip,     \
                        M.partial);                                                             \
     break;
       switch (lpr) { K1CSR(4) K1CSR(8) K1CSR(16) K1CSR(32) K1CSR(64) default: break; }
@@ -699,8 +730,8 @@ static void minres_iter(MpetSolver* S, Minres<T>& M, const Precond<T>& P, bool c
   bool cw_done = false;
   if constexpr (sizeof(T) == 4) {
     if (!csr && S->has_cw) {
-      CwEpiK1 cepi{EpiK1<float>{M.y, M.v, M.z, Ra, S->cw.dmask}, M.st, skip};
-      np = cw_launch(S->cw, M.z, cepi, M.partial, skip, s);
+      CwEpiK1 cepi{EpiK1<float>{M.y, nullptr, zc, Ra, S->cw.dmask}, M.st, skip};
+      np = cw_launch(S->cw, zc, cepi, M.partial, skip, s);
       cw_done = true;
     }
   }
@@ -711,7 +742,7 @@ static void minres_iter(MpetSolver* S, Minres<T>& M, const Precond<T>& P, bool c
     const int gu = k1_lanes(op->lanes_u, 3), gp = k1_lanes(op->lanes_p, S->F);
     long long ntiles = (op->nP2 + NT / gu - 1) / (NT / gu) + (op->nP1 + NT / gp - 1) / (NT / gp);
     np = grid_for(ntiles, 1, MAX_PARTIALS);
-    launch_k1_op<T>(S->F, gu, gp, dim3(np), s, *op, M.z, epi, M.st, skip, M.partial);
+    launch_k1_op<T>(S->F, gu, gp, dim3(np), s, *op, zc, epi, M.st, skip, M.partial);
   }
   if (!(csr && S->mb))
     hipLaunchKernelGGL(k1f_kernel, dim3(1), dim3(NT), 0, s, M.partial, np, M.st, skip);
@@ -721,15 +752,20 @@ static void minres_iter(MpetSolver* S, Minres<T>& M, const Precond<T>& P, bool c
     const K2In<T> k2{M.y, Rb, Ra, &M.st->alfa, &M.st->beta};
     Skip sk;
     sk.all = skip;
-    precond_apply(P, Ra, M.z, sk, s, &k2);
+    precond_apply(P, Ra, zn, sk, s, &k2);
   } else {
     hipLaunchKernelGGL((k2_kernel<T>), dim3(eb), dim3(NT), 0, s, n, M.y, Rb, Ra, M.st, skip);
-    apply_M(S, P, Ra, M.z, skip, s);
+    apply_M(S, P, Ra, zn, skip, s);
   }
   int dg = dot_grid(n);
-  hipLaunchKernelGGL((dot_kernel<T>), dim3(dg), dim3(NT), 0, s, n, Ra, M.z, skip, M.partial);
+  hipLaunchKernelGGL((dot_kernel<T>), dim3(dg), dim3(NT), 0, s, n, Ra, zn, skip, M.partial);
   hipLaunchKernelGGL(k4f_kernel, dim3(1), dim3(NT), 0, s, M.partial, dg, M.st, cur, next);
-  hipLaunchKernelGGL((k5_kernel<T>), dim3(eb), dim3(NT), 0, s, n, M.v, Wa, Wb, M.x, M.st, skip);
+  if (keep_v)
+    hipLaunchKernelGGL((k5_kernel<T>), dim3(eb), dim3(NT), 0, s, n, M.v, Wa, Wb, M.x, M.st,
+                       skip);
+  else
+    hipLaunchKernelGGL((k5z_kernel<T>), dim3(eb), dim3(NT), 0, s, n, (const T*)zc, Wa, Wb, M.x,
+                       M.st, skip);
 }
 
 // MINRES start: r1 = r2 = b, z = M^-1 b, beta1 = sqrt(b . z).  Returns beta1
@@ -805,6 +841,9 @@ static void op_residual64(MpetSolver* S, const double* b, const double* x, doubl
   if (S->csr_rp && (S->csr_only || (S->opt.mode == 0 && S->opt.use_csr)))
     launch_csr_spmv<double>(S->csr_rows, S->csr_rp, S->csr_ci, S->csr_v, x, b, S->opt_mask64,
                             r, S->csr_lpr, s);
+  else if (S->has_cw64)
+    cw_launch(S->cw64, x, CwEpiApplyT<double>{r, b, S->cw64.dmask}, (double*)nullptr,
+              (const int*)nullptr, s);
   else
     launch_op_apply<double>(S->op64, x, b, r, s);
   if (S->mb) {
@@ -1272,10 +1311,45 @@ extern "C" int mpet_solver_set_cell_operator(void* h, const MpetCellOpDesc* op) 
     S->cw_work_n = need;
   }
   CellOp c;
-  int rc = cw_from_desc(op, S->cw_work, c);
+  int rc = cw_from_desc<float>(op, S->cw_work, c);
   if (rc) return rc;
   S->cw = c;
   S->has_cw = true;
+  return OK;
+}
+
+// FP64 residuals through the cell-wise FP64 operator (op != NULL) or back
+// through the structured FP64 operator (op == NULL).  The residuals run
+// outside the captured MINRES graphs, so no graph is dropped.
+extern "C" int mpet_solver_set_cell_operator64(void* h, const MpetCellOpDesc* op) {
+  MpetSolver* S = (MpetSolver*)h;
+  if (!S || S->csr_only || S->mb) {
+    mpet_set_error("mpet_solver_set_cell_operator64: needs a solver on the structured "
+                   "operator");
+    return ERR_INVALID;
+  }
+  HIPCHK(hipDeviceSynchronize());
+  if (!op) {
+    S->has_cw64 = false;
+    return OK;
+  }
+  if (!op->fp64 || op->nP2 != S->op64.nP2 || op->nP1 != S->op64.nP1 || op->A != S->op64.A ||
+      op->nnz11 != S->op64.nnz11 || op->npart2 <= 0 || op->npart1 <= 0) {
+    mpet_set_error("mpet_solver_set_cell_operator64: not an FP64 description of the "
+                   "solver's operator");
+    return ERR_INVALID;
+  }
+  const long long need = cw_work_floats(op);
+  if (S->cw64_work_n < need) {
+    int rc = alloc_dev(S->allocs, &S->cw64_work, (size_t)need);
+    if (rc) return rc;
+    S->cw64_work_n = need;
+  }
+  CellOpT<double> c;
+  int rc = cw_from_desc<double>(op, S->cw64_work, c);
+  if (rc) return rc;
+  S->cw64 = c;
+  S->has_cw64 = true;
   return OK;
 }
 

@@ -266,34 +266,51 @@ class DeviceSystem:
         return self._cw
 
     def cell_work_floats(self):
+        """Work values of the cell-wise operator (its partials), either precision."""
         cw = self.cell_op()
         return 3 * cw["p2"]["npart"] + cw["p1"]["npart"]
 
-    def cell_op_desc(self, masked=True, lanes=None, xcd_chunk=None):
-        """MpetCellOpDesc of the cell-wise FP32 operator (the P1 x P1 block
-        from the FP32 structured copy)."""
-        if self.op32 is None:
-            raise ValueError("the cell-wise operator needs the FP32 structured copy")
+    def cell_geom64(self):
+        """FP64 per-cell geometry of the FP64 cell-wise operator, built once."""
+        cw = self.cell_op()
+        if "geom64" not in cw:
+            g = torch.empty(10 * cw["nC"], dtype=torch.float64, device=self.device)
+            check(_lib.lib().mpet_cell_geometry64(C.byref(self.mesh_desc), ptr(g),
+                                                  _lib.stream_ptr()))
+            cw["geom64"] = g
+        return cw["geom64"]
+
+    def cell_op_desc(self, masked=True, lanes=None, xcd_chunk=None, fp64=False):
+        """MpetCellOpDesc of the cell-wise operator: FP32 (the P1 x P1 block
+        from the FP32 structured copy) or FP64 (FP64 geometry, the P1 x P1
+        block from the FP64 primitives)."""
+        if fp64:
+            v, geom = self.op64, self.cell_geom64()
+        else:
+            if self.op32 is None:
+                raise ValueError("the cell-wise operator needs the FP32 structured copy")
+            v, geom = self.op32, self.cell_op()["geom"]
         cw = self.cell_op()
         t2, t1 = cw["p2"], cw["p1"]
-        v = self.op32
         lu, lp = lanes or self.cell_lanes
         xc = self.cell_xcd_chunk if xcd_chunk is None else xcd_chunk
         p11 = self.pat_dev["11"]
         return _lib.CellOpDesc(cw["nC"], self.nP2, self.nP1, self.A, ptr(t2["node"]),
-                               ptr(t1["node"]), ptr(cw["geom"]), self.CELLS_PER_BLOCK,
+                               ptr(t1["node"]), ptr(geom), self.CELLS_PER_BLOCK,
                                t2["npart"], t1["npart"], ptr(t2["bq"]), ptr(t2["lrp"]),
                                ptr(t2["pe"]), ptr(t2["lsl"]), ptr(t1["bq"]), ptr(t1["lrp"]),
                                ptr(t1["pe"]), ptr(t1["lsl"]), ptr(t2["rp"]), ptr(t1["rp"]),
                                ptr(p11[0]), ptr(p11[1]), self.nnz["11"], ptr(v["M1"]),
                                ptr(v["St"]), _lib.make_coef(self.Cm, self.Cs, self.mu),
-                               ptr(self.dmask) if masked else None, lu, lp, xc)
+                               ptr(self.dmask) if masked else None, lu, lp, xc, int(fp64))
 
     def cell_apply(self, x32, b=None, masked=True, lanes=None):
-        """y = A x (or b - A x) in FP32 through the cell-wise operator."""
+        """y = A x (or b - A x) through the cell-wise operator, in the
+        precision of x (FP32 or FP64)."""
         y = torch.empty_like(x32)
-        work = torch.empty(self.cell_work_floats(), dtype=torch.float32, device=self.device)
-        d = self.cell_op_desc(masked, lanes)
+        fp64 = x32.dtype == torch.float64
+        work = torch.empty(self.cell_work_floats(), dtype=x32.dtype, device=self.device)
+        d = self.cell_op_desc(masked, lanes, fp64=fp64)
         check(_lib.lib().mpet_cell_op_apply(C.byref(d), ptr(x32), ptr(b), ptr(y), ptr(work),
                                             _lib.stream_ptr()))
         return y

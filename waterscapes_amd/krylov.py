@@ -82,6 +82,8 @@ def _bind():
     L.mpet_solver_solve.restype = C.c_int
     L.mpet_solver_set_cell_operator.argtypes = [P, P]
     L.mpet_solver_set_cell_operator.restype = C.c_int
+    L.mpet_solver_set_cell_operator64.argtypes = [P, P]
+    L.mpet_solver_set_cell_operator64.restype = C.c_int
     L.mpet_precond_apply.argtypes = [P, C.c_int, P, P, P]
     L.mpet_precond_apply.restype = C.c_int
     return L
@@ -133,6 +135,10 @@ class SolverConfig:
     # inner FP32 MINRES of the mixed mode on the cell-wise (matrix-free)
     # operator instead of the FP32 copy of the assembled one
     cell_operator: bool = True
+    # the mixed mode's FP64 residuals (refinement) through the cell-wise
+    # FP64 operator (same formulas, FP64 geometry) instead of the assembled
+    # FP64 values: 2.6 GB fewer bytes per residual at 48^3
+    cell_operator64: bool = True
 
 
 def set_level_degrees(h, params: AmgParams):
@@ -318,9 +324,13 @@ class KrylovSolver:
         self.handle = h
         self.last_stats = None
         self.cell_desc = None
+        self.cell_desc64 = None
         if mixed and cfg.cell_operator:
             self.cell_desc = system.cell_op_desc()
             check(L.mpet_solver_set_cell_operator(h, C.byref(self.cell_desc)))
+            if cfg.cell_operator64:
+                self.cell_desc64 = system.cell_op_desc(fp64=True)
+                check(L.mpet_solver_set_cell_operator64(h, C.byref(self.cell_desc64)))
         self.tuning = {}
         tune = cfg.autotune if cfg.autotune is not None else system.n >= cfg.autotune_min_dofs
         if tune:
@@ -370,7 +380,14 @@ class KrylovSolver:
             self.cell_desc = s.cell_op_desc(lanes=(cell[0], cell[1]), xcd_chunk=cell[2])
             check(_bind().mpet_solver_set_cell_operator(self.handle, C.byref(self.cell_desc)))
             self.tuning["cell"] = list(cell)
-        if self.prec32 is not None and 1 in self.tuning and self.opt.mode in (1, 2):
+            if self.cell_desc64 is not None:
+                # the FP64 residuals: the same tables and launch shapes
+                self.cell_desc64 = s.cell_op_desc(lanes=(cell[0], cell[1]), xcd_chunk=cell[2],
+                                                  fp64=True)
+                check(_bind().mpet_solver_set_cell_operator64(self.handle,
+                                                              C.byref(self.cell_desc64)))
+        if self.prec32 is not None and 1 in self.tuning and self.opt.mode in (1, 2) and \
+                self.cell_desc64 is None:
             # the FP64 residual of the refinement runs the FP64 operator
             op64 = at.tune_operator(s, 0, cache)
             at.apply_tuning(self.handle, 0, op=op64)
