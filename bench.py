@@ -194,24 +194,26 @@ def _compact_p(kr, mixed):
 
 def iteration_bytes(eng, mixed):
     """Algorithmic HBM bytes of one MINRES iteration of the engine's solve
-    (krylov.hip minres_iter): K1 (operator apply with the fused MINRES
-    step), the block preconditioner (displacement and pressure V-cycles, the
-    pressure gather / scatter) and the BLAS-1 kernels (k2 3n, dot 2n, k5 6n
-    vector accesses).  Returns (bytes, parts)."""
+    (krylov.hip minres_iter): K1 (operator apply with the fused MINRES step:
+    + r1 read; v is not stored), the block preconditioner (displacement and
+    pressure V-cycles, the pressure gather / scatter) and the BLAS-1 work
+    (the K2 update y, r_old -> r made inside the preconditioner's first
+    kernels, 3n, less the n those kernels no longer read; dot 2n; k5 6n).
+    Returns (bytes, parts)."""
     sysm = eng.system
     kr = eng.krylov
     s = 4 if mixed else 8
     n = sysm.n
     if mixed and getattr(kr, "cell_desc", None) is not None:
-        k1 = cell_op_bytes(sysm) + 2 * n * s
+        k1 = cell_op_bytes(sysm) + n * s
     elif mixed or kr.csr_desc is None:
-        k1 = op_bytes(sysm, 1 if mixed else 0) + 2 * n * s
+        k1 = op_bytes(sysm, 1 if mixed else 0) + n * s
     else:
-        k1 = csr_bytes(sysm) + 2 * n * s
+        k1 = csr_bytes(sysm) + n * s
     vu = vcycle_bytes(kr.levels, kr.cfg.amg, 3, s, compact=_compact_u(kr, mixed), scaled=True)
     vp = vcycle_bytes(kr.plevels, kr.cfg.pamg, 1, s,
                       compact=_compact_p(kr, mixed)) + 4 * sysm.F * sysm.nP1 * s
-    blas = 11 * n * s
+    blas = 10 * n * s
     parts = {"k1": int(k1), "vcycle_u": int(vu), "vcycle_p": int(vp), "blas1": int(blas)}
     return int(k1 + vu + vp + blas), parts
 
