@@ -376,7 +376,8 @@ int mpet_op_apply(const MpetOpDesc* op, const void* x, const void* b, void* y,
  *   pe[k]                the partial at entry k of the node-sorted list
  *   inc2_rp / inc1_rp    node -> first entry of its partials (n + 1)
  * Work: 3*npart2 + npart1 values of the descriptor's precision.  lanes_u
- * in {4, 8, 16}, lanes_p in {8, 16} (others: 8).
+ * in {1, 4, 8, 16} (1: one thread per P2 node), lanes_p in {8, 16}
+ * (others: 8).
  * The reference always multiplies by the assembled PETSc matrix (:437). */
 typedef struct {
   int32_t nC, nP2, nP1, A;

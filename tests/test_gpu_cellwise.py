@@ -81,9 +81,10 @@ def test_cell_apply_matches_fp64_operator(kind, A):
     ref_r = dev.apply(x, b=b, dtype=0)
     y_r = dev.cell_apply(x.float(), b=b.float())
     _blockwise_close(space, y_r, ref_r, FP32_RTOL)
-    for lanes in ((4, 8), (16, 16)):
+    for lanes in ((1, 8), (4, 8), (16, 16)):
         y2 = dev.cell_apply(x.float(), lanes=lanes)
         _blockwise_close(space, y2, ref, FP32_RTOL)
+        _blockwise_close(space, dev.cell_apply(x, lanes=lanes), ref, FP64_RTOL)
     y_again = dev.cell_apply(x.float())
     assert torch.equal(y, y_again)
     # the FP64 form

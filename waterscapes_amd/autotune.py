@@ -40,7 +40,7 @@ from ._lib import check, ptr
 
 ARCH = "gfx950"
 LANES_OP = (8, 16, 32)
-LANES_CELL_U = (4, 8, 16)
+LANES_CELL_U = (1, 4, 8, 16)
 LANES_CELL_P = (8, 16)
 XCD_CHUNKS = (0, 4)
 LPR_ALL = (1, 2, 4, 8, 16, 32, 64)

@@ -235,7 +235,8 @@ struct CwEpiApplyT {
 };
 using CwEpiApply = CwEpiApplyT<float>;
 
-// Phase 2: GU lanes per P2 node row, GP lanes per P1 node row, grid-stride
+// Phase 2: GU lanes per P2 node row (GU = 1: one thread per node, which
+// writes all three rows), GP lanes per P1 node row, grid-stride
 // over row tiles (u tiles first), XCD-aware tile order.  The epilogue's
 // per-row results are summed per block into partial[blockIdx.x] (when given).
 template <int F, int GU, int GP, typename Epi, typename T>
@@ -278,7 +279,11 @@ cw_rows_kernel(CellOpT<T> op, const T* __restrict__ x, Epi epi, double* __restri
           y1 += __shfl_xor(y1, off, 64);
           y2 += __shfl_xor(y2, off, 64);
         }
-        if (lane < 3) {
+        if constexpr (GU == 1) {           // one thread per row: all three rows
+          acc += epi(3ll * a, y0);
+          acc += epi(3ll * a + 1, y1);
+          acc += epi(3ll * a + 2, y2);
+        } else if (lane < 3) {
           const T yc = lane == 0 ? y0 : (lane == 1 ? y1 : y2);
           acc += epi(3ll * a + lane, yc);
         }
@@ -355,12 +360,13 @@ template <typename Epi, int F, typename T>
 static void cw_rows_f(const CellOpT<T>& op, dim3 grid, hipStream_t s, const T* x, Epi epi,
                       double* partial) {
   switch (op.lanes_u) {
+    case 1: cw_rows_gp<Epi, F, 1>(op, op.lanes_p, grid, s, x, epi, partial); break;
     case 4: cw_rows_gp<Epi, F, 4>(op, op.lanes_p, grid, s, x, epi, partial); break;
     case 16: cw_rows_gp<Epi, F, 16>(op, op.lanes_p, grid, s, x, epi, partial); break;
     default: cw_rows_gp<Epi, F, 8>(op, op.lanes_p, grid, s, x, epi, partial); break;
   }
 }
-static inline int cw_lanes_u(int g) { return (g == 4 || g == 8 || g == 16) ? g : 8; }
+static inline int cw_lanes_u(int g) { return (g == 1 || g == 4 || g == 8 || g == 16) ? g : 8; }
 static inline int cw_lanes_p(int g) { return (g == 8 || g == 16) ? g : 8; }
 template <typename T>
 static inline int cw_row_tiles(const CellOpT<T>& op) {
