@@ -4,6 +4,8 @@
     c 1e-4, K 1e-3 / 1e-4, S_12 = 1e-2), theta = 1, dt = 0.01, 10 steps of
     the manufactured solution, Dirichlet everywhere: fp64, mixed and direct
     paths against the oracle's LU at every step (1e-8 relative L2 per field);
+  * the benchmark's own mixed configuration at 8^3 against the oracle's LU
+    at every step (1e-8);
   * cfg4: the seed-11 batch of 96 uneven instances (N 8..32, A 1..4) stepped
     20 times in mixed precision: operator symmetry per instance (sampled),
     a sample of the smallest instances against the oracle at every step, and
@@ -49,6 +51,56 @@ def test_cfg1_exact_workload_matches_oracle(precision, direct, extra):
         assert max(errs) <= RTOL, (precision, direct, k, errs)
         k += 1
     assert k == len(ref) == 10
+
+
+# ---------------------------------------------------------------------------
+# the headline configuration at the largest size the oracle's LU solves in
+# seconds
+# ---------------------------------------------------------------------------
+def _headline_oracle(solver, A, steps, seed=2024):
+    """The oracle's direct solve of bench.make_solver's workload (u = 0 on
+    z = 0, p_j = P0_j + 5 sin(2 pi t) on z = 1, log-normal per-cell K)."""
+    space = solver.up_.function_space()
+    mesh = space.mesh
+    m = oracle_mesh(space)
+    params = bench.workload_params(A)
+    Kc = bench.lognormal_cells(mesh.num_cells(), A, seed)
+    mid = mesh.facet_midpoints()
+    inv = np.uint64(2**63 - 1)
+    mk_u = np.full(mesh.num_facets(), inv, dtype=np.uint64)
+    mk_u[mesh.exterior & (np.abs(mid[:, 2]) < 1e-12)] = 0
+    mk_p = np.full(mesh.num_facets(), inv, dtype=np.uint64)
+    mk_p[mesh.exterior & (np.abs(mid[:, 2] - 1.0) < 1e-12)] = 0
+    pb = [lambda X, t, j=j: np.full(X.shape[0], bench.P0[j] + 5.0 * np.sin(2.0 * np.pi * t))
+          for j in range(A)]
+    prob = O.OracleProblem(m, params, dict(dt=0.0125, theta=1.0, T=steps * 0.0125),
+                           u_bar=lambda X, t: np.zeros((X.shape[0], 3)), p_bar=pb, K_cells=Kc,
+                           momentum_markers=mk_u, continuity_markers=[mk_p] * A)
+    up0 = np.zeros(prob.lay.n)
+    return [x.copy() for x, t in prob.solve_direct(up0)][:steps]
+
+
+@pytest.mark.timeout(600)
+def test_headline_configuration_at_8cube_matches_oracle():
+    """bench.py's exact mixed configuration (E = 1500, nu = 0.479,
+    log-normal K, adjacent exchange, u = 0 bottom, oscillating top
+    pressures; quartic extrapolated guess, outer 3e-11, inner 1e-6, the
+    cell-wise FP32 and FP64 operators) at 8^3 against the oracle's direct
+    solve at each of 6 steps: <= 1e-8 relative L2 per field."""
+    steps = 6
+    solver = bench.make_solver(8, 4, "mixed", extrapolate_order=4, outer_tol=3e-11,
+                               inner_rtol=1e-6)
+    solver.params["T"] = steps * 0.0125
+    kr = solver.engine().krylov
+    assert kr.cell_desc is not None and kr.cell_desc64 is not None
+    ref = _headline_oracle(solver, 4, steps)
+    space = solver.up_.function_space()
+    k = 0
+    for up, t in solver.solve():
+        errs = field_errors(space, up.vector().get_local(), ref[k])
+        assert max(errs) <= 1e-8, (k, errs, solver.solver_monitor["niter"])
+        k += 1
+    assert k == steps
 
 
 # ---------------------------------------------------------------------------
