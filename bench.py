@@ -1177,6 +1177,35 @@ def main():
             "achieved": round(ach_s, 1), "frac": round(ach_s / 8000.0, 4),
             "traffic": traffic_of("op_apply_48cube_A4_traffic.json"),
             "bytes_per_launch": int(bytes_s), "ms_per_launch": round(ms_s, 4)}
+    # the finest displacement level's Chebyshev step (the top kernel by time
+    # in the V-cycles): half values and 16-bit offsets, three columns, the
+    # solver's tuned lanes; bytes as vcycle_bytes counts a middle step
+    # (direction read and written)
+    kr = eng.krylov
+    if getattr(kr, "compact", None) is not None:
+        from waterscapes_amd import autotune as at
+        (ulevs, _), _p = kr._level_arrays(1)
+        ld = ulevs[0]
+        n0 = int(kr.levels[0]["n"])
+        nnz0 = int(ld["A_ci"].numel())
+        lpr0 = int(kr.tuning.get(1, {}).get("u_levels", [[4]])[0][0] or 4)
+        xs = torch.randn(3 * n0, device=x32.device, dtype=torch.float32)
+        bs = torch.randn_like(xs)
+        ys = torch.empty_like(xs)
+        ds = torch.zeros_like(xs)
+        Lt = at._bind()
+        ms_v = time_kernel(lambda: Lt.mpet_level_spmm(
+            1, 3, 3, n0, _lib.ptr(ld["A_rp"]), _lib.ptr(ld["A_ci"]), None, _lib.ptr(xs),
+            _lib.ptr(bs), _lib.ptr(ys), _lib.ptr(ld["dinv"]), _lib.ptr(ds), 0.5, 0.5,
+            _lib.ptr(kr.mask0), lpr0, _lib.ptr(ld["A_v16"]), _lib.ptr(ld["A_dci"]),
+            float(ld["vscale"]), sp))
+        bytes_v = nnz0 * 4 + (n0 + 1) * 4 + 3 * n0 * 4 + 4 * 3 * n0 * 4 + n0 * 4
+        ach_v = bytes_v / (ms_v * 1e-3) / 1e9
+        out["roofline_vcycle_level0"] = {
+            "kernel": f"spmm_kernel<float,3,{lpr0},3,true> (P2-level Chebyshev step of the "
+                      f"displacement V-cycle, half values, {nnz0} nonzeros x 3 columns)",
+            "achieved": round(ach_v, 1), "frac": round(ach_v / 8000.0, 4),
+            "bytes_per_launch": int(bytes_v), "ms_per_launch": round(ms_v, 4)}
     if not args.no_cpu_baseline and fp64_res is not None:
         # the plain FP64 path on the host at the full workload, a bounded
         # sample of MINRES iterations scaled by the iterations per step
