@@ -387,10 +387,15 @@ def main_batch(args):
     """Uneven batch: `--instances` instances stepped together (mixed), vs the
     same instances one at a time with the plain FP64 GPU path (timed on a
     random sample of `--sample` instances, scaled to the batch)."""
+    print(json.dumps(run_batch(args, args.instances, args.steps, args.warmup, args.sample)))
+
+
+def run_batch(args, instances, steps, warmup, sample):
+    """The uneven-batch measurement of main_batch; returns its JSON object."""
     import torch
     from waterscapes_amd.batch import MPETBatch
     rng = np.random.default_rng(args.seed)
-    specs = [uneven_instance(rng) for _ in range(args.instances)]
+    specs = [uneven_instance(rng) for _ in range(instances)]
     t0 = time.perf_counter()
     solvers = [make_uneven_solver(N, p) for N, p in specs]
     batch = MPETBatch(solvers, precision="mixed", outer_tol=args.outer_tol,
@@ -400,19 +405,19 @@ def main_batch(args):
     ndofs = sum(s.up.data.numel() for s in solvers)
     print(f"[bench] batch of {len(solvers)} ({ndofs} DOFs) set up in {setup:.1f}s",
           file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         batch.step()
         batch.commit()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     iters = []
-    for _ in range(args.steps):
+    for _ in range(steps):
         st = batch.step()
         batch.commit()
         iters.append(max(int(st[k].inner_iterations) for k in range(batch.K)))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t1
-    inst_steps = batch.K * args.steps
+    inst_steps = batch.K * steps
     mem = torch.cuda.max_memory_allocated()
     tuning = batch.tuning
     batch.close()
@@ -421,23 +426,23 @@ def main_batch(args):
     gc.collect()
     torch.cuda.empty_cache()
     # plain FP64 per-instance path on a sample
-    idx = rng.choice(len(specs), size=min(args.sample, len(specs)), replace=False)
+    idx = rng.choice(len(specs), size=min(sample, len(specs)), replace=False)
     seq_time, seq_steps = 0.0, 0
     for i in idx:
         N, p = specs[i]
         s = make_uneven_solver(N, p, precision="fp64")
         s.engine()
-        tot = run_steps(s, args.warmup, args.steps)[0]
+        tot = run_steps(s, warmup, steps)[0]
         seq_time += tot
-        seq_steps += args.steps
+        seq_steps += steps
         del s
         gc.collect()
     seq_rate = seq_steps / seq_time
     rate = inst_steps / wall
     out = {
         "metric": "uneven batch: instance-steps/s (mixed, batched) vs plain FP64 one at a time",
-        "value": round(rate, 4), "unit": "instance-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(1e3 * wall / args.steps, 3),
+        "value": round(rate, 4), "unit": "instance-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": round(1e3 * wall / steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32-in-fp64", "data": "synthetic",
         "config": {"workload": f"{len(specs)} uneven MPET instances (N 8..32, A 1..4), "
@@ -452,7 +457,7 @@ def main_batch(args):
                                "sample": [int(i) for i in idx]},
         "speedup_vs_fp64_one_at_a_time": round(rate / seq_rate, 3),
     }
-    print(json.dumps(out))
+    return out
 
 
 def serve_specs(seed, nreq, steps_scale):
@@ -943,6 +948,11 @@ def main():
     # the whole configuration from t = 0 (80 steps of dt = 0.0125) on the
     # mixed path after the timed window, reported beside it (0: off)
     ap.add_argument("--full-steps", type=int, default=80)
+    # the 96-instance uneven batch (20 steps each, the plain FP64 path one
+    # instance at a time on a sample of 8), also reported in the default
+    # line (0: off)
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-sample", type=int, default=8)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-reference", action="store_true")
     ap.add_argument("--tf-n", type=int, default=32)
@@ -1222,6 +1232,14 @@ def main():
                       f"{its:.1f} iterations per step (the GPU FP64 path's mean at the same "
                       f"rtol 1e-10); single-threaded sparse products, "
                       f"{len(os.sched_getaffinity(0))} threads visible"}
+    if args.batch_steps > 0 and not args.skip_fp64:
+        bt = run_batch(args, 96, args.batch_steps, 2, args.batch_sample)
+        out["batch"] = {k: bt[k] for k in ("metric", "value", "unit", "steps", "ms_per_step",
+                                           "dof_steps_per_s", "setup_s", "fp64_one_at_a_time",
+                                           "speedup_vs_fp64_one_at_a_time")}
+        out["batch"]["config"] = bt["config"]["workload"]
+        print(f"[bench] batch of 96: {bt['value']} instance-steps/s, "
+              f"{bt['speedup_vs_fp64_one_at_a_time']}x", file=sys.stderr, flush=True)
     print(json.dumps(out))
 
 if __name__ == "__main__":
