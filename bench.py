@@ -941,7 +941,10 @@ def main():
     # 48^3 against 197.8 ms at 1e-5, ~200 ms at 1e-4, 201-206 ms at 1e-7 /
     # 3e-7; the 96-instance batch 62.5 instance-steps/s against 60.3 at 1e-4)
     ap.add_argument("--inner-rtol", type=float, default=None)
-    ap.add_argument("--ex-order", type=int, default=4)
+    # order of the time extrapolation of the initial guess, both mixed and
+    # best-FP64 legs (48^3: 4 -> 12.85, 5 -> 13.14, 6 -> 13.18 steps/s over
+    # the timed window; 13.65, 14.87, 15.79 over the whole 80-step run)
+    ap.add_argument("--ex-order", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=8)
     ap.add_argument("--cpu-iters", type=int, default=4)

@@ -84,11 +84,11 @@ def _headline_oracle(solver, A, steps, seed=2024):
 def test_headline_configuration_at_8cube_matches_oracle():
     """bench.py's exact mixed configuration (E = 1500, nu = 0.479,
     log-normal K, adjacent exchange, u = 0 bottom, oscillating top
-    pressures; quartic extrapolated guess, outer 3e-11, inner 1e-6, the
+    pressures; extrapolated guess of order 6, outer 3e-11, inner 1e-6, the
     cell-wise FP32 and FP64 operators) at 8^3 against the oracle's direct
     solve at each of 6 steps: <= 1e-8 relative L2 per field."""
     steps = 6
-    solver = bench.make_solver(8, 4, "mixed", extrapolate_order=4, outer_tol=3e-11,
+    solver = bench.make_solver(8, 4, "mixed", extrapolate_order=6, outer_tol=3e-11,
                                inner_rtol=1e-6)
     solver.params["T"] = steps * 0.0125
     kr = solver.engine().krylov
