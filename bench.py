@@ -945,6 +945,9 @@ def main():
     # best-FP64 legs (48^3: 4 -> 12.85, 5 -> 13.14, 6 -> 13.18 steps/s over
     # the timed window; 13.65, 14.87, 15.79 over the whole 80-step run)
     ap.add_argument("--ex-order", type=int, default=6)
+    # each step extrapolates with the degree (up to --ex-order) that best
+    # predicted the previous step (--no-ex-adaptive: always the full order)
+    ap.add_argument("--no-ex-adaptive", dest="ex_adaptive", action="store_false")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=8)
     ap.add_argument("--cpu-iters", type=int, default=4)
@@ -997,6 +1000,7 @@ def main():
     K, W = args.steps, args.warmup
     t_setup = time.perf_counter()
     mixed = make_solver(args.n, args.A, "mixed", extrapolate_order=args.ex_order,
+                        extrapolate_adaptive=args.ex_adaptive,
                         outer_tol=args.outer_tol,
                         inner_rtol=args.inner_rtol)
     mixed.engine()
@@ -1024,7 +1028,8 @@ def main():
         # extrapolated initial guess, and a tolerance that meets the 1e-8
         # error bar
         best = make_solver(args.n, args.A, "fp64", krylov_rtol=1e-11, extrapolate=True,
-                           extrapolate_order=args.ex_order, fp64_operator="structured",
+                           extrapolate_order=args.ex_order,
+                           extrapolate_adaptive=args.ex_adaptive, fp64_operator="structured",
                            use_graphs=True, check_every=10)
         best.engine()
         best_res = run_steps(best, W, K, keep=True)
@@ -1032,6 +1037,7 @@ def main():
               file=sys.stderr, flush=True)
         best_eng = best.engine()
     mix_res = run_steps(mixed, W, K, keep=True)
+    ex_orders = list(mixed.engine().ex_used[-K:]) if args.ex_adaptive else None
     print(f"[bench] mixed {K} steps {mix_res[0]:.2f}s inner {mix_res[3]} outer {mix_res[4]}",
           file=sys.stderr, flush=True)
     eng = mixed.engine()
@@ -1133,6 +1139,7 @@ def main():
                        "p99": round(float(np.percentile(lat, 99)), 3)},
         "inner_iterations": mix_res[3],
         "outer_iterations": mix_res[4],
+        "extrapolation_orders": ex_orders,
         "max_rel_l2_err_vs_fp64_reference": max(errs) if errs else None,
         "hbm": bytes_report(eng, mix_res, True),
         "full_run": full,
@@ -1155,7 +1162,8 @@ def main():
                 "iterations": best_res[3], "rtol": 1e-11,
                 "operator": "structured node-pattern operator (FP64), hipGraph chunks, "
                             "host check every 10 iterations",
-                "initial_guess": f"extrapolation of order {args.ex_order} (as the mixed path)",
+                "initial_guess": f"extrapolation of order {args.ex_order}"
+                                 f"{' (adaptive)' if args.ex_adaptive else ''} (as the mixed path)",
                 "max_rel_l2_err_vs_fp64_reference": max(errs_best) if errs_best else None,
                 "hbm": bytes_report(best_eng, best_res, False)}
             out["speedup_mixed_vs_fp64_best"] = round(mix_sps / fb_sps, 3)

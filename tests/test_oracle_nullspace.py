@@ -220,14 +220,15 @@ def test_new_entry_points_reject_invalid_arguments():
     assert h.value is None
 
 
-@pytest.mark.parametrize("order", [1, 2, 3, 4])
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5, 6])
 def test_extrapolated_guess_is_exact_for_polynomials(order):
     """Host logic of the initial guess (StepEngine, MPETBatch): with the
     history filled, the guess reproduces states that are polynomials of
     degree `order` in time exactly; before that it uses the states it has."""
     import torch
     from waterscapes_amd.solver import extrapolated_guess, push_history
-    coef = torch.tensor([[0.3, -1.0, 2.0, 0.5, -0.25][: order + 1]], dtype=torch.float64)
+    coef = torch.tensor([[0.3, -1.0, 2.0, 0.5, -0.25, 0.125, -0.0625][: order + 1]],
+                        dtype=torch.float64)
 
     def state(t):
         return (coef * torch.tensor([t ** k for k in range(order + 1)],
@@ -244,3 +245,34 @@ def test_extrapolated_guess_is_exact_for_polynomials(order):
         push_history(hist, order, state(n))
         assert len(hist) == min(n + 1, order)
         assert torch.equal(hist[0], state(n))
+
+
+def test_adaptive_order_picks_the_degree_that_predicted_best():
+    """Host logic of the adaptive initial guess: on states that are a
+    quadratic in time the degree picked reproduces the last state (error at
+    rounding level); just after a kink in time (a start-up transient) the
+    linear guess, which reaches back least, is picked over the high
+    degrees."""
+    import torch
+    from waterscapes_amd.solver import adaptive_order, extrapolated_guess, push_history
+    nu, F, nP1 = 6, 3, 4
+    n = nu + F * nP1
+    rng = torch.Generator().manual_seed(3)
+    a, b, c = (torch.randn(n, generator=rng, dtype=torch.float64) for _ in range(3))
+
+    def quad(t):
+        return a + b * t + c * t * t
+    hist, tmp, x = [], torch.empty(n, dtype=torch.float64), torch.empty(n, dtype=torch.float64)
+    for t in range(8):
+        push_history(hist, 7, quad(float(t)))
+    cur = quad(8.0)
+    k = adaptive_order(hist, 6, cur, nu, F, nP1, tmp)
+    extrapolated_guess(hist[1:], k, hist[0], x)
+    assert k >= 2 and torch.allclose(x, cur, rtol=1e-10, atol=1e-10)
+
+    def kinked(t):
+        return a * max(t - 5.0, 0.0) + b      # flat, then linear from t = 5
+    hist = []
+    for t in range(7):
+        push_history(hist, 7, kinked(float(t)))
+    assert adaptive_order(hist, 6, kinked(7.0), nu, F, nP1, tmp) == 1

@@ -164,6 +164,34 @@ def extrapolated_guess(hist, order, up_prev, x):
             x.add_(hist[j - 1], alpha=float((-1) ** j * math.comb(k + 1, j + 1)))
 
 
+def adaptive_order(hist, order, up_prev, nu, F, nP1, tmp):
+    """The degree k in 1 .. min(order, len(hist) - 1) whose extrapolation
+    would have predicted the newest state up_prev best from the states
+    before it (max over the fields - u and the F pressures - of the relative
+    L2 error); min(order, len(hist)) when the history is too short to
+    compare.  tmp: a scratch vector like up_prev."""
+    kmax = min(order, len(hist) - 1)
+    if kmax < 1:
+        return min(order, len(hist))
+    scale = _field_norms(up_prev, nu, F, nP1)
+    best, best_err = kmax, None
+    for k in range(1, kmax + 1):
+        extrapolated_guess(hist[1:], k, hist[0], tmp)
+        tmp.sub_(up_prev)
+        err = (_field_norms(tmp, nu, F, nP1) / scale.clamp_min(1e-300)).max()
+        if best_err is None or bool(err < best_err):
+            best, best_err = k, err
+    return best
+
+
+def _field_norms(x, nu, F, nP1):
+    """L2 norms of the u part and of each of the F node-interleaved pressure
+    fields of x (entries past them, multipliers, are left out)."""
+    pu = torch.linalg.vector_norm(x[:nu]).reshape(1)
+    pp = torch.linalg.vector_norm(x[nu:nu + F * nP1].view(nP1, F), dim=0)
+    return torch.cat([pu, pp])
+
+
 def push_history(hist, order, up_prev):
     """Keep up_prev as the newest of at most `order` previous states."""
     if order < 1:
@@ -212,11 +240,16 @@ class StepEngine:
     batch_system)."""
 
     def __init__(self, problem, space, dt, theta, config: SolverConfig, device="cuda",
-                 extrapolate=False, batched=False, extrapolate_order=4, system=None):
+                 extrapolate=False, batched=False, extrapolate_order=4, system=None,
+                 extrapolate_adaptive=False):
         self.problem = problem
         self.batched = bool(batched)
         self.extrapolate = bool(extrapolate)
         self.ex_order = int(extrapolate_order)
+        # adaptive: each step extrapolates with the degree (up to ex_order)
+        # that best predicted the step just taken (one state more kept)
+        self.ex_adaptive = bool(extrapolate_adaptive)
+        self.ex_used = []
         if not 1 <= self.ex_order <= 6:
             raise ValueError("extrapolate_order must be 1 .. 6")
         # order 4 (quartic through five states): the 48^3 benchmark's
@@ -376,11 +409,17 @@ class StepEngine:
         if self.extrapolate:
             self._valid_history(float(t_n))
         if not testing_guess:
-            extrapolated_guess(self.hist if self.extrapolate else [], self.ex_order, up_prev, x)
+            order = self.ex_order
+            if self.extrapolate and self.ex_adaptive:
+                order = adaptive_order(self.hist, self.ex_order, up_prev, self.space.nu,
+                                       self.space.A + 1, self.space.dofmap.nP1, x)
+                self.ex_used.append(order)
+            extrapolated_guess(self.hist if self.extrapolate else [], order, up_prev, x)
         if self.extrapolate:
-            push_history(self.hist, self.ex_order, up_prev)
+            keep = self.ex_order + (1 if self.ex_adaptive else 0)
+            push_history(self.hist, keep, up_prev)
             self.hist_t.insert(0, float(t_n))
-            del self.hist_t[self.ex_order:]
+            del self.hist_t[keep:]
         if self.bc_dofs.numel():
             x.index_copy_(0, self.bc_dofs, g)
         st = self.krylov.solve(self.b, x)
@@ -444,6 +483,9 @@ class MPETTotalPressureSolver(object):
         params.add("p0_weight", "robust")
         params.add("extrapolate", None)
         params.add("extrapolate_order", 4)
+        # pick the degree (up to extrapolate_order) per step: the one that
+        # best predicted the previous step
+        params.add("extrapolate_adaptive", False)
         params.add("u_smoother_degree", 3)
         params.add("p_smoother_degree", 2)
         params.add("u_level_degrees", (1,))
@@ -524,7 +566,8 @@ class MPETTotalPressureSolver(object):
         smoother settings, dt, theta and the initial-guess policy."""
         p = self.params
         keys = ("precision", "krylov_rtol", "outer_tol", "inner_rtol", "maxit", "p0_weight",
-                "extrapolate", "extrapolate_order", "u_smoother_degree", "p_smoother_degree",
+                "extrapolate", "extrapolate_order", "extrapolate_adaptive", "u_smoother_degree",
+                "p_smoother_degree",
                 "u_level_degrees", "p_level_degrees", "u_max_coarse", "fp64_operator",
                 "fp32_operator",
                 "check_every", "use_graphs", "autotune", "dt", "theta", "device")
@@ -549,7 +592,8 @@ class MPETTotalPressureSolver(object):
             self._engine = StepEngine(self.problem, self.up_.function_space(),
                                       self.params["dt"], self.params["theta"], cfg,
                                       self.params["device"], extrapolate=ex,
-                                      extrapolate_order=int(self.params["extrapolate_order"]))
+                                      extrapolate_order=int(self.params["extrapolate_order"]),
+                                      extrapolate_adaptive=bool(self.params["extrapolate_adaptive"]))
             self._engine_key_built = key
         return self._engine
 
