@@ -348,6 +348,35 @@ def assemble_matrix(coords, cells, cell_p2, cell_p1, nP2, nP1, params,
     return M
 
 
+def rhs_matrix(coords, cells, cell_p1, nP2, nP1, params, dt, theta, K_cells=None):
+    """Matrix of the up_-linear part of b, rhs of F at t_n
+    (mpettotalpressuresolver.py:270-274, the rows w_i): for network
+    rows f >= 1, Bm[f, g] M (the storage and exchange coefficients of
+    pressure_coefficients) plus dt (1 - theta) K_f S on the diagonal block.
+    The u and p_0 rows are zero.  Passing a subset of the cells gives the
+    contributions of those cells only (rows whose cells all lie in the
+    subset are then exact)."""
+    A = int(params["J"])
+    lay = Layout(nP2, nP1, A)
+    _, lmbda = lame(params["E"], params["nu"])
+    _, Bm = pressure_coefficients(params["alpha"], params["c"], params["S"], lmbda, dt, theta)
+    et = element_tensors(coords, cells)
+    nC = cells.shape[0]
+    K = K_cells if K_cells is not None else np.tile(np.asarray(params["K"], float), (nC, 1))
+    rows, cols, vals = [], [], []
+    for f in range(1, A + 1):
+        for g in range(A + 1):
+            v = Bm[f, g] * et["M"]
+            if f == g:
+                v = v + dt * (1 - theta) * K[:, f - 1, None, None] * et["S"]
+            R, Cc = _blk(lay.pdofs(cell_p1, f), lay.pdofs(cell_p1, g), v)
+            rows.append(R.reshape(-1))
+            cols.append(Cc.reshape(-1))
+            vals.append(v.reshape(-1))
+    return sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(lay.n, lay.n)).tocsr()
+
+
 # ---------------------------------------------------------------------------
 # two-field formulation (MPETSolver)
 # ---------------------------------------------------------------------------
@@ -1027,34 +1056,10 @@ class OracleProblem:
                                                self.K_cells, self.lay)
             return self._B_mat
         m = self.m
-        A = self.A
-        mu, lmbda = lame(self.params["E"], self.params["nu"])
-        _, Bm = pressure_coefficients(self.params["alpha"], self.params["c"],
-                                      self.params["S"], lmbda, self.dt, self.theta)
-        et = element_tensors(m["coords"], m["cells"])
-        nC = m["cells"].shape[0]
-        K = self.K_cells if self.K_cells is not None else np.tile(
-            np.asarray(self.params["K"], float), (nC, 1))
-        rows, cols, vals = [], [], []
-        for f in range(1, A + 1):
-            for g in range(A + 1):
-                v = Bm[f, g] * et["M"]
-                if f == g:
-                    v = v + self.dt * (1 - self.theta) * K[:, f - 1, None, None] * et["S"]
-                R, Cc = _blk(self.lay.pdofs(m["cell_p1"], f), self.lay.pdofs(m["cell_p1"], g), v)
-                rows.append(R.reshape(-1))
-                cols.append(Cc.reshape(-1))
-                vals.append(v.reshape(-1))
-        if self._robin_mass is not None:
-            for i, rb in enumerate(self._robin_mass):
-                if rb is None:
-                    continue
-                # beta(t_theta) is re-evaluated per step in rhs_sources
-                pass
-        B = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
-                          shape=(self.lay.n, self.lay.n)).tocsr()
-        self._B_mat = B
-        return B
+        # beta(t_theta) of a Robin term is re-evaluated per step in rhs()
+        self._B_mat = rhs_matrix(m["coords"], m["cells"], m["cell_p1"], m["nP2"], m["nP1"],
+                                 self.params, self.dt, self.theta, self.K_cells)
+        return self._B_mat
 
     def rhs(self, up_prev, t_n):
         """b assembled as in solve_iterative :461-483 (before BCs)."""

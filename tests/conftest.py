@@ -57,3 +57,36 @@ def lognormal_K(nC, medians, sigma=0.5, seed=0):
     rng = np.random.default_rng(seed)
     med = np.asarray(medians, float)
     return med[None, :] * np.exp(sigma * rng.standard_normal((nC, med.size)))
+
+
+def sampled_row_residuals(m, params, K_cells, dt, theta, x, x_prev, rows):
+    """Residual of the oracle's equations (A x)_i = (B x_prev)_i at the given
+    rows (rows with no Dirichlet condition; a workload with no sources),
+    with A and B assembled by the oracle from only the cells that touch
+    those rows: returns (r, scale), scale_i = sum_j |A_ij x_j| + sum_j
+    |B_ij x_prev_j|.  Lets a solution of any size be checked against the
+    oracle's own assembly at a sample of its rows."""
+    from oracle import mpet_oracle as O
+    rows = np.asarray(rows, dtype=np.int64)
+    lay = O.Layout(m["nP2"], m["nP1"], int(params["J"]))
+    n2 = np.unique(rows[rows < lay.nu] // 3)
+    n1 = np.unique((rows[rows >= lay.nu] - lay.nu) // lay.F)
+    sel = np.isin(m["cell_p2"], n2).any(1) | np.isin(m["cell_p1"], n1).any(1)
+    c = np.nonzero(sel)[0]
+    Kc = None if K_cells is None else np.asarray(K_cells)[c]
+    args = (m["coords"], m["cells"][c], m["cell_p2"][c], m["cell_p1"][c], m["nP2"], m["nP1"],
+            params, dt, theta, Kc)
+    A = O.assemble_matrix(*args, which="A")[rows]
+    B = O.rhs_matrix(m["coords"], m["cells"][c], m["cell_p1"][c], m["nP2"], m["nP1"], params,
+                     dt, theta, Kc)[rows]
+    r = A @ x - B @ x_prev
+    scale = abs(A) @ np.abs(x) + abs(B) @ np.abs(x_prev)
+    return r, scale
+
+
+def field_row_ratios(lay, rows, r, scale):
+    """||r|| / ||scale|| over the sampled rows of each field (u, p_0 .. p_A)."""
+    rows = np.asarray(rows)
+    fld = np.where(rows < lay.nu, 0, 1 + (rows - lay.nu) % lay.F)
+    return [float(np.linalg.norm(r[fld == k]) / np.linalg.norm(scale[fld == k]))
+            for k in range(lay.F + 1) if np.any(fld == k)]

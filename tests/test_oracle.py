@@ -384,3 +384,55 @@ def test_mms_convergence_rates_with_facet_terms():
     ru = rate(errs[0]["u"], errs[1]["u"])
     rp1, rp2 = rate(errs[0]["p1"], errs[1]["p1"]), rate(errs[0]["p2"], errs[1]["p2"])
     assert ru > 1.70 and rp1 > 1.70 and rp2 > 1.70, (ru, rp1, rp2, errs)
+
+
+def _bench_like(n, A, steps, scale_K=1.0, seed=7):
+    """bench.py's workload family at n^3 (u = 0 on z = 0, p_j = P0_j +
+    5 sin(2 pi t) on z = 1, log-normal per-cell K, adjacent exchange)."""
+    from conftest import cfg2_params, lognormal_K
+    mesh, space, m = _small_problem(n, A)
+    params = cfg2_params(A)
+    Kc = lognormal_K(mesh.num_cells(), params["K"], seed=seed) * scale_K
+    mid = mesh.facet_midpoints()
+    inv = np.uint64(2**63 - 1)
+    mk_u = np.full(mesh.num_facets(), inv, dtype=np.uint64)
+    mk_u[mesh.exterior & (np.abs(mid[:, 2]) < 1e-12)] = 0
+    mk_p = np.full(mesh.num_facets(), inv, dtype=np.uint64)
+    mk_p[mesh.exterior & (np.abs(mid[:, 2] - 1.0) < 1e-12)] = 0
+    P0 = (8.0, 4.0, 2.0, 1.0)
+    pb = [lambda X, t, j=j: np.full(X.shape[0], P0[j] + 5.0 * np.sin(2.0 * np.pi * t))
+          for j in range(A)]
+    prob = O.OracleProblem(m, params, dict(dt=0.0125, theta=1.0, T=steps * 0.0125),
+                           u_bar=lambda X, t: np.zeros((X.shape[0], 3)), p_bar=pb, K_cells=Kc,
+                           momentum_markers=mk_u, continuity_markers=[mk_p] * A)
+    return m, params, Kc, prob
+
+
+def test_sampled_row_residuals_pin_the_full_assembly():
+    """The oracle's equations at a sample of rows, assembled from only the
+    cells that touch them (the check the 48^3 GPU step runs at full size):
+    the rows equal the full assembly's, the oracle's direct solution
+    satisfies them to round-off, and a 1% error in K or in E shows."""
+    from conftest import sampled_row_residuals, field_row_ratios
+    A = 2
+    m, params, Kc, prob = _bench_like(4, A, 2)
+    lay = prob.lay
+    x0 = np.zeros(lay.n)
+    states = [x.copy() for x, t in prob.solve_direct(x0)]
+    x_prev, x = states[0], states[1]
+    bc = np.unique(np.concatenate(prob.bc_dofs))
+    free = np.setdiff1d(np.arange(lay.n), bc)
+    rng = np.random.default_rng(3)
+    rows = np.sort(rng.choice(free, 120, replace=False))
+    # restricted rows == full rows
+    r, scale = sampled_row_residuals(m, params, Kc, 0.0125, 1.0, x, x_prev, rows)
+    full = prob.A_mat[rows] @ x - prob.B_matrix()[rows] @ x_prev
+    assert np.allclose(r, full, rtol=0, atol=1e-13 * np.max(scale))
+    ratios = field_row_ratios(lay, rows, r, scale)
+    assert len(ratios) == A + 2 and max(ratios) <= 1e-12, ratios
+    # planted slips in the assembly of the check: 1% in K, 1% in E
+    r1, s1 = sampled_row_residuals(m, params, Kc * 1.01, 0.0125, 1.0, x, x_prev, rows)
+    assert max(field_row_ratios(lay, rows, r1, s1)[2:]) >= 1e-6
+    pe = dict(params, E=params["E"] * 1.01)
+    r2, s2 = sampled_row_residuals(m, pe, Kc, 0.0125, 1.0, x, x_prev, rows)
+    assert field_row_ratios(lay, rows, r2, s2)[0] >= 1e-4
